@@ -1,0 +1,197 @@
+/*
+ * sifv_gpu.h - C ABI of the MI355X (gfx950) SemiImplicitFV hot path.
+ *
+ * Plain C linkage, plain pointers and sizes: no C++ or torch types cross
+ * this boundary.  Every entry point returns an int status (SIFV_OK == 0);
+ * sifv_last_error() returns a message for the most recent failure on the
+ * calling thread.
+ *
+ * Field arrays use the padded, x-fastest layout of the reference's
+ * RectilinearMesh::index (include/RectilinearMesh.hpp:70-73):
+ *     idx = (i + ngx) + nxT * ((j + ngy) + nyT * (k + ngz))
+ * with ghosts only in active dimensions, so one upload/download is one
+ * contiguous copy of totalCells doubles per field.
+ *
+ * One handle owns all device memory of one solver instance and one HIP
+ * stream.  Calls on one handle are not thread safe; distinct handles may be
+ * driven from distinct host threads.
+ */
+#ifndef SIFV_GPU_H
+#define SIFV_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIFV_ABI_VERSION 1
+#define SIFV_MAX_PHASES 8
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+    SIFV_OK = 0,
+    SIFV_ERR_INVALID = 1,      /* std::invalid_argument in the reference */
+    SIFV_ERR_HIP = 2,          /* HIP runtime failure */
+    SIFV_ERR_NONFINITE = 3,    /* NaN/Inf detected in the state */
+    SIFV_ERR_UNSUPPORTED = 4,  /* configuration outside the GPU path */
+    SIFV_ERR_NOMEM = 5
+};
+
+/* ---- enums (numeric values follow the reference's declaration order) -- */
+/* ReconstructionOrder, include/SimulationConfig.hpp:13-20 */
+enum { SIFV_WENO1 = 0, SIFV_WENO3 = 1, SIFV_WENO5 = 2,
+       SIFV_UPWIND1 = 3, SIFV_UPWIND3 = 4, SIFV_UPWIND5 = 5 };
+/* RiemannSolverType, include/RiemannSolver.hpp:16 */
+enum { SIFV_LF = 0, SIFV_RUSANOV = 1, SIFV_HLLC = 2 };
+/* BoundaryCondition, include/RectilinearMesh.hpp:14-20 */
+enum { SIFV_BC_SYMMETRY = 0, SIFV_BC_OUTFLOW = 1, SIFV_BC_PERIODIC = 2,
+       SIFV_BC_SLIPWALL = 3, SIFV_BC_NOSLIPWALL = 4 };
+/* VarSet, include/SolutionState.hpp:15-19 */
+enum { SIFV_VARSET_CONS = 0, SIFV_VARSET_PRIM = 1, SIFV_VARSET_ALL = 2 };
+/* Face ids, include/RectilinearMesh.hpp:110-115 */
+enum { SIFV_XLOW = 0, SIFV_XHIGH = 1, SIFV_YLOW = 2, SIFV_YHIGH = 3,
+       SIFV_ZLOW = 4, SIFV_ZHIGH = 5 };
+/* EOS kinds: IdealGasEOS (src/IdealGasEOS.cpp), StiffenedGasEOS
+ * (src/StiffenedGasEOS.cpp) */
+enum { SIFV_EOS_IDEAL = 0, SIFV_EOS_STIFFENED = 1 };
+/* Arithmetic of the bulk WENO/flux work.  SIFV_FP64 is the plain method;
+ * SIFV_MIXED stores the state and runs WENO/flux arithmetic in FP32 while
+ * the EOS energy<->pressure conversion, dt and conservation sums stay FP64. */
+enum { SIFV_FP64 = 0, SIFV_MIXED = 1 };
+
+/* Field ids for upload/download (SolutionState members,
+ * include/SolutionState.hpp:73-113). */
+enum {
+    SIFV_F_RHO = 0, SIFV_F_RHOU = 1, SIFV_F_RHOV = 2, SIFV_F_RHOW = 3,
+    SIFV_F_RHOE = 4, SIFV_F_VELU = 5, SIFV_F_VELV = 6, SIFV_F_VELW = 7,
+    SIFV_F_PRES = 8, SIFV_F_SIGMA = 9,
+    SIFV_F_ALPHARHO = 16, /* + phase, phase < SIFV_MAX_PHASES */
+    SIFV_F_ALPHA = 24     /* + phase */
+};
+
+/* ---- configuration mirror -------------------------------------------- */
+typedef struct sifv_phase_eos { double gamma, pInf; } sifv_phase_eos;
+
+/* Mirror of SimulationConfig and its *Params structs
+ * (include/SimulationConfig.hpp:22-91).  Body force, viscosity and surface
+ * tension are outside the GPU path and must be zero. */
+typedef struct sifv_config {
+    int dim, nGhost, RKOrder, useIGR, semiImplicit, reconOrder;
+    double wenoEps;
+    int step;
+    double time;
+    struct { double cfl, constDt, maxDt, minDt; } explicitParams;
+    struct { double cfl, maxDt, minDt; int maxPressureIters; double pressureTol; } semiImplicitParams;
+    struct { double alphaCoeff; int IGRIters, IGRWarmStartIters; } igrParams;
+    struct { int nPhases; sifv_phase_eos phases[SIFV_MAX_PHASES]; double alphaMin; } multiPhaseParams;
+    struct { double a[3], b[3], c[3], d[3]; } bodyForceParams;
+    double viscousMu;
+    double surfaceTensionSigma;
+} sifv_config;
+
+/* Mirror of the RectilinearMesh constructor arguments plus the boundary
+ * setup done through Runtime::setBoundaryCondition / createUniformMesh
+ * periods (src/Runtime.cpp:39-127, src/RectilinearMesh.cpp:14-56).
+ * yNodes/zNodes may be NULL for inactive dimensions ({0,1} is used). */
+typedef struct sifv_mesh {
+    int dim, nx, ny, nz, nGhost;
+    const double* xNodes;   /* nx+1 physical node coordinates */
+    const double* yNodes;   /* ny+1 */
+    const double* zNodes;   /* nz+1 */
+    int bc[6];              /* SIFV_BC_* per face */
+    int periods[3];         /* MPI-style periodicity flags (wrap on one device) */
+} sifv_mesh;
+
+typedef struct sifv_eos { int kind; double gamma, pInf; } sifv_eos;
+
+typedef struct sifv_handle sifv_handle;
+
+/* ---- life cycle ------------------------------------------------------- */
+/* Replaces the ExplicitSolver / SemiImplicitSolver constructors
+ * (include/ExplicitSolver.hpp:23-29, include/SemiImplicitSolver.hpp:24-31,
+ * src/ExplicitSolver.cpp:16-66): riemann is the enum the reference derives
+ * from RiemannSolver::name() (src/ExplicitSolver.cpp:31-34).  Validates the
+ * config as SimulationConfig::validate (include/SimulationConfig.hpp:123-209). */
+int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann,
+                const sifv_eos* eos, int precision, sifv_handle** out);
+int sifv_destroy(sifv_handle* h);
+const char* sifv_last_error(void);
+int sifv_abi_version(void);
+
+/* Number of padded cells (RectilinearMesh::totalCells,
+ * src/RectilinearMesh.cpp:83-85). */
+int sifv_total_cells(const sifv_handle* h, size_t* n);
+
+/* ---- state transfer (SolutionState member vectors) -------------------- */
+int sifv_upload(sifv_handle* h, int field, const double* host, size_t n);
+int sifv_download(sifv_handle* h, int field, double* host, size_t n);
+
+/* ---- explicit SSP-RK step --------------------------------------------- */
+/* Replaces ExplicitSolver::step (src/ExplicitSolver.cpp:68-203): acoustic
+ * dt (or constDt), dt = min(dt, targetDt) when targetDt > 0, RKOrder stages
+ * of cons->prim, ghost fill, [IGR], RHS and stage update, then the final
+ * cons->prim and ghost fill.  Blocks until dt is known; *dt_out = dt. */
+int sifv_explicit_step(sifv_handle* h, const sifv_config* cfg, double targetDt,
+                       double* dt_out);
+
+/* Runs up to nsteps explicit steps with device-resident dt and time, the
+ * way runTimeLoop does (src/RKTimeStepping.cpp:259-274) with targetDt =
+ * endTime - time and no output in between.  Stops early when time reaches
+ * endTime.  *time_inout is advanced, *steps_done counts the steps taken.
+ * One host synchronisation at the end. */
+int sifv_explicit_advance(sifv_handle* h, const sifv_config* cfg, int nsteps,
+                          double endTime, double* time_inout, int* steps_done);
+
+/* ---- semi-implicit step ------------------------------------------------ */
+/* Replaces SemiImplicitSolver::step (src/SemiImplicitSolver.cpp:106-236)
+ * with the Jacobi PressureSolver; *pressure_iters is
+ * lastPressureIterations() of the last stage. */
+int sifv_semi_implicit_step(sifv_handle* h, const sifv_config* cfg, double targetDt,
+                            double* dt_out, int* pressure_iters);
+
+/* ---- pressure solver --------------------------------------------------- */
+/* Replaces JacobiPressureSolver::solve (src/JacobiPressureSolver.cpp:9-55):
+ * host arrays of totalCells doubles in, pressure updated in place (interior
+ * cells; ghosts as left by the last sweep's ghost fill).  Returns the
+ * iteration count through *iters (maxIter when not converged, as the
+ * reference does).  residual_history may be NULL or hold maxIter doubles. */
+int sifv_jacobi_solve(sifv_handle* h, const double* rho, const double* rhoc2,
+                      const double* rhs, double* pressure, double dt,
+                      double tolerance, int maxIter, int* iters,
+                      double* residual_history);
+
+/* ---- ghost fill and time step ------------------------------------------ */
+/* Replaces RectilinearMesh::applyBoundaryConditions(state, varSet, halo)
+ * (src/RectilinearMesh.cpp:498-524) on the device-resident state. */
+int sifv_fill_ghosts(sifv_handle* h, int varset);
+/* Replaces RectilinearMesh::fillScalarGhosts(field, halo)
+ * (src/RectilinearMesh.cpp:526-624) for one device-resident field. */
+int sifv_fill_scalar_ghosts(sifv_handle* h, int field);
+/* Replaces computeAcousticTimeStep(mesh, state, eos, config, cfl, maxDt, comm)
+ * (src/RKTimeStepping.cpp:43-159) on the device-resident primitive fields. */
+int sifv_acoustic_dt(sifv_handle* h, double cfl, double maxDt, double* dt);
+/* Replaces computeAdvectiveTimeStep (src/RKTimeStepping.cpp:15-41,125-134). */
+int sifv_advective_dt(sifv_handle* h, double cfl, double maxDt, double* dt);
+/* Recomputes the primitive fields from the conservative ones on interior
+ * cells and fills ghosts (SolutionState::convertConservativeToPrimitiveVariables,
+ * src/SolutionState.cpp:187-205, or MixtureEOS::convertConservativeToPrimitive,
+ * src/MixtureEOS.cpp:99-143, followed by the PRIM ghost fill). */
+int sifv_cons_to_prim(sifv_handle* h);
+
+/* ---- measurement ---------------------------------------------------------- */
+/* When enabled, the library records HIP events around every launch of the
+ * fused RHS+stage kernel on its own stream.  sifv_timing_read returns the
+ * summed kernel time and the launch count since the last reset. */
+int sifv_timing_enable(sifv_handle* h, int on);
+int sifv_timing_read(sifv_handle* h, double* kernel_ms, long* launches, int reset);
+int sifv_synchronize(sifv_handle* h);
+/* Launch-shape override for the fused stage kernel (rows marched per wave);
+ * 0 restores the built-in choice.  Used by the autotuner. */
+int sifv_set_tile(sifv_handle* h, int rows_per_wave);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIFV_GPU_H */

@@ -1,0 +1,890 @@
+// sifv_api.hip - C ABI of the MI355X hot path (include/sifv_gpu.h).
+//
+// A handle owns the device-resident state of one solver instance:
+//   three conservative buffers (stage ping-pong; RK stage 3 writes back into
+//   the buffer holding U^n, which is read only pointwise in that stage),
+//   FP64 primitive arrays (rho, vel, pres, sigma) mirroring SolutionState's
+//   primitive members, and small scratch (dt partials, device scalars).
+// All work is enqueued on the handle's own HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sifv_gpu.h"
+#include "sifv_aux.hpp"
+#include "sifv_stage.hpp"
+
+using namespace sifv;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(SIFV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// buildExtendedNodes, src/RectilinearMesh.cpp:91-114
+std::vector<double> extended_nodes(const double* phys, int n, int ng) {
+    std::vector<double> e(n + 2 * ng + 1);
+    for (int i = 0; i <= n; ++i) e[ng + i] = phys[i];
+    for (int g = 1; g <= ng; ++g) {
+        int m = std::min(g - 1, n - 1);
+        e[ng - g] = e[ng - g + 1] - (phys[m + 1] - phys[m]);
+    }
+    for (int g = 1; g <= ng; ++g) {
+        int m = std::max(n - g, 0);
+        e[ng + n + g] = e[ng + n + g - 1] + (phys[m + 1] - phys[m]);
+    }
+    return e;
+}
+
+int required_ghosts(int order) {
+    switch (order) {
+        case SIFV_WENO3:
+        case SIFV_UPWIND3: return 2;
+        case SIFV_WENO5:
+        case SIFV_UPWIND5: return 3;
+        default: return 1;
+    }
+}
+
+// SimulationConfig::validate, include/SimulationConfig.hpp:123-209
+int validate_config(const sifv_config* c) {
+    if (c->dim < 1 || c->dim > 3) return fail(SIFV_ERR_INVALID, "dim must be 1, 2, or 3 (got " + std::to_string(c->dim) + ")");
+    if (c->RKOrder < 1 || c->RKOrder > 3) return fail(SIFV_ERR_INVALID, "RKOrder must be 1, 2, or 3 (got " + std::to_string(c->RKOrder) + ")");
+    if (c->reconOrder < 0 || c->reconOrder > 5) return fail(SIFV_ERR_INVALID, "unknown reconOrder");
+    int req = required_ghosts(c->reconOrder);
+    if (c->nGhost < req)
+        return fail(SIFV_ERR_INVALID, "nGhost=" + std::to_string(c->nGhost) +
+                                          " is too small for chosen reconOrder (need >= " + std::to_string(req) + ")");
+    if (c->semiImplicit) {
+        const auto& p = c->semiImplicitParams;
+        if (p.cfl <= 0) return fail(SIFV_ERR_INVALID, "semiImplicitParams.cfl must be > 0");
+        if (p.minDt <= 0) return fail(SIFV_ERR_INVALID, "semiImplicitParams.minDt must be > 0");
+        if (p.maxDt <= p.minDt) return fail(SIFV_ERR_INVALID, "semiImplicitParams.maxDt must be > minDt");
+        if (p.pressureTol <= 0) return fail(SIFV_ERR_INVALID, "semiImplicitParams.pressureTol must be > 0");
+        if (p.maxPressureIters <= 0) return fail(SIFV_ERR_INVALID, "semiImplicitParams.maxPressureIters must be > 0");
+    } else {
+        const auto& p = c->explicitParams;
+        if (p.cfl <= 0) return fail(SIFV_ERR_INVALID, "explicitParams.cfl must be > 0");
+        if (p.minDt <= 0) return fail(SIFV_ERR_INVALID, "explicitParams.minDt must be > 0");
+        if (p.maxDt <= p.minDt) return fail(SIFV_ERR_INVALID, "explicitParams.maxDt must be > minDt");
+    }
+    if (c->useIGR) {
+        const auto& p = c->igrParams;
+        if (p.alphaCoeff <= 0) return fail(SIFV_ERR_INVALID, "igrParams.alphaCoeff must be > 0");
+        if (p.IGRIters <= 0) return fail(SIFV_ERR_INVALID, "igrParams.IGRIters must be > 0");
+        if (p.IGRWarmStartIters < 0) return fail(SIFV_ERR_INVALID, "igrParams.IGRWarmStartIters must be >= 0");
+    }
+    const auto& mp = c->multiPhaseParams;
+    if (mp.nPhases < 0 || mp.nPhases > SIFV_MAX_PHASES) return fail(SIFV_ERR_INVALID, "multiPhaseParams.nPhases out of range");
+    if (mp.nPhases > 0) {
+        for (int ph = 0; ph < mp.nPhases; ++ph) {
+            if (mp.phases[ph].gamma <= 1.0)
+                return fail(SIFV_ERR_INVALID, "multiPhaseParams.phases[" + std::to_string(ph) + "].gamma must be > 1");
+            if (mp.phases[ph].pInf < 0.0)
+                return fail(SIFV_ERR_INVALID, "multiPhaseParams.phases[" + std::to_string(ph) + "].pInf must be >= 0");
+        }
+        if (mp.alphaMin <= 0.0) return fail(SIFV_ERR_INVALID, "multiPhaseParams.alphaMin must be > 0");
+    }
+    if (c->viscousMu < 0.0) return fail(SIFV_ERR_INVALID, "viscousParams.mu must be >= 0");
+    if (c->surfaceTensionSigma < 0.0) return fail(SIFV_ERR_INVALID, "surfaceTensionParams.sigma must be >= 0");
+    // terms outside the GPU path
+    if (c->viscousMu > 0.0) return fail(SIFV_ERR_UNSUPPORTED, "viscous stress is not on the GPU path");
+    if (c->surfaceTensionSigma > 0.0) return fail(SIFV_ERR_UNSUPPORTED, "surface tension is not on the GPU path");
+    for (int d = 0; d < 3; ++d)
+        if (c->bodyForceParams.a[d] != 0.0 || c->bodyForceParams.b[d] != 0.0)
+            return fail(SIFV_ERR_UNSUPPORTED, "body force is not on the GPU path");
+    return SIFV_OK;
+}
+
+struct Timing {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double ms = 0.0;
+    long launches = 0;
+};
+
+}  // namespace
+
+struct sifv_handle {
+    int dim, nx, ny, nz, ng, ngx, ngy, ngz, nxT, nyT, nzT;
+    size_t total;
+    int bc[6], sbc[6];
+    sifv_config cfg;
+    int riemann;
+    sifv_eos eos;
+    int precision;
+    int nph, nc;
+    hipStream_t stream = nullptr;
+    double *d_dx = nullptr, *d_dy = nullptr, *d_dz = nullptr;
+    void* cons[3] = {nullptr, nullptr, nullptr};
+    int cur = 0;
+    double* rhoP = nullptr;   // SolutionState::rho as seen by the primitive side
+    double* vel[3] = {nullptr, nullptr, nullptr};
+    double* pres = nullptr;
+    double* sigma = nullptr;
+    double* partial = nullptr;
+    int npartialCap = 0;
+    int npartialValid = 0;
+    double* scal = nullptr;   // [0] dt, [1] time
+    int* flags = nullptr;     // [0] active, [1] steps
+    double* hostPinned = nullptr;
+    bool dtNextValid = false;
+    bool primFresh = true;
+    int rowsOverride = 0;
+    Timing timing;
+    // Jacobi / semi-implicit scratch (allocated on first use)
+    double* jac[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double* resid = nullptr;
+    int residCap = 0;
+
+    size_t tsize() const { return precision == SIFV_MIXED ? sizeof(float) : sizeof(double); }
+    Grid grid() const {
+        Grid g;
+        g.dim = dim;
+        g.nx = nx; g.ny = ny; g.nz = nz;
+        g.ngx = ngx; g.ngy = ngy; g.ngz = ngz;
+        g.nxT = nxT; g.nyT = nyT; g.nzT = nzT;
+        for (int f = 0; f < 6; ++f) { g.bc[f] = bc[f]; g.sbc[f] = sbc[f]; }
+        g.dxw = d_dx; g.dyw = d_dy; g.dzw = d_dz;
+        return g;
+    }
+    Eos eosd() const {
+        Eos e;
+        e.gamma = eos.gamma;
+        e.pInf = eos.kind == SIFV_EOS_STIFFENED ? eos.pInf : 0.0;
+        e.stiffened = eos.kind == SIFV_EOS_STIFFENED;
+        return e;
+    }
+    Mix mix() const {
+        Mix m;
+        m.nph = nph;
+        for (int p = 0; p < kMaxPhases; ++p) {
+            m.gamma[p] = p < nph ? cfg.multiPhaseParams.phases[p].gamma : 1.4;
+            m.pInf[p] = p < nph ? cfg.multiPhaseParams.phases[p].pInf : 0.0;
+        }
+        m.alphaMin = cfg.multiPhaseParams.alphaMin;
+        return m;
+    }
+    template <typename T>
+    T* consField(int buf, int q) const { return static_cast<T*>(cons[buf]) + (size_t)q * total; }
+};
+
+namespace {
+
+int ensure_partials(sifv_handle* h, int n) {
+    if (n <= h->npartialCap) return SIFV_OK;
+    if (h->partial) HIPCHK(hipFree(h->partial));
+    h->partial = nullptr;
+    HIPCHK(hipMalloc(&h->partial, sizeof(double) * n));
+    h->npartialCap = n;
+    return SIFV_OK;
+}
+
+// ---- stage launch ---------------------------------------------------------
+template <typename T>
+using StageKernel = void (*)(StageParams<T>);
+
+template <typename T>
+StageKernel<T> pick_stage(int dim, int R, int rs) {
+#define SIFV_K(D, RR, S) \
+    if (dim == D && R == RR && rs == S) return &stage_xy_kernel<D, RR, S, T>;
+    SIFV_K(1, 1, 0) SIFV_K(1, 1, 1) SIFV_K(1, 1, 2)
+    SIFV_K(1, 2, 0) SIFV_K(1, 2, 1) SIFV_K(1, 2, 2)
+    SIFV_K(1, 3, 0) SIFV_K(1, 3, 1) SIFV_K(1, 3, 2)
+    SIFV_K(2, 1, 0) SIFV_K(2, 1, 1) SIFV_K(2, 1, 2)
+    SIFV_K(2, 2, 0) SIFV_K(2, 2, 1) SIFV_K(2, 2, 2)
+    SIFV_K(2, 3, 0) SIFV_K(2, 3, 1) SIFV_K(2, 3, 2)
+#undef SIFV_K
+    return nullptr;
+}
+
+int recon_radius(int order) { return required_ghosts(order); }
+
+void stage_shape(const sifv_handle* h, int& nstrip, int& nseg, int& rows) {
+    nstrip = (h->nx + kOwn - 1) / kOwn;
+    if (h->dim == 1) {
+        nseg = 1;
+        rows = 1;
+        return;
+    }
+    if (h->rowsOverride > 0) {
+        rows = h->rowsOverride;
+    } else {
+        // aim for >= ~8k waves so every CU holds several waves, but keep at
+        // least 8 rows per wave to amortise the 2-row warm-up
+        long target = 8192;
+        long r = (long)h->ny * nstrip / target;
+        rows = 8;
+        while (rows < 64 && rows * 2 <= r) rows *= 2;
+    }
+    rows = std::min(rows, h->ny);
+    nseg = (h->ny + rows - 1) / rows;
+}
+
+template <typename T>
+int launch_stage(sifv_handle* h, int src, int u0, int dst, const double* coef, bool rk1, bool last,
+                 const int* active) {
+    StageParams<T> P;
+    P.g = h->grid();
+    P.eos = h->eosd();
+    for (int q = 0; q < 5; ++q) {
+        int qq = q < h->nc ? q : 0;
+        P.src[q] = h->consField<T>(src, qq);
+        P.u0[q] = h->consField<T>(u0 >= 0 ? u0 : src, qq);
+        P.dst[q] = h->consField<T>(dst, qq);
+    }
+    P.sigma = h->sigma;
+    P.c1 = coef[0]; P.c2 = coef[1]; P.c3 = coef[2]; P.c4 = coef[3];
+    P.rk1 = rk1 ? 1 : 0;
+    P.dt = h->scal;
+    P.active = active;
+    P.eps = h->cfg.wenoEps;
+    P.incP = h->cfg.semiImplicit ? 0 : 1;
+    P.igr = h->cfg.useIGR ? 1 : 0;
+    P.weno = (h->cfg.reconOrder <= SIFV_WENO5) ? 1 : 0;
+    int nstrip, nseg, rows;
+    stage_shape(h, nstrip, nseg, rows);
+    P.rows = rows;
+    P.nseg = nseg;
+    int nb = nstrip * nseg;
+    bool wantDt = last && h->cfg.explicitParams.constDt <= 0.0;
+    if (wantDt) {
+        int rc = ensure_partials(h, nb);
+        if (rc) return rc;
+        P.dtPartial = h->partial;
+    } else {
+        P.dtPartial = nullptr;
+    }
+    StageKernel<T> k = pick_stage<T>(h->dim, recon_radius(h->cfg.reconOrder), h->riemann);
+    if (!k) return fail(SIFV_ERR_UNSUPPORTED, "no stage kernel for this configuration");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (h->timing.on) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, h->stream));
+    }
+    hipLaunchKernelGGL(k, dim3(nb), dim3(kWave), 0, h->stream, P);
+    HIPCHK(hipGetLastError());
+    if (h->timing.on) {
+        HIPCHK(hipEventRecord(e1, h->stream));
+        h->timing.pending.push_back({e0, e1});
+    }
+    if (wantDt) h->npartialValid = nb;
+    return SIFV_OK;
+}
+
+template <typename T>
+int launch_stage3d(sifv_handle* h, int src, int u0, int dst, const double* coef, bool rk1, bool last,
+                   const int* active);
+
+int harvest_timing(sifv_handle* h) {
+    for (auto& pr : h->timing.pending) {
+        float ms = 0.0f;
+        HIPCHK(hipEventSynchronize(pr.second));
+        HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+        h->timing.ms += ms;
+        h->timing.launches += 1;
+        HIPCHK(hipEventDestroy(pr.first));
+        HIPCHK(hipEventDestroy(pr.second));
+    }
+    h->timing.pending.clear();
+    return SIFV_OK;
+}
+
+const double kRK[3][3][4] = {
+    {{1, 0, 1, 1}, {0, 0, 0, 0}, {0, 0, 0, 0}},
+    {{1, 0, 1, 1}, {1, 1, 1, 2}, {0, 0, 0, 0}},
+    {{1, 0, 1, 1}, {1, 3, 1, 4}, {2, 1, 2, 3}},
+};
+
+int launch_dt_cells(sifv_handle* h, bool advective) {
+    DtArgs A;
+    A.g = h->grid();
+    A.eos = h->eosd();
+    A.mix = h->mix();
+    A.rho = h->rhoP;
+    for (int d = 0; d < 3; ++d) A.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
+    A.pres = h->pres;
+    for (int p = 0; p < kMaxPhases; ++p) {
+        int pp = p < h->nph ? p : 0;
+        A.alphaRho[p] = h->nph ? h->consField<double>(h->cur, h->dim + 2 + pp) : nullptr;
+        A.alpha[p] = h->nph ? h->consField<double>(h->cur, h->dim + 2 + h->nph + pp) : nullptr;
+    }
+    A.advective = advective ? 1 : 0;
+    size_t n = (size_t)h->nx * h->ny * h->nz;
+    int nb = (int)std::min<size_t>((n + kRedThreads - 1) / kRedThreads, 2048);
+    int rc = ensure_partials(h, nb);
+    if (rc) return rc;
+    A.partial = h->partial;
+    hipLaunchKernelGGL(dt_cells_kernel, dim3(nb), dim3(kRedThreads), 0, h->stream, A);
+    HIPCHK(hipGetLastError());
+    h->npartialValid = nb;
+    return SIFV_OK;
+}
+
+int launch_finalize(sifv_handle* h, double cfl, double maxDt, double minDt, double constDt,
+                    double target, bool clampMin, bool loop, double endTime) {
+    DtFinalize F;
+    F.partial = h->partial;
+    F.npartial = h->npartialValid;
+    F.cfl = cfl;
+    F.maxDt = maxDt;
+    F.minDt = minDt;
+    F.constDt = constDt;
+    F.target = target;
+    F.clampMin = clampMin ? 1 : 0;
+    F.loop = loop ? 1 : 0;
+    F.endTime = endTime;
+    F.scal = h->scal;
+    F.flags = h->flags;
+    hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(kRedThreads), 0, h->stream, F);
+    HIPCHK(hipGetLastError());
+    return SIFV_OK;
+}
+
+// Prim refresh after a step: cons->prim on interior cells plus the PRIM
+// ghost fill (the finalize block of ExplicitSolver::step, src/ExplicitSolver.cpp:192-200).
+int fill_dirs(sifv_handle* h, double** f, const int* comp, int nf, bool scalar);
+
+template <typename T>
+int launch_c2p(sifv_handle* h) {
+    C2PArgs<T> A;
+    A.g = h->grid();
+    A.eos = h->eosd();
+    A.mix = h->mix();
+    for (int q = 0; q < 5 + 2 * kMaxPhases; ++q) A.cons[q] = h->consField<T>(h->cur, q < h->nc ? q : 0);
+    A.rho = h->rhoP;
+    for (int d = 0; d < 3; ++d) A.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
+    A.pres = h->pres;
+    size_t n = (size_t)h->nx * h->ny * h->nz;
+    int nb = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(cons_to_prim_kernel<T>, dim3(nb), dim3(256), 0, h->stream, A);
+    HIPCHK(hipGetLastError());
+    return SIFV_OK;
+}
+
+int prim_field_set(sifv_handle* h, double** f, int* comp) {
+    int n = 0;
+    f[n] = h->rhoP; comp[n++] = 0;
+    for (int d = 0; d < h->dim; ++d) { f[n] = h->vel[d]; comp[n++] = 1 + d; }
+    f[n] = h->pres; comp[n++] = 0;
+    f[n] = h->sigma; comp[n++] = 0;
+    if (h->precision == SIFV_FP64)
+        for (int q = h->dim + 2; q < h->nc; ++q) { f[n] = h->consField<double>(h->cur, q); comp[n++] = 0; }
+    return n;
+}
+
+int refresh_prim(sifv_handle* h) {
+    if (h->primFresh) return SIFV_OK;
+    int rc = h->precision == SIFV_MIXED ? launch_c2p<float>(h) : launch_c2p<double>(h);
+    if (rc) return rc;
+    if (h->nph == 0 && h->precision == SIFV_FP64) {
+        // single phase: SolutionState::rho is the conservative density
+        HIPCHK(hipMemcpyAsync(h->rhoP, h->consField<double>(h->cur, 0), h->total * sizeof(double),
+                              hipMemcpyDeviceToDevice, h->stream));
+    }
+    double* f[kMaxFill];
+    int comp[kMaxFill];
+    int nf = prim_field_set(h, f, comp);
+    rc = fill_dirs(h, f, comp, nf, false);
+    if (rc) return rc;
+    h->primFresh = true;
+    return SIFV_OK;
+}
+
+int fill_dirs(sifv_handle* h, double** f, const int* comp, int nf, bool scalar) {
+    for (int dir = 0; dir < h->dim; ++dir) {
+        FillArgs A;
+        A.g = h->grid();
+        A.dir = dir;
+        A.scalar = scalar ? 1 : 0;
+        A.nf = nf;
+        for (int q = 0; q < kMaxFill; ++q) {
+            A.f[q] = q < nf ? f[q] : nullptr;
+            A.comp[q] = q < nf ? comp[q] : 0;
+        }
+        size_t n0, n1, ngd;
+        if (dir == 0) { n0 = h->ny; n1 = h->nz; ngd = h->ngx; }
+        else if (dir == 1) { n0 = h->nxT; n1 = h->nz; ngd = h->ngy; }
+        else { n0 = h->nxT; n1 = h->nyT; ngd = h->ngz; }
+        size_t tot = n0 * n1 * 2 * ngd;
+        int nb = (int)std::max<size_t>(1, std::min<size_t>((tot + 255) / 256, 4096));
+        hipLaunchKernelGGL(fill_dir_kernel, dim3(nb), dim3(256), 0, h->stream, A);
+        HIPCHK(hipGetLastError());
+    }
+    return SIFV_OK;
+}
+
+// Enqueue one explicit step (dt finalize + RK stages).  loop: time-loop mode
+// with device-resident time and the active flag.
+int enqueue_explicit_step(sifv_handle* h, double targetDt, bool loop, double endTime) {
+    const sifv_config& c = h->cfg;
+    int rc;
+    if (c.explicitParams.constDt <= 0.0 && !h->dtNextValid) {
+        rc = launch_dt_cells(h, false);
+        if (rc) return rc;
+    }
+    rc = launch_finalize(h, c.explicitParams.cfl, c.explicitParams.maxDt, c.explicitParams.minDt,
+                         c.explicitParams.constDt, targetDt, false, loop, endTime);
+    if (rc) return rc;
+    const int ord = c.RKOrder;
+    const int A = h->cur, B = (h->cur + 1) % 3, C = (h->cur + 2) % 3;
+    const int* active = loop ? h->flags : nullptr;
+    auto stage = [&](int s, int src, int u0, int dst) {
+        bool last = s == ord - 1;
+        if (h->dim == 3)
+            return h->precision == SIFV_MIXED
+                       ? launch_stage3d<float>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active)
+                       : launch_stage3d<double>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active);
+        return h->precision == SIFV_MIXED
+                   ? launch_stage<float>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active)
+                   : launch_stage<double>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active);
+    };
+    if (ord == 1) {
+        rc = stage(0, A, -1, B);
+        if (rc) return rc;
+        h->cur = B;
+    } else if (ord == 2) {
+        rc = stage(0, A, -1, B);
+        if (!rc) rc = stage(1, B, A, A);
+        if (rc) return rc;
+    } else {
+        rc = stage(0, A, -1, B);
+        if (!rc) rc = stage(1, B, A, C);
+        if (!rc) rc = stage(2, C, A, A);
+        if (rc) return rc;
+    }
+    h->dtNextValid = c.explicitParams.constDt <= 0.0;
+    h->primFresh = false;
+    return SIFV_OK;
+}
+
+int check_step_config(sifv_handle* h, const sifv_config* cfg) {
+    if (cfg) {
+        // per-call fields of SimulationConfig (step/time, and parameters the
+        // reference reads on every call)
+        h->cfg.step = cfg->step;
+        h->cfg.time = cfg->time;
+    }
+    return SIFV_OK;
+}
+
+}  // namespace
+
+namespace {
+template <typename T>
+int launch_stage3d(sifv_handle*, int, int, int, const double*, bool, bool, const int*) {
+    return fail(SIFV_ERR_UNSUPPORTED, "3D stage kernel not available yet");
+}
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* sifv_last_error(void) { return g_err.c_str(); }
+int sifv_abi_version(void) { return SIFV_ABI_VERSION; }
+
+int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, const sifv_eos* eos,
+                int precision, sifv_handle** out) {
+    if (!mesh || !cfg || !eos || !out) return fail(SIFV_ERR_INVALID, "null argument");
+    *out = nullptr;
+    int rc = validate_config(cfg);
+    if (rc) return rc;
+    if (mesh->dim != cfg->dim) return fail(SIFV_ERR_INVALID, "mesh.dim != config.dim");
+    if (mesh->nGhost != cfg->nGhost) return fail(SIFV_ERR_INVALID, "mesh.nGhost != config.nGhost");
+    if (mesh->nx < 1 || (mesh->dim >= 2 && mesh->ny < 1) || (mesh->dim >= 3 && mesh->nz < 1))
+        return fail(SIFV_ERR_INVALID, "mesh needs at least one cell per active direction");
+    if (!mesh->xNodes || (mesh->dim >= 2 && !mesh->yNodes) || (mesh->dim >= 3 && !mesh->zNodes))
+        return fail(SIFV_ERR_INVALID, "missing node array");
+    if (riemann < SIFV_LF || riemann > SIFV_HLLC) return fail(SIFV_ERR_INVALID, "unknown Riemann solver");
+    if (precision != SIFV_FP64 && precision != SIFV_MIXED) return fail(SIFV_ERR_INVALID, "unknown precision");
+    if (eos->kind != SIFV_EOS_IDEAL && eos->kind != SIFV_EOS_STIFFENED) return fail(SIFV_ERR_INVALID, "unknown EOS");
+    for (int f = 0; f < 6; ++f)
+        if (mesh->bc[f] < 0 || mesh->bc[f] > 4) return fail(SIFV_ERR_INVALID, "unknown boundary condition");
+    if (cfg->multiPhaseParams.nPhases > 0)
+        return fail(SIFV_ERR_UNSUPPORTED, "multiphase states are not on the explicit GPU path yet");
+    if (cfg->useIGR) return fail(SIFV_ERR_UNSUPPORTED, "IGR is not on the GPU path yet");
+    if (precision == SIFV_MIXED && cfg->semiImplicit)
+        return fail(SIFV_ERR_UNSUPPORTED, "mixed precision is available for the explicit path only");
+
+    auto* h = new sifv_handle();
+    h->dim = mesh->dim;
+    h->nx = mesh->nx;
+    h->ny = mesh->dim >= 2 ? mesh->ny : 1;
+    h->nz = mesh->dim >= 3 ? mesh->nz : 1;
+    h->ng = mesh->nGhost;
+    h->ngx = h->ng;
+    h->ngy = h->dim >= 2 ? h->ng : 0;
+    h->ngz = h->dim >= 3 ? h->ng : 0;
+    h->nxT = h->nx + 2 * h->ngx;
+    h->nyT = h->ny + 2 * h->ngy;
+    h->nzT = h->nz + 2 * h->ngz;
+    h->total = (size_t)h->nxT * h->nyT * h->nzT;
+    h->cfg = *cfg;
+    h->riemann = riemann;
+    h->eos = *eos;
+    h->precision = precision;
+    h->nph = cfg->multiPhaseParams.nPhases;
+    h->nc = h->dim + 2 + 2 * h->nph;
+    const int nn[3] = {h->nx, h->ny, h->nz};
+    for (int f = 0; f < 6; ++f) {
+        int d = f / 2;
+        bool periodic = d < h->dim && mesh->periods[d] != 0;
+        int b = periodic ? SIFV_BC_PERIODIC : mesh->bc[f];
+        switch (b) {
+            case SIFV_BC_PERIODIC: h->bc[f] = BK_WRAP; break;
+            case SIFV_BC_SYMMETRY: h->bc[f] = BK_MIRROR; break;
+            case SIFV_BC_SLIPWALL: h->bc[f] = BK_SLIP; break;
+            case SIFV_BC_NOSLIPWALL: h->bc[f] = BK_NOSLIP; break;
+            default: h->bc[f] = BK_OUTFLOW; break;
+        }
+        h->sbc[f] = h->bc[f] == BK_WRAP ? BK_WRAP : BK_OUTFLOW;
+        if (d < h->dim && h->bc[f] != BK_OUTFLOW && nn[d] < h->ng) {
+            delete h;
+            return fail(SIFV_ERR_INVALID, "mirror/periodic ghost rules need at least nGhost cells per direction");
+        }
+    }
+    static const double unit[2] = {0.0, 1.0};
+    std::vector<double> xe = extended_nodes(mesh->xNodes, h->nx, h->ngx);
+    std::vector<double> ye = extended_nodes(h->dim >= 2 ? mesh->yNodes : unit, h->ny, h->ngy);
+    std::vector<double> ze = extended_nodes(h->dim >= 3 ? mesh->zNodes : unit, h->nz, h->ngz);
+    std::vector<double> dx(h->nxT), dy(h->nyT), dz(h->nzT);
+    for (int i = 0; i < h->nxT; ++i) dx[i] = xe[i + 1] - xe[i];
+    for (int j = 0; j < h->nyT; ++j) dy[j] = ye[j + 1] - ye[j];
+    for (int k = 0; k < h->nzT; ++k) dz[k] = ze[k + 1] - ze[k];
+
+    auto bail = [&](int code) {
+        sifv_destroy(h);
+        return code;
+    };
+#define CK(expr)                                                                                   \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return bail(fail(SIFV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    CK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    CK(hipMalloc(&h->d_dx, sizeof(double) * h->nxT));
+    CK(hipMalloc(&h->d_dy, sizeof(double) * h->nyT));
+    CK(hipMalloc(&h->d_dz, sizeof(double) * h->nzT));
+    CK(hipMemcpy(h->d_dx, dx.data(), sizeof(double) * h->nxT, hipMemcpyHostToDevice));
+    CK(hipMemcpy(h->d_dy, dy.data(), sizeof(double) * h->nyT, hipMemcpyHostToDevice));
+    CK(hipMemcpy(h->d_dz, dz.data(), sizeof(double) * h->nzT, hipMemcpyHostToDevice));
+    size_t cb = h->tsize() * h->total * h->nc;
+    for (int b = 0; b < 3; ++b) {
+        CK(hipMalloc(&h->cons[b], cb));
+        CK(hipMemset(h->cons[b], 0, cb));
+    }
+    CK(hipMalloc(&h->rhoP, sizeof(double) * h->total));
+    CK(hipMemset(h->rhoP, 0, sizeof(double) * h->total));
+    for (int d = 0; d < h->dim; ++d) {
+        CK(hipMalloc(&h->vel[d], sizeof(double) * h->total));
+        CK(hipMemset(h->vel[d], 0, sizeof(double) * h->total));
+    }
+    CK(hipMalloc(&h->pres, sizeof(double) * h->total));
+    CK(hipMemset(h->pres, 0, sizeof(double) * h->total));
+    CK(hipMalloc(&h->sigma, sizeof(double) * h->total));
+    CK(hipMemset(h->sigma, 0, sizeof(double) * h->total));
+    CK(hipMalloc(&h->scal, sizeof(double) * 4));
+    CK(hipMemset(h->scal, 0, sizeof(double) * 4));
+    CK(hipMalloc(&h->flags, sizeof(int) * 4));
+    CK(hipMemset(h->flags, 0, sizeof(int) * 4));
+    CK(hipHostMalloc(&h->hostPinned, sizeof(double) * 4, hipHostMallocDefault));
+#undef CK
+    h->dtNextValid = false;
+    h->primFresh = true;
+    *out = h;
+    return SIFV_OK;
+}
+
+int sifv_destroy(sifv_handle* h) {
+    if (!h) return SIFV_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (auto& pr : h->timing.pending) {
+        hipEventDestroy(pr.first);
+        hipEventDestroy(pr.second);
+    }
+    hipFree(h->d_dx);
+    hipFree(h->d_dy);
+    hipFree(h->d_dz);
+    for (int b = 0; b < 3; ++b) hipFree(h->cons[b]);
+    hipFree(h->rhoP);
+    for (int d = 0; d < 3; ++d) hipFree(h->vel[d]);
+    hipFree(h->pres);
+    hipFree(h->sigma);
+    hipFree(h->partial);
+    hipFree(h->scal);
+    hipFree(h->flags);
+    for (int q = 0; q < 6; ++q) hipFree(h->jac[q]);
+    hipFree(h->resid);
+    if (h->hostPinned) hipHostFree(h->hostPinned);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return SIFV_OK;
+}
+
+int sifv_total_cells(const sifv_handle* h, size_t* n) {
+    if (!h || !n) return fail(SIFV_ERR_INVALID, "null argument");
+    *n = h->total;
+    return SIFV_OK;
+}
+
+namespace {
+// Resolve a field id to (cons buffer slot q, or prim array); returns -1 on
+// an unknown/absent field.
+int cons_slot(const sifv_handle* h, int field) {
+    if (field == SIFV_F_RHO) return 0;
+    if (field >= SIFV_F_RHOU && field <= SIFV_F_RHOW) {
+        int d = field - SIFV_F_RHOU;
+        return d < h->dim ? 1 + d : -1;
+    }
+    if (field == SIFV_F_RHOE) return h->dim + 1;
+    if (field >= SIFV_F_ALPHARHO && field < SIFV_F_ALPHARHO + SIFV_MAX_PHASES) {
+        int p = field - SIFV_F_ALPHARHO;
+        return p < h->nph ? h->dim + 2 + p : -1;
+    }
+    if (field >= SIFV_F_ALPHA && field < SIFV_F_ALPHA + SIFV_MAX_PHASES) {
+        int p = field - SIFV_F_ALPHA;
+        return p < h->nph ? h->dim + 2 + h->nph + p : -1;
+    }
+    return -1;
+}
+double* prim_ptr(sifv_handle* h, int field) {
+    if (field >= SIFV_F_VELU && field <= SIFV_F_VELW) {
+        int d = field - SIFV_F_VELU;
+        return d < h->dim ? h->vel[d] : nullptr;
+    }
+    if (field == SIFV_F_PRES) return h->pres;
+    if (field == SIFV_F_SIGMA) return h->sigma;
+    return nullptr;
+}
+}  // namespace
+
+int sifv_upload(sifv_handle* h, int field, const double* host, size_t n) {
+    if (!h || !host) return fail(SIFV_ERR_INVALID, "null argument");
+    if (n != h->total) return fail(SIFV_ERR_INVALID, "array length != totalCells");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    int q = cons_slot(h, field);
+    if (q >= 0) {
+        if (h->precision == SIFV_MIXED) {
+            std::vector<float> tmp(host, host + n);
+            HIPCHK(hipMemcpy(h->consField<float>(h->cur, q), tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        } else {
+            HIPCHK(hipMemcpy(h->consField<double>(h->cur, q), host, n * sizeof(double), hipMemcpyHostToDevice));
+        }
+        if (field == SIFV_F_RHO) HIPCHK(hipMemcpy(h->rhoP, host, n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        double* p = prim_ptr(h, field);
+        if (!p) return fail(SIFV_ERR_INVALID, "field not present for this configuration");
+        HIPCHK(hipMemcpy(p, host, n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    h->dtNextValid = false;
+    return SIFV_OK;
+}
+
+int sifv_download(sifv_handle* h, int field, double* host, size_t n) {
+    if (!h || !host) return fail(SIFV_ERR_INVALID, "null argument");
+    if (n != h->total) return fail(SIFV_ERR_INVALID, "array length != totalCells");
+    int rc = refresh_prim(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    int q = cons_slot(h, field);
+    if (field == SIFV_F_RHO) {
+        HIPCHK(hipMemcpy(host, h->rhoP, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else if (q >= 0) {
+        if (h->precision == SIFV_MIXED) {
+            std::vector<float> tmp(n);
+            HIPCHK(hipMemcpy(tmp.data(), h->consField<float>(h->cur, q), n * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t t = 0; t < n; ++t) host[t] = tmp[t];
+        } else {
+            HIPCHK(hipMemcpy(host, h->consField<double>(h->cur, q), n * sizeof(double), hipMemcpyDeviceToHost));
+        }
+    } else {
+        double* p = prim_ptr(h, field);
+        if (!p) return fail(SIFV_ERR_INVALID, "field not present for this configuration");
+        HIPCHK(hipMemcpy(host, p, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return SIFV_OK;
+}
+
+int sifv_explicit_step(sifv_handle* h, const sifv_config* cfg, double targetDt, double* dt_out) {
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    if (h->cfg.semiImplicit) return fail(SIFV_ERR_INVALID, "handle was created for the semi-implicit solver");
+    check_step_config(h, cfg);
+    int rc = enqueue_explicit_step(h, targetDt, false, 0.0);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->hostPinned, h->scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    rc = harvest_timing(h);
+    if (rc) return rc;
+    if (dt_out) *dt_out = h->hostPinned[0];
+    if (!std::isfinite(h->hostPinned[0])) return fail(SIFV_ERR_NONFINITE, "non-finite time step");
+    return SIFV_OK;
+}
+
+int sifv_explicit_advance(sifv_handle* h, const sifv_config* cfg, int nsteps, double endTime,
+                          double* time_inout, int* steps_done) {
+    if (!h || !time_inout) return fail(SIFV_ERR_INVALID, "null argument");
+    if (h->cfg.semiImplicit) return fail(SIFV_ERR_INVALID, "handle was created for the semi-implicit solver");
+    if (nsteps < 0) return fail(SIFV_ERR_INVALID, "nsteps < 0");
+    check_step_config(h, cfg);
+    hipLaunchKernelGGL(loop_reset_kernel, dim3(1), dim3(1), 0, h->stream, h->scal, h->flags, *time_inout);
+    HIPCHK(hipGetLastError());
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = enqueue_explicit_step(h, -1.0, true, endTime);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(h->hostPinned, h->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    int* hflags = reinterpret_cast<int*>(h->hostPinned + 2);
+    HIPCHK(hipMemcpyAsync(hflags, h->flags, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    int rc = harvest_timing(h);
+    if (rc) return rc;
+    *time_inout = h->hostPinned[1];
+    if (steps_done) *steps_done = hflags[1];
+    if (!std::isfinite(h->hostPinned[1])) return fail(SIFV_ERR_NONFINITE, "non-finite time");
+    return SIFV_OK;
+}
+
+int sifv_fill_ghosts(sifv_handle* h, int varset) {
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    if (h->precision != SIFV_FP64) return fail(SIFV_ERR_UNSUPPORTED, "ghost fill API needs an FP64 handle");
+    if (varset < SIFV_VARSET_CONS || varset > SIFV_VARSET_ALL) return fail(SIFV_ERR_INVALID, "unknown varset");
+    // The single-phase explicit path keeps rho in the conservative buffer;
+    // sync the primitive-side copy first so both sides hold the same density.
+    if (h->nph == 0)
+        HIPCHK(hipMemcpyAsync(h->rhoP, h->consField<double>(h->cur, 0), h->total * sizeof(double),
+                              hipMemcpyDeviceToDevice, h->stream));
+    double* f[kMaxFill];
+    int comp[kMaxFill];
+    int n = 0;
+    f[n] = h->rhoP; comp[n++] = 0;
+    for (int d = 0; d < h->dim; ++d) {
+        if (varset != SIFV_VARSET_PRIM) { f[n] = h->consField<double>(h->cur, 1 + d); comp[n++] = 1 + d; }
+        if (varset != SIFV_VARSET_CONS) { f[n] = h->vel[d]; comp[n++] = 1 + d; }
+    }
+    if (varset != SIFV_VARSET_PRIM) { f[n] = h->consField<double>(h->cur, h->dim + 1); comp[n++] = 0; }
+    if (varset != SIFV_VARSET_CONS) { f[n] = h->pres; comp[n++] = 0; }
+    f[n] = h->sigma; comp[n++] = 0;
+    for (int q = h->dim + 2; q < h->nc; ++q) { f[n] = h->consField<double>(h->cur, q); comp[n++] = 0; }
+    int rc = fill_dirs(h, f, comp, n, false);
+    if (rc) return rc;
+    if (h->nph == 0)
+        HIPCHK(hipMemcpyAsync(h->consField<double>(h->cur, 0), h->rhoP, h->total * sizeof(double),
+                              hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return SIFV_OK;
+}
+
+int sifv_fill_scalar_ghosts(sifv_handle* h, int field) {
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    double* p = nullptr;
+    if (field == SIFV_F_RHO) p = h->rhoP;
+    else p = prim_ptr(h, field);
+    if (!p) return fail(SIFV_ERR_INVALID, "scalar fill supports rho, vel*, pres, sigma");
+    double* f[1] = {p};
+    int comp[1] = {0};
+    int rc = fill_dirs(h, f, comp, 1, true);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return SIFV_OK;
+}
+
+int sifv_acoustic_dt(sifv_handle* h, double cfl, double maxDt, double* dt) {
+    if (!h || !dt) return fail(SIFV_ERR_INVALID, "null argument");
+    int rc = launch_dt_cells(h, false);
+    if (!rc) rc = launch_finalize(h, cfl, maxDt, 0.0, -1.0, -1.0, false, false, 0.0);
+    if (rc) return rc;
+    h->dtNextValid = false;
+    HIPCHK(hipMemcpyAsync(h->hostPinned, h->scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *dt = h->hostPinned[0];
+    return SIFV_OK;
+}
+
+int sifv_advective_dt(sifv_handle* h, double cfl, double maxDt, double* dt) {
+    if (!h || !dt) return fail(SIFV_ERR_INVALID, "null argument");
+    int rc = launch_dt_cells(h, true);
+    if (!rc) rc = launch_finalize(h, cfl, maxDt, 0.0, -1.0, -1.0, false, false, 0.0);
+    if (rc) return rc;
+    h->dtNextValid = false;
+    HIPCHK(hipMemcpyAsync(h->hostPinned, h->scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *dt = h->hostPinned[0];
+    return SIFV_OK;
+}
+
+int sifv_cons_to_prim(sifv_handle* h) {
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    h->primFresh = false;
+    int rc = refresh_prim(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return SIFV_OK;
+}
+
+int sifv_timing_enable(sifv_handle* h, int on) {
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    h->timing.on = on != 0;
+    return SIFV_OK;
+}
+
+int sifv_timing_read(sifv_handle* h, double* kernel_ms, long* launches, int reset) {
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    int rc = harvest_timing(h);
+    if (rc) return rc;
+    if (kernel_ms) *kernel_ms = h->timing.ms;
+    if (launches) *launches = h->timing.launches;
+    if (reset) {
+        h->timing.ms = 0.0;
+        h->timing.launches = 0;
+    }
+    return SIFV_OK;
+}
+
+int sifv_synchronize(sifv_handle* h) {
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return SIFV_OK;
+}
+
+int sifv_set_tile(sifv_handle* h, int rows_per_wave) {
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    if (rows_per_wave < 0 || rows_per_wave > 4096) return fail(SIFV_ERR_INVALID, "rows_per_wave out of range");
+    h->rowsOverride = rows_per_wave;
+    return SIFV_OK;
+}
+
+int sifv_semi_implicit_step(sifv_handle* h, const sifv_config* cfg, double targetDt, double* dt_out,
+                            int* pressure_iters) {
+    (void)h; (void)cfg; (void)targetDt; (void)dt_out; (void)pressure_iters;
+    return fail(SIFV_ERR_UNSUPPORTED, "semi-implicit step not available yet");
+}
+
+int sifv_jacobi_solve(sifv_handle* h, const double* rho, const double* rhoc2, const double* rhs,
+                      double* pressure, double dt, double tolerance, int maxIter, int* iters,
+                      double* residual_history) {
+    (void)h; (void)rho; (void)rhoc2; (void)rhs; (void)pressure; (void)dt; (void)tolerance;
+    (void)maxIter; (void)iters; (void)residual_history;
+    return fail(SIFV_ERR_UNSUPPORTED, "Jacobi solver not available yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,354 @@
+// sifv_device.hpp - device building blocks of the fused finite-volume kernels
+// (gfx950, wave64).  Everything here is __device__ inline and templated on
+// the arithmetic type T of the bulk reconstruction/flux work (double for the
+// plain method, float for the mixed-precision form).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace sifv {
+
+constexpr int kMaxPhases = 8;
+constexpr int kWave = 64;
+
+// Ghost-cell rule of one domain face after resolving periodicity.
+enum BcKind : int { BK_WRAP = 0, BK_MIRROR = 1, BK_SLIP = 2, BK_NOSLIP = 3, BK_OUTFLOW = 4 };
+
+// Geometry and boundary description shared by all kernels.
+struct Grid {
+    int dim;
+    int nx, ny, nz;
+    int ngx, ngy, ngz;
+    int nxT, nyT, nzT;
+    int bc[6];            // BcKind per face (PRIM/CONS fills)
+    int sbc[6];           // BcKind per face for scalar fills (wrap or outflow)
+    const double* dxw;    // dx(i) at dxw[i + ngx], i in [-ngx, nx+ngx)
+    const double* dyw;
+    const double* dzw;
+};
+
+__device__ __forceinline__ size_t cell_index(const Grid& g, int i, int j, int k) {
+    return (size_t)(i + g.ngx) + (size_t)g.nxT * ((size_t)(j + g.ngy) + (size_t)g.nyT * (size_t)(k + g.ngz));
+}
+
+// Source index along one axis for a (possibly ghost) index c, following
+// RectilinearMesh::fillGhostX/Y/Z (src/RectilinearMesh.cpp:137-418) and the
+// periodic self exchange (src/HaloExchange.cpp:414-465).  flipN/flipT report
+// the normal / tangential velocity sign flips of wall rules.
+__device__ __forceinline__ int map_axis(int c, int n, int lo, int hi, bool& flipN, bool& flipT) {
+    if (c < 0) {
+        switch (lo) {
+            case BK_WRAP: return c + n;
+            case BK_OUTFLOW: return 0;
+            case BK_SLIP: flipN = !flipN; return -c - 1;
+            case BK_NOSLIP: flipN = !flipN; flipT = !flipT; return -c - 1;
+            default: return -c - 1;
+        }
+    }
+    if (c >= n) {
+        switch (hi) {
+            case BK_WRAP: return c - n;
+            case BK_OUTFLOW: return n - 1;
+            case BK_SLIP: flipN = !flipN; return 2 * n - 1 - c;
+            case BK_NOSLIP: flipN = !flipN; flipT = !flipT; return 2 * n - 1 - c;
+            default: return 2 * n - 1 - c;
+        }
+    }
+    return c;
+}
+
+// Scalar ghost rule (fillScalarGhosts, src/RectilinearMesh.cpp:424-624):
+// periodic wraps, every other rule is zero-gradient.
+__device__ __forceinline__ int map_axis_scalar(int c, int n, int lo, int hi) {
+    if (c < 0) return lo == BK_WRAP ? c + n : 0;
+    if (c >= n) return hi == BK_WRAP ? c - n : n - 1;
+    return c;
+}
+
+__device__ __forceinline__ double dmax_ref(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double dmin_ref(double a, double b) { return (b < a) ? b : a; }
+template <typename T>
+__device__ __forceinline__ T tmax(T a, T b) { return (a < b) ? b : a; }
+template <typename T>
+__device__ __forceinline__ T tmin(T a, T b) { return (b < a) ? b : a; }
+
+// Equation of state constants of the single-phase path.
+struct Eos {
+    double gamma, pInf;
+    int stiffened;
+};
+
+// cons -> prim of one cell in FP64: IdealGasEOS::toPrimitive
+// (src/IdealGasEOS.cpp:46-64) / StiffenedGasEOS::toPrimitive
+// (src/StiffenedGasEOS.cpp:47-65).  rho is passed through unclamped because
+// the reconstruction reads SolutionState::rho, which the conversion does not
+// modify (src/SolutionState.cpp:187-205).
+template <int DIM>
+__device__ __forceinline__ void cons_to_prim(const Eos& e, double rho, const double* mom, double E,
+                                             double* u, double& p) {
+    double rs = dmax_ref(rho, 1e-14);
+    double ke = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        u[d] = mom[d] / rs;
+        ke += u[d] * u[d];
+    }
+    ke *= 0.5 * rs;
+    double ein = E - ke;
+    if (e.stiffened)
+        p = (e.gamma - 1.0) * ein - e.gamma * e.pInf;
+    else
+        p = dmax_ref((e.gamma - 1.0) * ein, 1e-14);
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction of one cell: returns the right state of its low face
+// ("minus") and the left state of its high face ("plus") from the 2R-1 cell
+// stencil v[0..2R-2] centred on the cell.  Formulas: Reconstructor::weno3*,
+// weno5*, upwind3*, upwind5* (src/Reconstruction.cpp:83-200).  For WENO5 the
+// three smoothness indicators are shared by both sides (the right-biased
+// stencil is the mirror image of the left-biased one).
+// ---------------------------------------------------------------------------
+template <int R, typename T>
+struct Recon;
+
+template <typename T>
+struct Recon<1, T> {
+    __device__ __forceinline__ static void run(const T* v, T, bool, T& minus, T& plus) {
+        minus = v[0];
+        plus = v[0];
+    }
+};
+
+template <typename T>
+struct Recon<2, T> {
+    __device__ __forceinline__ static void run(const T* v, T eps, bool weno, T& minus, T& plus) {
+        const T h = T(0.5), th = T(1.5);
+        T pl0 = -h * v[0] + th * v[1];
+        T pl1 = h * v[1] + h * v[2];
+        T pr0 = h * v[0] + h * v[1];
+        T pr1 = th * v[1] - h * v[2];
+        if (!weno) {
+            plus = (T(1.0) / T(3.0)) * pl0 + (T(2.0) / T(3.0)) * pl1;
+            minus = (T(2.0) / T(3.0)) * pr0 + (T(1.0) / T(3.0)) * pr1;
+            return;
+        }
+        T b0 = (v[1] - v[0]) * (v[1] - v[0]);
+        T b1 = (v[2] - v[1]) * (v[2] - v[1]);
+        T q0 = T(1.0) / ((eps + b0) * (eps + b0));
+        T q1 = T(1.0) / ((eps + b1) * (eps + b1));
+        T aL0 = (T(1.0) / T(3.0)) * q0, aL1 = (T(2.0) / T(3.0)) * q1;
+        T aR0 = (T(2.0) / T(3.0)) * q0, aR1 = (T(1.0) / T(3.0)) * q1;
+        plus = (aL0 * pl0 + aL1 * pl1) / (aL0 + aL1);
+        minus = (aR0 * pr0 + aR1 * pr1) / (aR0 + aR1);
+    }
+};
+
+template <typename T>
+struct Recon<3, T> {
+    __device__ __forceinline__ static void run(const T* v, T eps, bool weno, T& minus, T& plus) {
+        const T c13 = T(1.0) / T(3.0), c76 = T(7.0) / T(6.0), c116 = T(11.0) / T(6.0);
+        const T c16 = T(1.0) / T(6.0), c56 = T(5.0) / T(6.0);
+        // left-biased candidates (face c+1/2)
+        T pl0 = c13 * v[0] - c76 * v[1] + c116 * v[2];
+        T pl1 = -c16 * v[1] + c56 * v[2] + c13 * v[3];
+        T pl2 = c13 * v[2] + c56 * v[3] - c16 * v[4];
+        // right-biased candidates (face c-1/2), mirrored stencil
+        T pr0 = c13 * v[4] - c76 * v[3] + c116 * v[2];
+        T pr1 = -c16 * v[3] + c56 * v[2] + c13 * v[1];
+        T pr2 = c13 * v[2] + c56 * v[1] - c16 * v[0];
+        const T d0 = T(1.0) / T(10.0), d1 = T(6.0) / T(10.0), d2 = T(3.0) / T(10.0);
+        if (!weno) {
+            plus = d0 * pl0 + d1 * pl1 + d2 * pl2;
+            minus = d0 * pr0 + d1 * pr1 + d2 * pr2;
+            return;
+        }
+        const T c1312 = T(13.0) / T(12.0), c14 = T(1.0) / T(4.0);
+        T t0 = v[0] - T(2.0) * v[1] + v[2], s0 = v[0] - T(4.0) * v[1] + T(3.0) * v[2];
+        T t1 = v[1] - T(2.0) * v[2] + v[3], s1 = v[1] - v[3];
+        T t2 = v[2] - T(2.0) * v[3] + v[4], s2 = T(3.0) * v[2] - T(4.0) * v[3] + v[4];
+        T b0 = c1312 * t0 * t0 + c14 * s0 * s0;
+        T b1 = c1312 * t1 * t1 + c14 * s1 * s1;
+        T b2 = c1312 * t2 * t2 + c14 * s2 * s2;
+        T e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+        T q0 = T(1.0) / (e0 * e0);
+        T q1 = T(1.0) / (e1 * e1);
+        T q2 = T(1.0) / (e2 * e2);
+        T aL0 = d0 * q0, aL1 = d1 * q1, aL2 = d2 * q2;
+        T aR0 = d0 * q2, aR1 = d1 * q1, aR2 = d2 * q0;
+        plus = (aL0 * pl0 + aL1 * pl1 + aL2 * pl2) / (aL0 + aL1 + aL2);
+        minus = (aR0 * pr0 + aR1 * pr1 + aR2 * pr2) / (aR0 + aR1 + aR2);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Riemann fluxes along axis D of a DIM-dimensional problem.  Face states are
+// primitive (rho, u[DIM], p) with gammaEff/piInfEff (constant per face for a
+// single phase).  Formulas: computeHLLCFlux (src/HLLCSolver.cpp:7-161),
+// computeRusanovFlux (src/RusanovSolver.cpp:7-83), computeLFFlux
+// (src/LFSolver.cpp:8-75), helpers include/RiemannSolver.hpp:58-78.
+// Output F[0]=mass, F[1..DIM]=momentum, F[DIM+1]=energy, F[DIM+2]=pressure
+// flux, F[DIM+3]=face velocity.
+// ---------------------------------------------------------------------------
+template <int DIM, typename T>
+struct FaceState {
+    T rho, u[DIM], p, sig;
+};
+
+template <int DIM, typename T>
+__device__ __forceinline__ T sound_direct(const FaceState<DIM, T>& w, T g, T pi) {
+    return sqrt(g * tmax(w.p + pi, T(1e-14)) / tmax(w.rho, T(1e-14)));
+}
+
+template <int DIM, typename T>
+__device__ __forceinline__ T rhoE_state(const FaceState<DIM, T>& w, T g, T pi) {
+    T q = w.u[0] * w.u[0];
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) q += w.u[d] * w.u[d];
+    T ke = T(0.5) * w.rho * q;
+    return (w.p + g * pi) / (g - T(1.0)) + ke;
+}
+
+enum { RS_LF = 0, RS_RUSANOV = 1, RS_HLLC = 2 };
+
+template <int RS, int D, int DIM, typename T>
+__device__ __forceinline__ void riemann(const FaceState<DIM, T>& L, const FaceState<DIM, T>& R, T g,
+                                        T pi, bool incP, bool igr, T* F) {
+    T uL = L.u[D], uR = R.u[D];
+    if (RS == RS_HLLC) {
+        T EL = rhoE_state(L, g, pi), ER = rhoE_state(R, g, pi);
+        T sL, sR, sS;
+        if (incP) {
+            T cL = sound_direct(L, g, pi), cR = sound_direct(R, g, pi);
+            sL = tmin(uL - cL, uR - cR);
+            sR = tmax(uL + cL, uR + cR);
+            sS = (R.p - L.p + L.rho * uL * (sL - uL) - R.rho * uR * (sR - uR)) /
+                 (L.rho * (sL - uL) - R.rho * (sR - uR));
+        } else {
+            sL = tmin(uL, uR);
+            sR = tmax(uL, uR);
+            sS = T(0.5) * (uL + uR);
+        }
+        if (sL >= T(0) || sR <= T(0)) {
+            const FaceState<DIM, T>& W = (sL >= T(0)) ? L : R;
+            T un = (sL >= T(0)) ? uL : uR;
+            T E = (sL >= T(0)) ? EL : ER;
+            F[0] = W.rho * un;
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) F[1 + i] = W.rho * W.u[i] * un;
+            F[DIM + 1] = E * un;
+            if (incP) {
+                F[1 + D] += W.p;
+                F[DIM + 1] += W.p * un;
+            }
+            F[DIM + 2] = W.p * un;
+            F[DIM + 3] = un;
+        } else {
+            const bool left = sS >= T(0);
+            const FaceState<DIM, T>& W = left ? L : R;
+            T un = left ? uL : uR;
+            T s = left ? sL : sR;
+            T E = left ? EL : ER;
+            T rS = W.rho * (s - un) / (s - sS);
+            F[0] = W.rho * un + s * (rS - W.rho);
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) {
+                T ui = W.u[i];
+                T ruS = (i == D) ? rS * (ui + (sS - un)) : rS * ui;
+                T base = W.rho * ui * un;
+                if (incP && i == D) base = base + W.p;
+                F[1 + i] = base + s * (ruS - W.rho * ui);
+            }
+            T e = E / W.rho;
+            if (incP) {
+                T ES = rS * (e + (sS - un) * (sS + W.p / (W.rho * (s - un))));
+                F[DIM + 1] = (E + W.p) * un + s * (ES - E);
+            } else {
+                T ES = rS * (e + (sS - un) * sS);
+                F[DIM + 1] = E * un + s * (ES - E);
+            }
+            F[DIM + 2] = W.p * sS;
+            F[DIM + 3] = sS;
+        }
+    } else {
+        // Rusanov and LF share the central + dissipation form.
+        T EL = rhoE_state(L, g, pi), ER = rhoE_state(R, g, pi);
+        T S;
+        if (RS == RS_RUSANOV) {
+            T aL = fabs(uL), aR = fabs(uR);
+            if (incP) {
+                T cL = sound_direct(L, g, pi), cR = sound_direct(R, g, pi);
+                S = tmax(aL + cL, aR + cR);
+            } else {
+                S = tmax(aL, aR);
+            }
+            T mL = L.rho * uL, mR = R.rho * uR;
+            F[0] = T(0.5) * (mL + mR) - T(0.5) * S * (R.rho - L.rho);
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) {
+                T pL = L.rho * L.u[i] * uL, pR = R.rho * R.u[i] * uR;
+                if (incP && i == D) {
+                    pL += L.p;
+                    pR += R.p;
+                }
+                F[1 + i] = T(0.5) * (pL + pR) - T(0.5) * S * (R.rho * R.u[i] - L.rho * L.u[i]);
+            }
+            T eL = EL * uL, eR = ER * uR;
+            if (incP) {
+                eL += L.p * uL;
+                eR += R.p * uR;
+            }
+            F[DIM + 1] = T(0.5) * (eL + eR) - T(0.5) * S * (ER - EL);
+        } else {
+            T qL = L.u[0] * L.u[0], qR = R.u[0] * R.u[0];
+#pragma unroll
+            for (int d = 1; d < DIM; ++d) {
+                qL += L.u[d] * L.u[d];
+                qR += R.u[d] * R.u[d];
+            }
+            qL = sqrt(qL);
+            qR = sqrt(qR);
+            if (incP) {
+                T cL = sound_direct(L, g, pi), cR = sound_direct(R, g, pi);
+                S = tmax(qL, qR) + tmax(cL, cR);
+            } else {
+                S = tmax(qL, qR);
+            }
+            F[0] = T(0.5) * (L.rho * uL + R.rho * uR) - T(0.5) * S * (R.rho - L.rho);
+#pragma unroll
+            for (int i = 0; i < DIM; ++i)
+                F[1 + i] = T(0.5) * (L.rho * L.u[i] * uL + R.rho * R.u[i] * uR) -
+                           T(0.5) * S * (R.rho * R.u[i] - L.rho * L.u[i]);
+            F[DIM + 1] = T(0.5) * (EL * uL + ER * uR) - T(0.5) * S * (ER - EL);
+            if (incP) {
+                F[1 + D] += T(0.5) * (L.p + R.p);
+                F[DIM + 1] += T(0.5) * (L.p * uL + R.p * uR);
+            }
+            if (igr) {
+                F[1 + D] += T(0.5) * (L.sig + R.sig);
+                F[DIM + 1] += T(0.5) * (L.sig * uL + R.sig * uR);
+            }
+        }
+        F[DIM + 3] = T(0.5) * (uL + uR);
+        F[DIM + 2] = T(0.5) * (L.p * uL + R.p * uR) - T(0.5) * S * (R.p - L.p);
+    }
+}
+
+// Wave-wide minimum of a double (wave64, DPP/bpermute through __shfl_xor).
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = dmin_ref(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double w = __shfl_xor(v, o, kWave);
+        v = (v < w) ? w : v;
+    }
+    return v;
+}
+
+}  // namespace sifv

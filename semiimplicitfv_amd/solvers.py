@@ -1,0 +1,193 @@
+"""GPU-backed mirrors of the reference's solver classes.
+
+ExplicitSolver / SemiImplicitSolver keep the reference constructor and
+step() signatures (include/ExplicitSolver.hpp:23-46,
+include/SemiImplicitSolver.hpp:24-46) and drive libsifv_hip.so through the
+C ABI.  step() is a drop-in: it uploads the host SolutionState, advances it
+on the GPU and writes every field back, exactly like the reference mutates
+its SolutionState.  advance() keeps the state resident on the device for
+many steps (the time-loop form used for measurements).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from .config import SimulationConfig
+from .mesh import RectilinearMesh
+from .state import SolutionState
+
+
+class _GpuSolverBase:
+    def __init__(self, mesh: RectilinearMesh, riemannSolver, eos, igrSolver, config: SimulationConfig,
+                 precision: str = "fp64"):
+        config.validate()
+        self._lib = _abi.load_library()
+        self.mesh = mesh
+        self.config = config
+        self.eos = eos
+        self.riemann = int(riemannSolver.kind())
+        self.precision = {"fp64": _abi.FP64, "mixed": _abi.MIXED}[precision]
+        h = C.c_void_p()
+        mc = mesh.to_c()
+        cc = config.to_c()
+        ec = eos.to_c()
+        _abi.check(self._lib.sifv_create(C.byref(mc), C.byref(cc), self.riemann, C.byref(ec), self.precision,
+                                         C.byref(h)))
+        self._h = h
+        n = C.c_size_t()
+        _abi.check(self._lib.sifv_total_cells(self._h, C.byref(n)))
+        self.n = n.value
+        if self.n != mesh.totalCells():
+            raise RuntimeError("device and host disagree on totalCells")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sifv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state transfer
+    def upload(self, state: SolutionState):
+        for fid in state.field_ids():
+            a = state.field(fid)
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            _abi.check(self._lib.sifv_upload(self._h, fid, _abi.dptr(a), a.size))
+
+    def download(self, state: SolutionState):
+        for fid in state.field_ids():
+            a = state.field(fid)
+            buf = np.empty(self.n, dtype=np.float64)
+            _abi.check(self._lib.sifv_download(self._h, fid, _abi.dptr(buf), buf.size))
+            a[:] = buf
+
+    def download_field(self, fid: int) -> np.ndarray:
+        buf = np.empty(self.n, dtype=np.float64)
+        _abi.check(self._lib.sifv_download(self._h, fid, _abi.dptr(buf), buf.size))
+        return buf
+
+    def upload_field(self, fid: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _abi.check(self._lib.sifv_upload(self._h, fid, _abi.dptr(a), a.size))
+
+    # ---- reference-style helpers
+    def fill_ghosts(self, varset: int):
+        _abi.check(self._lib.sifv_fill_ghosts(self._h, varset))
+
+    def fill_scalar_ghosts(self, fid: int):
+        _abi.check(self._lib.sifv_fill_scalar_ghosts(self._h, fid))
+
+    def acoustic_dt(self, cfl: float, maxDt: float) -> float:
+        d = C.c_double()
+        _abi.check(self._lib.sifv_acoustic_dt(self._h, cfl, maxDt, C.byref(d)))
+        return d.value
+
+    def advective_dt(self, cfl: float, maxDt: float) -> float:
+        d = C.c_double()
+        _abi.check(self._lib.sifv_advective_dt(self._h, cfl, maxDt, C.byref(d)))
+        return d.value
+
+    def cons_to_prim(self):
+        _abi.check(self._lib.sifv_cons_to_prim(self._h))
+
+    def synchronize(self):
+        _abi.check(self._lib.sifv_synchronize(self._h))
+
+    def set_tile(self, rows: int):
+        _abi.check(self._lib.sifv_set_tile(self._h, rows))
+
+    def timing(self, on: bool = True):
+        _abi.check(self._lib.sifv_timing_enable(self._h, int(on)))
+
+    def read_timing(self, reset: bool = True):
+        ms = C.c_double()
+        n = C.c_long()
+        _abi.check(self._lib.sifv_timing_read(self._h, C.byref(ms), C.byref(n), int(reset)))
+        return ms.value, n.value
+
+
+class ExplicitSolver(_GpuSolverBase):
+    """ExplicitSolver (include/ExplicitSolver.hpp:21-82) on the GPU."""
+
+    def step(self, config: SimulationConfig, mesh: RectilinearMesh, state: SolutionState,
+             targetDt: float = -1.0) -> float:
+        self.upload(state)
+        dt = self.step_resident(config, targetDt)
+        self.download(state)
+        return dt
+
+    def step_resident(self, config: SimulationConfig, targetDt: float = -1.0) -> float:
+        """One step on the device-resident state (no host transfer)."""
+        d = C.c_double()
+        cc = config.to_c()
+        _abi.check(self._lib.sifv_explicit_step(self._h, C.byref(cc), float(targetDt), C.byref(d)))
+        return d.value
+
+    def advance(self, config: SimulationConfig, nsteps: int, endTime: float, time: float = 0.0):
+        """Up to nsteps steps of runTimeLoop (targetDt = endTime - time, no
+        output in between) on the resident state; returns (time, steps)."""
+        t = C.c_double(time)
+        s = C.c_int()
+        cc = config.to_c()
+        _abi.check(self._lib.sifv_explicit_advance(self._h, C.byref(cc), int(nsteps), float(endTime), C.byref(t),
+                                                   C.byref(s)))
+        return t.value, s.value
+
+
+class SemiImplicitSolver(_GpuSolverBase):
+    """SemiImplicitSolver (include/SemiImplicitSolver.hpp:22-96) with the
+    Jacobi pressure solver, on the GPU."""
+
+    def __init__(self, mesh, riemannSolver, pressureSolver, eos, igrSolver, config, precision="fp64"):
+        super().__init__(mesh, riemannSolver, eos, igrSolver, config, precision)
+        self.pressureSolver_ = pressureSolver
+        self.lastIters_ = 0
+
+    def step(self, config, mesh, state, targetDt=-1.0):
+        self.upload(state)
+        dt = self.step_resident(config, targetDt)
+        self.download(state)
+        return dt
+
+    def step_resident(self, config, targetDt=-1.0):
+        d = C.c_double()
+        it = C.c_int()
+        cc = config.to_c()
+        _abi.check(self._lib.sifv_semi_implicit_step(self._h, C.byref(cc), float(targetDt), C.byref(d),
+                                                     C.byref(it)))
+        self.lastIters_ = it.value
+        return d.value
+
+    def lastPressureIterations(self):
+        return self.lastIters_
+
+
+class JacobiPressureSolver:
+    """PressureSolver::solve (include/PressureSolver.hpp:17-40) backed by the
+    GPU Jacobi sweep.  Needs a handle for the mesh (any solver built on the
+    same mesh)."""
+
+    def __init__(self, gpu_solver: _GpuSolverBase | None = None):
+        self.gpu = gpu_solver
+
+    def name(self):
+        return "Jacobi"
+
+    def solve(self, mesh, rho, rhoc2, rhs, pressure, dt, tolerance, maxIter, history=None):
+        lib = _abi.load_library()
+        it = C.c_int()
+        hist = history if history is not None else None
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, rhoc2, rhs)]
+        if not (pressure.flags.c_contiguous and pressure.dtype == np.float64):
+            raise ValueError("pressure must be a contiguous float64 array (updated in place)")
+        _abi.check(lib.sifv_jacobi_solve(self.gpu._h, _abi.dptr(arrs[0]), _abi.dptr(arrs[1]), _abi.dptr(arrs[2]),
+                                         _abi.dptr(pressure), float(dt), float(tolerance), int(maxIter),
+                                         C.byref(it), _abi.dptr(hist) if hist is not None else None))
+        return it.value
