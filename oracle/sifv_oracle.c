@@ -42,7 +42,8 @@ typedef struct {
     size_t total;
     double *xe, *ye, *ze; /* extended node arrays */
     int bc[6];
-    int wrap[6]; /* face wraps periodically (Periodic BC or MPI periods) */
+    int wrap[6];    /* face wraps periodically (Periodic BC or MPI periods) */
+    int mpiwrap[6]; /* wrap comes from MPI periods (halo self exchange) */
 } or_mesh;
 
 /* buildExtendedNodes, src/RectilinearMesh.cpp:91-114 */
@@ -140,6 +141,7 @@ or_solver* or_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann,
          * faces (src/Runtime.cpp:123-127); the halo self-exchange wraps. */
         m->bc[f] = periodic ? SIFV_BC_OUTFLOW : mesh->bc[f];
         m->wrap[f] = periodic || mesh->bc[f] == SIFV_BC_PERIODIC;
+        m->mpiwrap[f] = periodic;
     }
     s->riemann = riemann;
     s->eosKind = eos->kind;
@@ -201,9 +203,12 @@ void or_destroy(or_solver* s) {
 /* ------------------------------------------------------------------ */
 /* ghost fill                                                           */
 /* ------------------------------------------------------------------ */
-/* copyCell_P / copyCell_C / copyCell, src/SolutionState.cpp:79-147 */
+/* copyCell_P / copyCell_C / copyCell, src/SolutionState.cpp:79-147.  For
+ * faces wrapped by the MPI self exchange the copied set is the halo pack's
+ * (src/HaloExchange.cpp:206-262): identical except that the CONS pack
+ * carries no sigma. */
 static void copy_cell(const or_mesh* m, double* const* f, int varset, size_t dst, size_t src,
-                      double sU, double sV, double sW) {
+                      double sU, double sV, double sW, int exch) {
     const int vel[3] = {SIFV_F_VELU, SIFV_F_VELV, SIFV_F_VELW};
     const int mom[3] = {SIFV_F_RHOU, SIFV_F_RHOV, SIFV_F_RHOW};
     const double sg[3] = {sU, sV, sW};
@@ -214,7 +219,7 @@ static void copy_cell(const or_mesh* m, double* const* f, int varset, size_t dst
     }
     if (varset != SIFV_VARSET_PRIM && f[SIFV_F_RHOE]) f[SIFV_F_RHOE][dst] = f[SIFV_F_RHOE][src];
     if (varset != SIFV_VARSET_CONS && f[SIFV_F_PRES]) f[SIFV_F_PRES][dst] = f[SIFV_F_PRES][src];
-    if (f[SIFV_F_SIGMA]) f[SIFV_F_SIGMA][dst] = f[SIFV_F_SIGMA][src];
+    if (f[SIFV_F_SIGMA] && !(exch && varset == SIFV_VARSET_CONS)) f[SIFV_F_SIGMA][dst] = f[SIFV_F_SIGMA][src];
     for (int ph = 0; ph < MAXPH; ++ph) {
         if (f[SIFV_F_ALPHARHO + ph]) f[SIFV_F_ALPHARHO + ph][dst] = f[SIFV_F_ALPHARHO + ph][src];
         if (f[SIFV_F_ALPHA + ph]) f[SIFV_F_ALPHA + ph][dst] = f[SIFV_F_ALPHA + ph][src];
@@ -250,7 +255,7 @@ static void fill_ghosts(const or_mesh* m, double* const* f, int varset) {
                     int face = SIFV_XLOW + side;
                     int si = ghost_src(m, face, m->nx, g, &sn, &st);
                     int gi = side ? m->nx - 1 + g : -g;
-                    copy_cell(m, f, varset, IDX(m, gi, j, k), IDX(m, si, j, k), sn, st, st);
+                    copy_cell(m, f, varset, IDX(m, gi, j, k), IDX(m, si, j, k), sn, st, st, m->mpiwrap[face]);
                 }
     if (m->dim >= 2)
         for (int k = 0; k < m->nz; ++k)
@@ -260,7 +265,7 @@ static void fill_ghosts(const or_mesh* m, double* const* f, int varset) {
                         int face = SIFV_YLOW + side;
                         int sj = ghost_src(m, face, m->ny, g, &sn, &st);
                         int gj = side ? m->ny - 1 + g : -g;
-                        copy_cell(m, f, varset, IDX(m, i, gj, k), IDX(m, i, sj, k), st, sn, st);
+                        copy_cell(m, f, varset, IDX(m, i, gj, k), IDX(m, i, sj, k), st, sn, st, m->mpiwrap[face]);
                     }
     if (m->dim >= 3)
         for (int j = -m->ngy; j < m->ny + m->ngy; ++j)
@@ -270,7 +275,7 @@ static void fill_ghosts(const or_mesh* m, double* const* f, int varset) {
                         int face = SIFV_ZLOW + side;
                         int sk = ghost_src(m, face, m->nz, g, &sn, &st);
                         int gk = side ? m->nz - 1 + g : -g;
-                        copy_cell(m, f, varset, IDX(m, i, j, gk), IDX(m, i, j, sk), st, st, sn);
+                        copy_cell(m, f, varset, IDX(m, i, j, gk), IDX(m, i, j, sk), st, st, sn, m->mpiwrap[face]);
                     }
 }
 

@@ -126,6 +126,7 @@ struct sifv_handle {
     int dim, nx, ny, nz, ng, ngx, ngy, ngz, nxT, nyT, nzT;
     size_t total;
     int bc[6], sbc[6];
+    int periods[3];
     sifv_config cfg;
     int riemann;
     sifv_eos eos;
@@ -361,7 +362,8 @@ int launch_finalize(sifv_handle* h, double cfl, double maxDt, double minDt, doub
 
 // Prim refresh after a step: cons->prim on interior cells plus the PRIM
 // ghost fill (the finalize block of ExplicitSolver::step, src/ExplicitSolver.cpp:192-200).
-int fill_dirs(sifv_handle* h, double** f, const int* comp, int nf, bool scalar);
+int fill_dirs(sifv_handle* h, double** f, const int* comp, int nf, bool scalar,
+              const double* skipOnPeriods = nullptr);
 
 template <typename T>
 int launch_c2p(sifv_handle* h) {
@@ -395,11 +397,6 @@ int refresh_prim(sifv_handle* h) {
     if (h->primFresh) return SIFV_OK;
     int rc = h->precision == SIFV_MIXED ? launch_c2p<float>(h) : launch_c2p<double>(h);
     if (rc) return rc;
-    if (h->nph == 0 && h->precision == SIFV_FP64) {
-        // single phase: SolutionState::rho is the conservative density
-        HIPCHK(hipMemcpyAsync(h->rhoP, h->consField<double>(h->cur, 0), h->total * sizeof(double),
-                              hipMemcpyDeviceToDevice, h->stream));
-    }
     double* f[kMaxFill];
     int comp[kMaxFill];
     int nf = prim_field_set(h, f, comp);
@@ -409,16 +406,24 @@ int refresh_prim(sifv_handle* h) {
     return SIFV_OK;
 }
 
-int fill_dirs(sifv_handle* h, double** f, const int* comp, int nf, bool scalar) {
+int fill_dirs(sifv_handle* h, double** f, const int* comp, int nf, bool scalar, const double* skipOnPeriods) {
     for (int dir = 0; dir < h->dim; ++dir) {
         FillArgs A;
         A.g = h->grid();
         A.dir = dir;
         A.scalar = scalar ? 1 : 0;
-        A.nf = nf;
-        for (int q = 0; q < kMaxFill; ++q) {
-            A.f[q] = q < nf ? f[q] : nullptr;
-            A.comp[q] = q < nf ? comp[q] : 0;
+        int m = 0;
+        for (int q = 0; q < nf; ++q) {
+            // the halo pack of a periods axis leaves this field out
+            if (skipOnPeriods && f[q] == skipOnPeriods && h->periods[dir]) continue;
+            A.f[m] = f[q];
+            A.comp[m] = comp[q];
+            ++m;
+        }
+        A.nf = m;
+        for (int q = m; q < kMaxFill; ++q) {
+            A.f[q] = nullptr;
+            A.comp[q] = 0;
         }
         size_t n0, n1, ngd;
         if (dir == 0) { n0 = h->ny; n1 = h->nz; ngd = h->ngx; }
@@ -546,6 +551,7 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
     h->nph = cfg->multiPhaseParams.nPhases;
     h->nc = h->dim + 2 + 2 * h->nph;
     const int nn[3] = {h->nx, h->ny, h->nz};
+    for (int d = 0; d < 3; ++d) h->periods[d] = d < h->dim && mesh->periods[d] != 0;
     for (int f = 0; f < 6; ++f) {
         int d = f / 2;
         bool periodic = d < h->dim && mesh->periods[d] != 0;
@@ -785,7 +791,7 @@ int sifv_fill_ghosts(sifv_handle* h, int varset) {
     if (varset != SIFV_VARSET_CONS) { f[n] = h->pres; comp[n++] = 0; }
     f[n] = h->sigma; comp[n++] = 0;
     for (int q = h->dim + 2; q < h->nc; ++q) { f[n] = h->consField<double>(h->cur, q); comp[n++] = 0; }
-    int rc = fill_dirs(h, f, comp, n, false);
+    int rc = fill_dirs(h, f, comp, n, false, varset == SIFV_VARSET_CONS ? h->sigma : nullptr);
     if (rc) return rc;
     if (h->nph == 0)
         HIPCHK(hipMemcpyAsync(h->consField<double>(h->cur, 0), h->rhoP, h->total * sizeof(double),

@@ -211,6 +211,7 @@ __global__ void __launch_bounds__(256) cons_to_prim_kernel(C2PArgs<T> A) {
             double mom[3], u[3], p;
             for (int d = 0; d < dim; ++d) mom[d] = (double)A.cons[1 + d][id];
             double rho = (double)A.cons[0][id];
+            A.rho[id] = rho;
             if (dim == 1) cons_to_prim<1>(A.eos, rho, mom, (double)A.cons[2][id], u, p);
             else if (dim == 2) cons_to_prim<2>(A.eos, rho, mom, (double)A.cons[3][id], u, p);
             else cons_to_prim<3>(A.eos, rho, mom, (double)A.cons[4][id], u, p);

@@ -1,0 +1,152 @@
+"""Writes the golden fixtures of tests/golden/ from the compiled reference.
+
+    make -C oracle && python tests/golden/make_golden.py
+
+Every expected output here comes from the unmodified reference sources
+(oracle/_ref/libsifv_ref.so, built from the reference checkout by
+oracle/Makefile) on seeded inputs; the inputs are stored next to the
+outputs.  The fixtures are data only (float64 arrays in .npz files).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import pyoracle as po  # noqa: E402
+from semiimplicitfv_amd import (BoundaryCondition, IdealGasEOS, ReconstructionOrder, SimulationConfig,  # noqa: E402
+                                SolutionState, StiffenedGasEOS, cases, uniform_mesh)
+from semiimplicitfv_amd.cases import _fill  # noqa: E402
+from semiimplicitfv_amd.timeloop import run_time_loop  # noqa: E402
+from tests.golden.problems import (bubble_3d, jacobi_problem, liquid_gas_tube, random_ghost_case,  # noqa: E402
+                                   recon_lines, riemann_pairs)
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def save(name, **arrs):
+    np.savez_compressed(os.path.join(OUT, name), **arrs)
+    print("wrote", name, sum(a.nbytes for a in arrs.values()), "bytes raw")
+
+
+def g_recon():
+    out = {}
+    for order in (1, 2, 4, 5):
+        nx, ng, lines = recon_lines(order)
+        L = np.zeros((len(lines), nx + 1))
+        R = np.zeros((len(lines), nx + 1))
+        for q, v in enumerate(lines):
+            L[q], R[q] = po.recon_line_ref(order, 1e-6, nx, ng, v)
+        out[f"v{order}"] = lines
+        out[f"L{order}"] = L
+        out[f"R{order}"] = R
+    save("recon_kat.npz", **out)
+
+
+def g_riemann():
+    out = {}
+    for key, (Ls, Rs, meta) in riemann_pairs().items():
+        rtype, d, dim, incP, igr, nph = meta
+        F = np.array([po.riemann_flux("ref", rtype, l, r, d, dim, incP, igr, nph) for l, r in zip(Ls, Rs)])
+        out[key + "_L"] = Ls
+        out[key + "_R"] = Rs
+        out[key + "_F"] = F
+    save("riemann_kat.npz", **out)
+
+
+def g_sod():
+    cfg, mesh, st, eos, rs = cases.sod_1d(1024)
+    s = po.CpuSolver(mesh, cfg, rs.kind(), eos, "ref")
+    dts = []
+    t = 0.0
+    while t < 0.2:
+        dt = s.explicit_step(st, cfg, 0.2 - t)
+        dts.append(dt)
+        t += dt
+        cfg.step += 1
+    save("sod1024_final.npz", dts=np.array(dts), rho=st.rho, rhoU=st.rhoU, rhoE=st.rhoE, velU=st.velU,
+         pres=st.pres)
+
+
+def g_vortex():
+    cfg, mesh, st, eos, rs = cases.isentropic_vortex_2d(64)
+    s = po.CpuSolver(mesh, cfg, rs.kind(), eos, "ref")
+    dts = []
+    for k in range(50):
+        cfg.step = k
+        dts.append(s.explicit_step(st, cfg))
+    save("vortex64_50steps.npz", dts=np.array(dts), rho=st.rho, rhoU=st.rhoU, rhoV=st.rhoV, rhoE=st.rhoE)
+
+
+def g_ghost():
+    out = {}
+    for dim in (1, 2, 3):
+        cfg, mesh, st, bcs, periods = random_ghost_case(dim)
+        s = po.CpuSolver(mesh, cfg, 2, IdealGasEOS(1.4, 1.0, cfg), "ref")
+        for vs in (0, 1, 2):
+            t = st.copy()
+            s.fill_ghosts(t, vs)
+            for fid in t.field_ids():
+                out[f"d{dim}_vs{vs}_f{fid}"] = t.field(fid)
+        a = st.pres.copy()
+        s.fill_scalar(a)
+        out[f"d{dim}_scalar"] = a
+        for fid in st.field_ids():
+            out[f"d{dim}_in_f{fid}"] = st.field(fid)
+    save("ghost_fill.npz", **out)
+
+
+def g_jacobi():
+    cfg, mesh, rho, rhoc2, rhs, p0, dt = jacobi_problem()
+    s = po.CpuSolver(mesh, cfg, 2, IdealGasEOS(1.4, 1.0, cfg), "ref")
+    out = {"rho": rho, "rhoc2": rhoc2, "rhs": rhs, "p0": p0, "dt": np.array([dt])}
+    for tol, mi in ((1e-6, 1000), (1e-300, 7)):
+        p = p0.copy()
+        it = s.jacobi(rho, rhoc2, rhs, p, dt, tol, mi)
+        out[f"p_tol{tol:g}"] = p
+        out[f"it_tol{tol:g}"] = np.array([it])
+    save("jacobi_3d.npz", **out)
+
+
+def g_semi():
+    cfg, mesh, st, eos, rs = bubble_3d(12, perturb=True)
+    s = po.CpuSolver(mesh, cfg, rs.kind(), eos, "ref")
+    its, dts = [], []
+    for k in range(3):
+        cfg.step = k
+        dt, it = s.semi_step(st, cfg)
+        its.append(it)
+        dts.append(dt)
+    out = {"dts": np.array(dts), "iters": np.array(its)}
+    for fid in st.field_ids():
+        out[f"f{fid}"] = st.field(fid)
+    save("bubble3d_semi.npz", **out)
+
+
+def g_liquid_gas():
+    cfg, mesh, st, eos, rs = liquid_gas_tube(200)
+    s = po.CpuSolver(mesh, cfg, rs.kind(), eos, "ref")
+    dts = []
+    for k in range(60):
+        cfg.step = k
+        dts.append(s.explicit_step(st, cfg))
+    out = {"dts": np.array(dts)}
+    for fid in st.field_ids():
+        out[f"f{fid}"] = st.field(fid)
+    save("liquid_gas_1d.npz", **out)
+
+
+if __name__ == "__main__":
+    assert po.available("ref"), "build oracle/_ref first (make -C oracle)"
+    g_recon()
+    g_riemann()
+    g_sod()
+    g_vortex()
+    g_ghost()
+    g_jacobi()
+    g_semi()
+    g_liquid_gas()

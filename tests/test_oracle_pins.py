@@ -1,0 +1,321 @@
+"""Pins of the CPU oracle (oracle/sifv_oracle.c).
+
+1. Against the reference's own published data: examples/1D_advection/
+   explicitError.csv (LF + WENO5 + SSP-RK3, CFL 0.8) is reproduced to all
+   printed digits.
+2. Against golden fixtures written by the compiled reference
+   (tests/golden/make_golden.py): reconstruction and Riemann-flux KATs,
+   Sod-1024 to t=0.2, vortex 64^2, ghost fills (bitwise), Jacobi, the 3D
+   two-phase semi-implicit step and the two-phase explicit tube.
+3. Against mathematics: polynomial exactness of the reconstructions, mirror
+   symmetry, flux consistency F(U,U)=F(U), HLLC contact preservation,
+   Jacobi fixed point = direct solve, discrete conservation.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from semiimplicitfv_amd import (BoundaryCondition, IdealGasEOS, LFSolver, ReconstructionOrder, SimulationConfig,
+                                SolutionState, cases, uniform_mesh)
+from semiimplicitfv_amd.cases import _fill
+from semiimplicitfv_amd.timeloop import run_time_loop
+from tests.golden import problems
+from tests.helpers import make_case, rel_errs
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(G, name))
+
+
+# --------------------------------------------------------------------------
+# 1. published reference data
+# --------------------------------------------------------------------------
+CSV = {128: (1.262842e-04, 3.006748e-04, 1.278738e-03), 256: (6.437110e-06, 1.750551e-05, 7.790240e-05)}
+
+
+def advection_errors(po, N, impl="c"):
+    """examples/1D_advection/1D_advection.cpp:36-220 with useSemiImplicit=false."""
+    cfg = SimulationConfig(dim=1, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5)
+    cfg.explicitParams.cfl = 0.8
+    cfg.explicitParams.maxDt = 1e-2
+    cfg.validate()
+    mesh = uniform_mesh(cfg, N, 0.0, 1.0, periods=(1, 0, 0))
+    st = SolutionState()
+    st.allocate(mesh.totalCells(), cfg)
+    eos = IdealGasEOS(1.4, 287.0, cfg)
+    xc, _, _ = mesh.centroids()
+    dx = xc - 0.5
+    dx = dx - 1.0 * np.round(dx / 1.0)
+    rho = 1.225 * (1.0 + 0.01 * np.exp(-(dx * dx) / (0.05 * 0.05)))
+    _fill(mesh, st, eos, rho.reshape(1, 1, N), [np.full((1, 1, N), 10.0)], np.full((1, 1, N), 101325.0))
+    rho0 = mesh.interior(st.rho).copy()
+    s = po.CpuSolver(mesh, cfg, LFSolver(eos).kind(), eos, impl)
+    T = 0.1
+    run_time_loop(cfg, lambda tdt: s.explicit_step(st, cfg, tdt), T, T / 100.0)
+    s.cons_to_prim(st, cfg)
+    d = mesh.interior(st.rho) - rho0
+    return np.abs(d).sum() / N, math.sqrt((d * d).sum() / N), np.abs(d).max()
+
+
+@pytest.mark.parametrize("N", [128, 256])
+def test_published_advection_errors(oracle_c, N):
+    e = advection_errors(oracle_c, N)
+    for got, want in zip(e, CSV[N]):
+        assert float(f"{got:.6e}") == want, (got, want)
+
+
+# --------------------------------------------------------------------------
+# 2. golden fixtures from the compiled reference
+# --------------------------------------------------------------------------
+def _face_states(po, order, v, nx, ng):
+    R = 3 if order in (2, 5) else 2
+    L = np.zeros(nx + 1)
+    Rt = np.zeros(nx + 1)
+    for f in range(nx + 1):
+        c = f + ng
+        L[f] = po.recon_c(order, 0, v[c - R:c + R - 1], 1e-6)
+        Rt[f] = po.recon_c(order, 1, v[c - R + 1:c + R], 1e-6)
+    return L, Rt
+
+
+@pytest.mark.parametrize("order", [1, 2, 4, 5])
+def test_recon_kat(oracle_c, order):
+    d = load("recon_kat.npz")
+    nx, ng, _ = problems.recon_lines(order)
+    for q, v in enumerate(d[f"v{order}"]):
+        L, R = _face_states(oracle_c, order, v, nx, ng)
+        scale = np.abs(v).max()
+        assert np.abs(L - d[f"L{order}"][q]).max() <= 1e-14 * scale
+        assert np.abs(R - d[f"R{order}"][q]).max() <= 1e-14 * scale
+
+
+def test_riemann_kat(oracle_c):
+    d = load("riemann_kat.npz")
+    for key, (Ls, Rs, meta) in problems.riemann_pairs().items():
+        rtype, dr, dim, incP, igr, nph = meta
+        F = d[key + "_F"]
+        for q in range(len(Ls)):
+            got = oracle_c.riemann_flux("c", rtype, d[key + "_L"][q], d[key + "_R"][q], dr, dim, incP, igr, nph)
+            scale = np.abs(F[q]).max()
+            assert np.abs(got - F[q]).max() <= 1e-13 * scale, (key, q)
+
+
+def test_sod_golden(oracle_c):
+    d = load("sod1024_final.npz")
+    cfg, mesh, st, eos, rs = cases.sod_1d(1024)
+    s = oracle_c.CpuSolver(mesh, cfg, rs.kind(), eos, "c")
+    dts = []
+    t = 0.0
+    while t < 0.2:
+        dts.append(s.explicit_step(st, cfg, 0.2 - t))
+        t += dts[-1]
+        cfg.step += 1
+    assert len(dts) == len(d["dts"]) == 748
+    # the last dt is endTime - sum(dt): compare on the time scale
+    assert np.abs(np.array(dts) - d["dts"]).max() <= 1e-15
+    for name in ("rho", "rhoU", "rhoE", "velU", "pres"):
+        l1, li = rel_errs(getattr(st, name), d[name])
+        assert l1 <= 1e-13 and li <= 1e-12, (name, l1, li)
+
+
+def test_vortex_golden(oracle_c):
+    d = load("vortex64_50steps.npz")
+    cfg, mesh, st, eos, rs = cases.isentropic_vortex_2d(64)
+    s = oracle_c.CpuSolver(mesh, cfg, rs.kind(), eos, "c")
+    for k in range(50):
+        cfg.step = k
+        assert abs(s.explicit_step(st, cfg) - d["dts"][k]) <= 1e-14 * d["dts"][k]
+    for name in ("rho", "rhoU", "rhoV", "rhoE"):
+        l1, li = rel_errs(mesh.interior(getattr(st, name)), mesh.interior(d[name]))
+        assert l1 <= 1e-13 and li <= 1e-13, (name, l1, li)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_ghost_fill_golden_bitwise(oracle_c, dim):
+    d = load("ghost_fill.npz")
+    cfg, mesh, st, bcs, periods = problems.random_ghost_case(dim)
+    for fid in st.field_ids():
+        assert np.array_equal(st.field(fid), d[f"d{dim}_in_f{fid}"])
+    s = oracle_c.CpuSolver(mesh, cfg, 2, IdealGasEOS(1.4, 1.0, cfg), "c")
+    for vs in (0, 1, 2):
+        t = st.copy()
+        s.fill_ghosts(t, vs)
+        for fid in t.field_ids():
+            assert np.array_equal(t.field(fid), d[f"d{dim}_vs{vs}_f{fid}"]), (vs, fid)
+    a = st.pres.copy()
+    s.fill_scalar(a)
+    assert np.array_equal(a, d[f"d{dim}_scalar"])
+
+
+def test_jacobi_golden(oracle_c):
+    d = load("jacobi_3d.npz")
+    cfg, mesh, rho, rhoc2, rhs, p0, dt = problems.jacobi_problem()
+    assert np.array_equal(rho, d["rho"]) and np.array_equal(rhs, d["rhs"])
+    s = oracle_c.CpuSolver(mesh, cfg, 2, IdealGasEOS(1.4, 1.0, cfg), "c")
+    for tol, mi in ((1e-6, 1000), (1e-300, 7)):
+        p = p0.copy()
+        it = s.jacobi(rho, rhoc2, rhs, p, dt, tol, mi)
+        assert it == int(d[f"it_tol{tol:g}"][0])
+        l1, li = rel_errs(mesh.interior(p), mesh.interior(d[f"p_tol{tol:g}"]))
+        assert l1 <= 1e-14 and li <= 1e-14
+
+
+def test_semi_implicit_bubble_golden(oracle_c):
+    d = load("bubble3d_semi.npz")
+    cfg, mesh, st, eos, rs = problems.bubble_3d(12, perturb=True)
+    s = oracle_c.CpuSolver(mesh, cfg, rs.kind(), eos, "c")
+    for k in range(3):
+        cfg.step = k
+        dt, it = s.semi_step(st, cfg)
+        assert it == d["iters"][k] and dt == d["dts"][k]
+    for fid in st.field_ids():
+        l1, li = rel_errs(mesh.interior(st.field(fid)), mesh.interior(d[f"f{fid}"]))
+        assert l1 <= 1e-11 and li <= 1e-10, (fid, l1, li)
+
+
+def test_liquid_gas_golden(oracle_c):
+    d = load("liquid_gas_1d.npz")
+    cfg, mesh, st, eos, rs = problems.liquid_gas_tube(200)
+    s = oracle_c.CpuSolver(mesh, cfg, rs.kind(), eos, "c")
+    for k in range(60):
+        cfg.step = k
+        assert abs(s.explicit_step(st, cfg) - d["dts"][k]) <= 1e-13 * d["dts"][k]
+    for fid in st.field_ids():
+        l1, li = rel_errs(mesh.interior(st.field(fid)), mesh.interior(d[f"f{fid}"]))
+        assert l1 <= 1e-12 and li <= 1e-11, (fid, l1, li)
+
+
+# --------------------------------------------------------------------------
+# 3. mathematics
+# --------------------------------------------------------------------------
+def _cell_averages(coef, edges):
+    """Exact cell averages of the polynomial sum coef[k] x^k."""
+    P = np.polynomial.polynomial.Polynomial(coef).integ()
+    return (P(edges[1:]) - P(edges[:-1])) / np.diff(edges)
+
+
+@pytest.mark.parametrize("order,deg", [(2, 2), (1, 1), (5, 4), (4, 2)])
+def test_recon_polynomial_exactness(oracle_c, order, deg):
+    """WENO5 candidates are exact for quadratics (so is any convex mix),
+    WENO3 for linears, UPWIND5 for quartics, UPWIND3 for quadratics."""
+    rng = np.random.default_rng(order)
+    R = 3 if order in (2, 5) else 2
+    for _ in range(20):
+        coef = rng.standard_normal(deg + 1)
+        h = 0.1
+        edges = np.arange(-R, R) * h
+        edges = np.append(edges, R * h)
+        v = _cell_averages(coef, edges)
+        face = np.polynomial.polynomial.polyval  # left face of cell 0 is x=0; right face is x=h
+        left_state = oracle_c.recon_c(order, 0, v[:2 * R - 1], 1e-6)  # stencil centred on cell -1 -> face x=0
+        right_state = oracle_c.recon_c(order, 1, v[1:2 * R], 1e-6)  # stencil centred on cell 0 -> face x=0
+        exact = face(0.0, coef)
+        assert abs(left_state - exact) <= 1e-11 * (1 + abs(exact))
+        assert abs(right_state - exact) <= 1e-11 * (1 + abs(exact))
+
+
+def test_weno5_mirror_symmetry(oracle_c):
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        v = rng.standard_normal(5) * 10 ** rng.uniform(-3, 3)
+        a = oracle_c.recon_c(2, 1, v, 1e-6)
+        b = oracle_c.recon_c(2, 0, v[::-1].copy(), 1e-6)
+        assert abs(a - b) <= 1e-14 * np.abs(v).max()
+
+
+@pytest.mark.parametrize("rtype", [0, 1, 2])
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_flux_consistency(oracle_c, rtype, dim):
+    """F(U, U) is the physical Euler flux along the face normal."""
+    rng = np.random.default_rng(rtype * 3 + dim)
+    for dr in range(dim):
+        for _ in range(20):
+            w = np.zeros(16)
+            w[0] = 10 ** rng.uniform(-1, 1)
+            w[1:1 + dim] = rng.uniform(-3, 3, dim)
+            w[4] = 10 ** rng.uniform(-1, 1)
+            w[6] = 1.4
+            F = oracle_c.riemann_flux("c", rtype, w, w, dr, dim, 1, 0, 0)
+            rho, u, p = w[0], w[1:4], w[4]
+            un = u[dr]
+            E = p / 0.4 + 0.5 * rho * (u * u).sum()
+            want = [rho * un] + [rho * u[i] * un + (p if i == dr else 0.0) for i in range(3)] + [(E + p) * un]
+            want[1 + dim:4] = [0.0] * (3 - dim)
+            assert np.allclose(F[:5], want, rtol=1e-13, atol=1e-13 * max(1.0, np.abs(want).max()))
+            assert F[14] == pytest.approx(un, rel=1e-15, abs=1e-15)
+
+
+def test_hllc_stationary_contact(oracle_c):
+    """HLLC resolves a stationary contact exactly: no mass/energy flux."""
+    L = np.zeros(16)
+    R = np.zeros(16)
+    L[0], L[4], L[6] = 2.0, 1.0, 1.4
+    R[0], R[4], R[6] = 0.3, 1.0, 1.4
+    F = oracle_c.riemann_flux("c", 2, L, R, 0, 1, 1, 0, 0)
+    assert F[0] == 0.0 and F[4] == 0.0 and F[1] == 1.0
+
+
+def test_jacobi_fixed_point_is_direct_solve(oracle_c):
+    """The converged Jacobi iterate solves (I + dt^2 rho c^2 L) p = rhs with
+    L the variable-coefficient face Laplacian assembled here independently."""
+    cfg = SimulationConfig(dim=2, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5, semiImplicit=True)
+    rng = np.random.default_rng(2)
+    xn = np.concatenate([[0.0], np.cumsum(0.5 + rng.random(6))])
+    yn = np.concatenate([[0.0], np.cumsum(0.5 + rng.random(5))])
+    from semiimplicitfv_amd import RectilinearMesh
+    mesh = RectilinearMesh(cfg, xn, yn, periods=(0, 1, 0))
+    nx, ny = mesh.nx(), mesh.ny()
+    N = mesh.totalCells()
+    rho = np.zeros(N)
+    mesh.interior(rho)[...] = rng.uniform(0.5, 3.0, (1, ny, nx))
+    rho = problems._fill_scalar_np(mesh, rho)
+    rhoc2 = np.zeros(N)
+    mesh.interior(rhoc2)[...] = rng.uniform(1.0, 2.0, (1, ny, nx))
+    rhs = np.zeros(N)
+    mesh.interior(rhs)[...] = rng.standard_normal((1, ny, nx))
+    dt = 0.3
+    p = np.zeros(N)
+    s = oracle_c.CpuSolver(mesh, cfg, 2, IdealGasEOS(1.4, 1.0, cfg), "c")
+    it = s.jacobi(rho, rhoc2, rhs, p, dt, 1e-13, 100000)
+    assert it < 100000
+    # independent assembly: cells (i,j), x faces zero-gradient, y periodic
+    A = np.zeros((nx * ny, nx * ny))
+    b = np.zeros(nx * ny)
+    R = rho.reshape(mesh.shape3())[0]
+    gx, gy = mesh.ngx(), mesh.ngy()
+    for j in range(ny):
+        for i in range(nx):
+            r = j * nx + i
+            c2 = rhoc2.reshape(mesh.shape3())[0, gy + j, gx + i] * dt * dt
+            A[r, r] += 1.0
+            b[r] = rhs.reshape(mesh.shape3())[0, gy + j, gx + i]
+            for axis, (di, dj) in enumerate(((1, 0), (0, 1))):
+                for sgn in (-1, 1):
+                    ni, nj = i + sgn * di, j + sgn * dj
+                    h = mesh.dx(i) if axis == 0 else mesh.dy(j)
+                    hn = mesh.dx(ni) if axis == 0 else mesh.dy(nj)
+                    rf = 0.5 * (R[gy + j, gx + i] + R[gy + nj, gx + ni])
+                    cf = 1.0 / (rf * 0.5 * (h + hn) * h)
+                    A[r, r] += c2 * cf
+                    # neighbour column: zero-gradient in x, periodic in y
+                    ci = min(max(ni, 0), nx - 1)
+                    cj = nj % ny
+                    A[r, cj * nx + ci] -= c2 * cf
+    x = np.linalg.solve(A, b)
+    got = mesh.interior(p)[0].reshape(-1)
+    assert np.abs(got - x).max() <= 1e-10 * np.abs(x).max()
+
+
+def test_explicit_conservation_periodic(oracle_c):
+    cfg, mesh, st, eos = make_case(2, (24, 20), periods=(1, 1, 0), seed=1)
+    s = oracle_c.CpuSolver(mesh, cfg, 2, eos, "c")
+    tot0 = [mesh.interior(st.field(f)).sum() for f in (0, 1, 2, 4)]
+    for k in range(10):
+        s.explicit_step(st, cfg)
+    tot1 = [mesh.interior(st.field(f)).sum() for f in (0, 1, 2, 4)]
+    for a, b in zip(tot0, tot1):
+        assert abs(a - b) <= 1e-13 * max(1.0, abs(a))
