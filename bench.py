@@ -44,7 +44,27 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--rows", type=int, default=0, help="rows marched per wave (0 = built-in choice)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib: int, reps: int = 5):
+    """Plain device-to-device copies of a known size through the HIP runtime's
+    copy kernel: the reference point for scaling FETCH_SIZE / WRITE_SIZE."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": n, "reps": reps}))
 
 
 def run_gpu(n, steps, warmup, precision, rows=0):
@@ -74,6 +94,20 @@ def run_gpu(n, steps, warmup, precision, rows=0):
     g.close()
     return {"seconds": el, "kernel_ms": kms, "launches": launches, "rho": rho, "finite": finite,
             "cells": n * n, "dt_sum": t_sim}
+
+
+def measured_traffic(n, precision):
+    """HBM bytes per stage launch from the committed rocprofv3 PMC summary of
+    this workload (FETCH_SIZE + WRITE_SIZE scaled by a copy-kernel
+    calibration), or None when no summary exists for it."""
+    name = f"stage_xy_weno5_hllc_{precision}_vortex{n}_pmc.md"
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline():
@@ -121,6 +155,9 @@ def cpu_baseline():
 
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
@@ -149,7 +186,7 @@ def main():
         "config": {"workload": f"2D isentropic vortex {a.n}x{a.n}, periodic [0,10]^2, WENO5+HLLC+SSP-RK3, CFL 0.6",
                    "cells": cells, "precision": a.precision},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(a.n, a.precision),
                      "kernel": "stage_xy_kernel<2,3,HLLC>", "bytes_per_launch": bytes_per_launch,
                      "mean_launch_ms": mean_launch_s * 1e3, "launches": res["launches"]},
         "finite": res["finite"],
