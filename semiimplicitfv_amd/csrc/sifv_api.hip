@@ -17,7 +17,8 @@
 
 #include "../../include/sifv_gpu.h"
 #include "sifv_aux.hpp"
-#include "sifv_stage.hpp"
+#include "sifv_kernels.hpp"
+#include "sifv_semi.hpp"
 
 using namespace sifv;
 
@@ -150,10 +151,16 @@ struct sifv_handle {
     bool primFresh = true;
     int rowsOverride = 0;
     Timing timing;
-    // Jacobi / semi-implicit scratch (allocated on first use)
+    // 3D: x+y part of the right-hand side (n_fc fields)
+    void* part = nullptr;
+    int nfc = 0;
+    // Jacobi / semi-implicit scratch (allocated on first use):
+    // [0],[1] pressure ping-pong, [2] rho c^2, [3] rhs, [4] advected pressure, [5] density copy
     double* jac[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double* resid = nullptr;
     int residCap = 0;
+    double* hostResid = nullptr;
+    int lastIters = 0;
 
     size_t tsize() const { return precision == SIFV_MIXED ? sizeof(float) : sizeof(double); }
     Grid grid() const {
@@ -199,23 +206,6 @@ int ensure_partials(sifv_handle* h, int n) {
 }
 
 // ---- stage launch ---------------------------------------------------------
-template <typename T>
-using StageKernel = void (*)(StageParams<T>);
-
-template <typename T>
-StageKernel<T> pick_stage(int dim, int R, int rs) {
-#define SIFV_K(D, RR, S) \
-    if (dim == D && R == RR && rs == S) return &stage_xy_kernel<D, RR, S, T>;
-    SIFV_K(1, 1, 0) SIFV_K(1, 1, 1) SIFV_K(1, 1, 2)
-    SIFV_K(1, 2, 0) SIFV_K(1, 2, 1) SIFV_K(1, 2, 2)
-    SIFV_K(1, 3, 0) SIFV_K(1, 3, 1) SIFV_K(1, 3, 2)
-    SIFV_K(2, 1, 0) SIFV_K(2, 1, 1) SIFV_K(2, 1, 2)
-    SIFV_K(2, 2, 0) SIFV_K(2, 2, 1) SIFV_K(2, 2, 2)
-    SIFV_K(2, 3, 0) SIFV_K(2, 3, 1) SIFV_K(2, 3, 2)
-#undef SIFV_K
-    return nullptr;
-}
-
 int recon_radius(int order) { return required_ghosts(order); }
 
 void stage_shape(const sifv_handle* h, int& nstrip, int& nseg, int& rows) {
@@ -225,13 +215,14 @@ void stage_shape(const sifv_handle* h, int& nstrip, int& nseg, int& rows) {
         rows = 1;
         return;
     }
+    long planes = h->dim == 3 ? h->nz : 1;
     if (h->rowsOverride > 0) {
         rows = h->rowsOverride;
     } else {
         // aim for >= ~8k waves so every CU holds several waves, but keep at
         // least 8 rows per wave to amortise the 2-row warm-up
         long target = 8192;
-        long r = (long)h->ny * nstrip / target;
+        long r = (long)h->ny * nstrip * planes / target;
         rows = 8;
         while (rows < 64 && rows * 2 <= r) rows *= 2;
     }
@@ -239,61 +230,117 @@ void stage_shape(const sifv_handle* h, int& nstrip, int& nseg, int& rows) {
     nseg = (h->ny + rows - 1) / rows;
 }
 
-template <typename T>
-int launch_stage(sifv_handle* h, int src, int u0, int dst, const double* coef, bool rk1, bool last,
-                 const int* active) {
-    StageParams<T> P;
-    P.g = h->grid();
-    P.eos = h->eosd();
-    for (int q = 0; q < 5; ++q) {
-        int qq = q < h->nc ? q : 0;
-        P.src[q] = h->consField<T>(src, qq);
-        P.u0[q] = h->consField<T>(u0 >= 0 ? u0 : src, qq);
-        P.dst[q] = h->consField<T>(dst, qq);
-    }
-    P.sigma = h->sigma;
-    P.c1 = coef[0]; P.c2 = coef[1]; P.c3 = coef[2]; P.c4 = coef[3];
-    P.rk1 = rk1 ? 1 : 0;
-    P.dt = h->scal;
-    P.active = active;
-    P.eps = h->cfg.wenoEps;
-    P.incP = h->cfg.semiImplicit ? 0 : 1;
-    P.igr = h->cfg.useIGR ? 1 : 0;
-    P.weno = (h->cfg.reconOrder <= SIFV_WENO5) ? 1 : 0;
-    int nstrip, nseg, rows;
-    stage_shape(h, nstrip, nseg, rows);
-    P.rows = rows;
-    P.nseg = nseg;
-    int nb = nstrip * nseg;
-    bool wantDt = last && h->cfg.explicitParams.constDt <= 0.0;
-    if (wantDt) {
-        int rc = ensure_partials(h, nb);
-        if (rc) return rc;
-        P.dtPartial = h->partial;
+void zpass_shape(const sifv_handle* h, int& nstrip, int& nseg, int& rows) {
+    nstrip = (h->nx + kWave - 1) / kWave;
+    if (h->rowsOverride > 0) {
+        rows = h->rowsOverride;
     } else {
-        P.dtPartial = nullptr;
+        long target = 8192;
+        long r = (long)h->nz * nstrip * h->ny / target;
+        rows = 8;
+        while (rows < 64 && rows * 2 <= r) rows *= 2;
     }
-    StageKernel<T> k = pick_stage<T>(h->dim, recon_radius(h->cfg.reconOrder), h->riemann);
-    if (!k) return fail(SIFV_ERR_UNSUPPORTED, "no stage kernel for this configuration");
+    rows = std::min(rows, h->nz);
+    nseg = (h->nz + rows - 1) / rows;
+}
+
+template <typename T>
+StageKernels<T> stage_kernels(const sifv_handle* h, bool semi);
+
+template <>
+StageKernels<double> stage_kernels<double>(const sifv_handle* h, bool semi) {
+    int R = recon_radius(h->cfg.reconOrder);
+    if (h->dim == 1) return stage_kernels_1d(R, h->riemann, h->nph, semi);
+    if (h->dim == 2) return stage_kernels_2d(R, h->riemann, h->nph, semi);
+    return semi ? stage_kernels_3d_semi(R, h->riemann, h->nph) : stage_kernels_3d(R, h->riemann, h->nph);
+}
+
+template <>
+StageKernels<float> stage_kernels<float>(const sifv_handle* h, bool semi) {
+    if (semi || h->nph) return StageKernels<float>();
+    return stage_kernels_fp32(h->dim, recon_radius(h->cfg.reconOrder), h->riemann);
+}
+
+int timed_launch(sifv_handle* h, const void* fn, dim3 grid, dim3 block, void** args) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (h->timing.on) {
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, h->stream));
     }
-    hipLaunchKernelGGL(k, dim3(nb), dim3(kWave), 0, h->stream, P);
-    HIPCHK(hipGetLastError());
+    HIPCHK(hipLaunchKernel(fn, grid, block, args, 0, h->stream));
     if (h->timing.on) {
         HIPCHK(hipEventRecord(e1, h->stream));
         h->timing.pending.push_back({e0, e1});
     }
-    if (wantDt) h->npartialValid = nb;
     return SIFV_OK;
 }
 
+// One RK stage of the fused kernel(s).  semi: advective fluxes + star update
+// (writes pres / vel / pAdv) instead of the explicit update.
 template <typename T>
-int launch_stage3d(sifv_handle* h, int src, int u0, int dst, const double* coef, bool rk1, bool last,
-                   const int* active);
+int launch_stage(sifv_handle* h, int src, int u0, int dst, const double* coef, bool rk1, bool last,
+                 const int* active, bool semi) {
+    StageParams<T> P;
+    P.g = h->grid();
+    P.eos = h->eosd();
+    P.mix = h->mix();
+    P.total = h->total;
+    P.src = static_cast<const T*>(h->cons[src]);
+    P.u0 = static_cast<const T*>(h->cons[u0 >= 0 ? u0 : src]);
+    P.dst = static_cast<T*>(h->cons[dst]);
+    P.part = static_cast<T*>(h->part);
+    P.sigma = h->sigma;
+    P.c1 = coef[0]; P.c2 = coef[1]; P.c3 = coef[2]; P.c4 = coef[3];
+    P.rk1 = rk1 ? 1 : 0;
+    P.dt = h->scal;
+    P.active = active;
+    P.eps = h->cfg.wenoEps;
+    P.igr = h->cfg.useIGR ? 1 : 0;
+    P.weno = (h->cfg.reconOrder <= SIFV_WENO5) ? 1 : 0;
+    P.pres = h->pres;
+    for (int d = 0; d < 3; ++d) P.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
+    P.pAdv = h->jac[4];
+    StageKernels<T> k = stage_kernels<T>(h, semi);
+    if (!k.xy || (h->dim == 3 && !k.z)) return fail(SIFV_ERR_UNSUPPORTED, "no stage kernel for this configuration");
+    const bool wantDt = last && !semi && h->cfg.explicitParams.constDt <= 0.0;
+    int nstrip, nseg, rows;
+    stage_shape(h, nstrip, nseg, rows);
+    P.rows = rows;
+    P.nseg = nseg;
+    P.nstrip = nstrip;
+    int nb = nstrip * nseg * (h->dim == 3 ? h->nz : 1);
+    P.dtPartial = nullptr;
+    if (wantDt && h->dim < 3) {
+        int rc = ensure_partials(h, nb);
+        if (rc) return rc;
+        P.dtPartial = h->partial;
+    }
+    void* args[] = {&P};
+    int rc = timed_launch(h, reinterpret_cast<const void*>(k.xy), dim3(nb), dim3(kWave), args);
+    if (rc) return rc;
+    if (wantDt && h->dim < 3) h->npartialValid = nb;
+    if (h->dim == 3) {
+        StageParams<T> Z = P;
+        int zs, zn, zr;
+        zpass_shape(h, zs, zn, zr);
+        Z.rows = zr;
+        Z.nseg = zn;
+        Z.nstrip = zs;
+        int nbz = zs * zn * h->ny;
+        Z.dtPartial = nullptr;
+        if (wantDt) {
+            rc = ensure_partials(h, nbz);
+            if (rc) return rc;
+            Z.dtPartial = h->partial;
+        }
+        void* zargs[] = {&Z};
+        rc = timed_launch(h, reinterpret_cast<const void*>(k.z), dim3(nbz), dim3(kWave), zargs);
+        if (rc) return rc;
+        if (wantDt) h->npartialValid = nbz;
+    }
+    return SIFV_OK;
+}
 
 int harvest_timing(sifv_handle* h) {
     for (auto& pr : h->timing.pending) {
@@ -315,12 +362,12 @@ const double kRK[3][3][4] = {
     {{1, 0, 1, 1}, {1, 3, 1, 4}, {2, 1, 2, 3}},
 };
 
-int launch_dt_cells(sifv_handle* h, bool advective) {
+int launch_dt_cells(sifv_handle* h, bool advective, const double* rho = nullptr) {
     DtArgs A;
     A.g = h->grid();
     A.eos = h->eosd();
     A.mix = h->mix();
-    A.rho = h->rhoP;
+    A.rho = rho ? rho : h->rhoP;
     for (int d = 0; d < 3; ++d) A.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
     A.pres = h->pres;
     for (int p = 0; p < kMaxPhases; ++p) {
@@ -395,8 +442,18 @@ int prim_field_set(sifv_handle* h, double** f, int* comp) {
 
 int refresh_prim(sifv_handle* h) {
     if (h->primFresh) return SIFV_OK;
-    int rc = h->precision == SIFV_MIXED ? launch_c2p<float>(h) : launch_c2p<double>(h);
-    if (rc) return rc;
+    int rc;
+    if (h->cfg.semiImplicit) {
+        // the semi-implicit step leaves the solved pressure and the corrected
+        // velocity in the primitive arrays (no closing conversion,
+        // src/SemiImplicitSolver.cpp:216-233); only the density copy and the
+        // PRIM ghost fill are outstanding
+        HIPCHK(hipMemcpyAsync(h->rhoP, h->consField<double>(h->cur, 0), h->total * sizeof(double),
+                              hipMemcpyDeviceToDevice, h->stream));
+    } else {
+        rc = h->precision == SIFV_MIXED ? launch_c2p<float>(h) : launch_c2p<double>(h);
+        if (rc) return rc;
+    }
     double* f[kMaxFill];
     int comp[kMaxFill];
     int nf = prim_field_set(h, f, comp);
@@ -454,13 +511,9 @@ int enqueue_explicit_step(sifv_handle* h, double targetDt, bool loop, double end
     const int* active = loop ? h->flags : nullptr;
     auto stage = [&](int s, int src, int u0, int dst) {
         bool last = s == ord - 1;
-        if (h->dim == 3)
-            return h->precision == SIFV_MIXED
-                       ? launch_stage3d<float>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active)
-                       : launch_stage3d<double>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active);
         return h->precision == SIFV_MIXED
-                   ? launch_stage<float>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active)
-                   : launch_stage<double>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active);
+                   ? launch_stage<float>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active, false)
+                   : launch_stage<double>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active, false);
     };
     if (ord == 1) {
         rc = stage(0, A, -1, B);
@@ -481,6 +534,169 @@ int enqueue_explicit_step(sifv_handle* h, double targetDt, bool loop, double end
     return SIFV_OK;
 }
 
+int elementwise_blocks(const sifv_handle* h) {
+    size_t n = (size_t)h->nx * h->ny * h->nz;
+    return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192));
+}
+
+int ensure_jacobi(sifv_handle* h, int maxIter) {
+    for (int q = 0; q < 6; ++q) {
+        if (h->jac[q]) continue;
+        HIPCHK(hipMalloc(&h->jac[q], sizeof(double) * h->total));
+        HIPCHK(hipMemset(h->jac[q], 0, sizeof(double) * h->total));
+    }
+    if (maxIter > h->residCap) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        hipFree(h->resid);
+        if (h->hostResid) hipHostFree(h->hostResid);
+        h->resid = nullptr;
+        h->hostResid = nullptr;
+        h->residCap = 0;
+        HIPCHK(hipMalloc(&h->resid, sizeof(double) * maxIter));
+        HIPCHK(hipHostMalloc(&h->hostResid, sizeof(double) * maxIter, hipHostMallocDefault));
+        h->residCap = maxIter;
+    }
+    return SIFV_OK;
+}
+
+int launch_copy_interior(sifv_handle* h, const double* src, double* dst) {
+    hipLaunchKernelGGL(copy_interior_kernel, dim3(elementwise_blocks(h)), dim3(256), 0, h->stream, h->grid(), src,
+                       dst);
+    HIPCHK(hipGetLastError());
+    return SIFV_OK;
+}
+
+// Jacobi sweeps from h->jac[0] (iterate 0) ping-ponging through jac[0]/jac[1]
+// until max |p_new - p| < tol or maxIter sweeps (JacobiPressureSolver::solve,
+// src/JacobiPressureSolver.cpp:57-108).  The sweeps are enqueued in chunks;
+// between chunks the host reads the per-sweep residuals, and a sweep whose
+// predecessor converged returns at once, so a chunk may overshoot harmlessly.
+// *n = sweeps done (the reference's iteration count); iterate k lives in
+// jac[k % 2] and h->hostResid[0..n) holds the residual history.
+int run_jacobi(sifv_handle* h, const double* rho, const double* dt, double tol, int maxIter, int* n) {
+    JacobiArgs J;
+    J.g = h->grid();
+    J.dim = h->dim;
+    J.rho = rho;
+    J.rhoc2 = h->jac[2];
+    J.rhs = h->jac[3];
+    J.resid = h->resid;
+    J.tol = tol;
+    J.dt = dt;
+    const int nb = elementwise_blocks(h);
+    int it = 0, seen = 0, chunk = 4;
+    *n = maxIter;
+    while (it < maxIter) {
+        const int end = std::min(maxIter, it + chunk);
+        for (; it < end; ++it) {
+            J.iter = it;
+            J.pin = h->jac[it % 2];
+            J.pout = h->jac[(it + 1) % 2];
+            void* args[] = {&J};
+            int rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_kernel), dim3(nb), dim3(256), args);
+            if (rc) return rc;
+        }
+        HIPCHK(hipMemcpyAsync(h->hostResid + seen, h->resid + seen, sizeof(double) * (end - seen),
+                              hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int t = seen; t < end; ++t)
+            if (h->hostResid[t] < tol) {
+                *n = t + 1;
+                return SIFV_OK;
+            }
+        seen = end;
+        chunk = std::min(chunk * 2, 64);
+    }
+    return SIFV_OK;
+}
+
+// One semi-implicit step (SemiImplicitSolver::step, src/SemiImplicitSolver.cpp:106-236):
+// advective dt, then per RK stage the fused advective stage kernel (star state,
+// advected pressure), PRIM ghosts of the star density / velocity, the pressure
+// system set-up, Jacobi sweeps and the correction.
+int enqueue_semi_step(sifv_handle* h, double targetDt) {
+    const sifv_config& c = h->cfg;
+    const auto& sp = c.semiImplicitParams;
+    int rc = ensure_jacobi(h, sp.maxPressureIters);
+    if (rc) return rc;
+    rc = launch_dt_cells(h, true, h->consField<double>(h->cur, 0));
+    if (!rc) rc = launch_finalize(h, sp.cfl, sp.maxDt, sp.minDt, -1.0, targetDt, true, false, 0.0);
+    if (rc) return rc;
+    const int ord = c.RKOrder;
+    const int A = h->cur, B = (h->cur + 1) % 3, C = (h->cur + 2) % 3;
+    int src[3], u0[3], dst[3];
+    if (ord == 1) { src[0] = A; u0[0] = -1; dst[0] = B; }
+    else if (ord == 2) { src[0] = A; u0[0] = -1; dst[0] = B; src[1] = B; u0[1] = A; dst[1] = A; }
+    else {
+        src[0] = A; u0[0] = -1; dst[0] = B;
+        src[1] = B; u0[1] = A; dst[1] = C;
+        src[2] = C; u0[2] = A; dst[2] = A;
+    }
+    const int nb = elementwise_blocks(h);
+    for (int s = 0; s < ord; ++s) {
+        rc = launch_stage<double>(h, src[s], u0[s], dst[s], kRK[ord - 1][s], ord == 1, false, nullptr, true);
+        if (rc) return rc;
+        double* rho = h->consField<double>(dst[s], 0);
+        double* f[4];
+        int comp[4];
+        int nf = 0;
+        f[nf] = rho; comp[nf++] = 0;
+        for (int d = 0; d < h->dim; ++d) { f[nf] = h->vel[d]; comp[nf++] = 1 + d; }
+        rc = fill_dirs(h, f, comp, nf, false);
+        if (rc) return rc;
+        SemiArgs S;
+        S.g = h->grid();
+        S.eos = h->eosd();
+        S.mix = h->mix();
+        S.dim = h->dim;
+        S.nph = h->nph;
+        S.total = h->total;
+        S.rho = rho;
+        for (int ph = 0; ph < kMaxPhases; ++ph) {
+            int pp = ph < h->nph ? ph : 0;
+            S.ar[ph] = h->nph ? h->consField<double>(dst[s], h->dim + 2 + pp) : nullptr;
+            S.al[ph] = h->nph ? h->consField<double>(dst[s], h->dim + 2 + h->nph + pp) : nullptr;
+        }
+        S.pres = h->pres;
+        for (int d = 0; d < 3; ++d) {
+            S.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
+            S.velOut[d] = h->vel[d] ? h->vel[d] : h->vel[0];
+        }
+        S.pAdv = h->jac[4];
+        S.rhoc2 = h->jac[2];
+        S.rhs = h->jac[3];
+        S.p0 = h->jac[0];
+        S.resid = h->resid;
+        S.maxIter = sp.maxPressureIters;
+        S.dt = h->scal;
+        hipLaunchKernelGGL(semi_setup_kernel, dim3(nb), dim3(256), 0, h->stream, S);
+        HIPCHK(hipGetLastError());
+        int n = 0;
+        rc = run_jacobi(h, rho, h->scal, sp.pressureTol, sp.maxPressureIters, &n);
+        if (rc) return rc;
+        h->lastIters = n;
+        CorrectArgs K;
+        K.g = h->grid();
+        K.eos = h->eosd();
+        K.mix = h->mix();
+        K.dim = h->dim;
+        K.nph = h->nph;
+        K.total = h->total;
+        K.cons = h->consField<double>(dst[s], 0);
+        K.p = h->jac[n % 2];
+        K.sigma = h->sigma;
+        for (int d = 0; d < 3; ++d) K.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
+        K.pres = h->pres;
+        K.dt = h->scal;
+        hipLaunchKernelGGL(semi_correct_kernel, dim3(nb), dim3(256), 0, h->stream, K);
+        HIPCHK(hipGetLastError());
+    }
+    h->cur = dst[ord - 1];
+    h->dtNextValid = false;
+    h->primFresh = false;
+    return SIFV_OK;
+}
+
 int check_step_config(sifv_handle* h, const sifv_config* cfg) {
     if (cfg) {
         // per-call fields of SimulationConfig (step/time, and parameters the
@@ -491,13 +707,6 @@ int check_step_config(sifv_handle* h, const sifv_config* cfg) {
     return SIFV_OK;
 }
 
-}  // namespace
-
-namespace {
-template <typename T>
-int launch_stage3d(sifv_handle*, int, int, int, const double*, bool, bool, const int*) {
-    return fail(SIFV_ERR_UNSUPPORTED, "3D stage kernel not available yet");
-}
 }  // namespace
 
 // ===========================================================================
@@ -525,11 +734,11 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
     if (eos->kind != SIFV_EOS_IDEAL && eos->kind != SIFV_EOS_STIFFENED) return fail(SIFV_ERR_INVALID, "unknown EOS");
     for (int f = 0; f < 6; ++f)
         if (mesh->bc[f] < 0 || mesh->bc[f] > 4) return fail(SIFV_ERR_INVALID, "unknown boundary condition");
-    if (cfg->multiPhaseParams.nPhases > 0)
-        return fail(SIFV_ERR_UNSUPPORTED, "multiphase states are not on the explicit GPU path yet");
+    if (cfg->multiPhaseParams.nPhases != 0 && cfg->multiPhaseParams.nPhases != 2)
+        return fail(SIFV_ERR_UNSUPPORTED, "the GPU path carries single-phase or two-phase states");
     if (cfg->useIGR) return fail(SIFV_ERR_UNSUPPORTED, "IGR is not on the GPU path yet");
-    if (precision == SIFV_MIXED && cfg->semiImplicit)
-        return fail(SIFV_ERR_UNSUPPORTED, "mixed precision is available for the explicit path only");
+    if (precision == SIFV_MIXED && (cfg->semiImplicit || cfg->multiPhaseParams.nPhases > 0))
+        return fail(SIFV_ERR_UNSUPPORTED, "mixed precision is available for the single-phase explicit path only");
 
     auto* h = new sifv_handle();
     h->dim = mesh->dim;
@@ -614,6 +823,10 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
     CK(hipMalloc(&h->flags, sizeof(int) * 4));
     CK(hipMemset(h->flags, 0, sizeof(int) * 4));
     CK(hipHostMalloc(&h->hostPinned, sizeof(double) * 4, hipHostMallocDefault));
+    if (h->dim == 3) {
+        h->nfc = h->dim + 2 + 2 * h->nph + 2;
+        CK(hipMalloc(&h->part, h->tsize() * h->total * h->nfc));
+    }
 #undef CK
     h->dtNextValid = false;
     h->primFresh = true;
@@ -639,8 +852,10 @@ int sifv_destroy(sifv_handle* h) {
     hipFree(h->partial);
     hipFree(h->scal);
     hipFree(h->flags);
+    hipFree(h->part);
     for (int q = 0; q < 6; ++q) hipFree(h->jac[q]);
     hipFree(h->resid);
+    if (h->hostResid) hipHostFree(h->hostResid);
     if (h->hostPinned) hipHostFree(h->hostPinned);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
@@ -774,11 +989,10 @@ int sifv_fill_ghosts(sifv_handle* h, int varset) {
     if (!h) return fail(SIFV_ERR_INVALID, "null handle");
     if (h->precision != SIFV_FP64) return fail(SIFV_ERR_UNSUPPORTED, "ghost fill API needs an FP64 handle");
     if (varset < SIFV_VARSET_CONS || varset > SIFV_VARSET_ALL) return fail(SIFV_ERR_INVALID, "unknown varset");
-    // The single-phase explicit path keeps rho in the conservative buffer;
-    // sync the primitive-side copy first so both sides hold the same density.
-    if (h->nph == 0)
-        HIPCHK(hipMemcpyAsync(h->rhoP, h->consField<double>(h->cur, 0), h->total * sizeof(double),
-                              hipMemcpyDeviceToDevice, h->stream));
+    // The conservative buffer holds the authoritative density; sync the
+    // primitive-side copy first so both sides hold the same values.
+    HIPCHK(hipMemcpyAsync(h->rhoP, h->consField<double>(h->cur, 0), h->total * sizeof(double),
+                          hipMemcpyDeviceToDevice, h->stream));
     double* f[kMaxFill];
     int comp[kMaxFill];
     int n = 0;
@@ -793,9 +1007,8 @@ int sifv_fill_ghosts(sifv_handle* h, int varset) {
     for (int q = h->dim + 2; q < h->nc; ++q) { f[n] = h->consField<double>(h->cur, q); comp[n++] = 0; }
     int rc = fill_dirs(h, f, comp, n, false, varset == SIFV_VARSET_CONS ? h->sigma : nullptr);
     if (rc) return rc;
-    if (h->nph == 0)
-        HIPCHK(hipMemcpyAsync(h->consField<double>(h->cur, 0), h->rhoP, h->total * sizeof(double),
-                              hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->consField<double>(h->cur, 0), h->rhoP, h->total * sizeof(double),
+                          hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return SIFV_OK;
 }
@@ -881,16 +1094,57 @@ int sifv_set_tile(sifv_handle* h, int rows_per_wave) {
 
 int sifv_semi_implicit_step(sifv_handle* h, const sifv_config* cfg, double targetDt, double* dt_out,
                             int* pressure_iters) {
-    (void)h; (void)cfg; (void)targetDt; (void)dt_out; (void)pressure_iters;
-    return fail(SIFV_ERR_UNSUPPORTED, "semi-implicit step not available yet");
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    if (!h->cfg.semiImplicit) return fail(SIFV_ERR_INVALID, "handle was created for the explicit solver");
+    check_step_config(h, cfg);
+    int rc = enqueue_semi_step(h, targetDt);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->hostPinned, h->scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    rc = harvest_timing(h);
+    if (rc) return rc;
+    if (dt_out) *dt_out = h->hostPinned[0];
+    if (pressure_iters) *pressure_iters = h->lastIters;
+    if (!std::isfinite(h->hostPinned[0])) return fail(SIFV_ERR_NONFINITE, "non-finite time step");
+    return SIFV_OK;
 }
 
 int sifv_jacobi_solve(sifv_handle* h, const double* rho, const double* rhoc2, const double* rhs,
                       double* pressure, double dt, double tolerance, int maxIter, int* iters,
                       double* residual_history) {
-    (void)h; (void)rho; (void)rhoc2; (void)rhs; (void)pressure; (void)dt; (void)tolerance;
-    (void)maxIter; (void)iters; (void)residual_history;
-    return fail(SIFV_ERR_UNSUPPORTED, "Jacobi solver not available yet");
+    if (!h || !rho || !rhoc2 || !rhs || !pressure) return fail(SIFV_ERR_INVALID, "null argument");
+    if (maxIter <= 0) return fail(SIFV_ERR_INVALID, "maxIter must be > 0");
+    if (!(tolerance > 0.0)) return fail(SIFV_ERR_INVALID, "tolerance must be > 0");
+    int rc = ensure_jacobi(h, maxIter);
+    if (rc) return rc;
+    const size_t nb = h->total * sizeof(double);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(h->jac[5], rho, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->jac[2], rhoc2, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->jac[3], rhs, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->jac[0], pressure, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->scal + 3, &dt, sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(h->resid, 0, sizeof(double) * maxIter, h->stream));
+    int n = 0;
+    rc = run_jacobi(h, h->jac[5], h->scal + 3, tolerance, maxIter, &n);
+    if (rc) return rc;
+    // the reference's array leaves the loop holding the scalar-ghost fill of
+    // the penultimate iterate around the interior of the final one
+    // (src/JacobiPressureSolver.cpp:64-104)
+    double* prev = h->jac[(n - 1) % 2];
+    double* fin = h->jac[n % 2];
+    double* f[1] = {prev};
+    int comp[1] = {0};
+    rc = fill_dirs(h, f, comp, 1, true);
+    if (rc) return rc;
+    rc = launch_copy_interior(h, fin, prev);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(pressure, prev, nb, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (iters) *iters = n;
+    if (residual_history)
+        for (int t = 0; t < n; ++t) residual_history[t] = h->hostResid[t];
+    return SIFV_OK;
 }
 
 }  // extern "C"

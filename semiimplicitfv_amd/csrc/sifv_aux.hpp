@@ -1,7 +1,7 @@
 // sifv_aux.hpp - time-step reductions, cons->prim and ghost-fill kernels.
 #pragma once
 
-#include "sifv_device.hpp"
+#include "sifv_stage.hpp"
 
 namespace sifv {
 
@@ -30,26 +30,6 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
     for (int q = 0; q < nw; ++q) m = (m < sh[q]) ? sh[q] : m;
     __syncthreads();
     return m;
-}
-
-// Mixture parameters (MultiPhaseParams, include/SimulationConfig.hpp:43-52).
-struct Mix {
-    int nph;
-    double gamma[kMaxPhases], pInf[kMaxPhases];
-    double alphaMin;
-};
-
-// Wood sound speed, MixtureEOS::mixtureSoundSpeed (src/MixtureEOS.cpp:48-63).
-__device__ __forceinline__ double mix_sound(const Mix& m, double rho, double p, const double* a,
-                                            const double* ar) {
-    double s = 0.0;
-    for (int ph = 0; ph < m.nph; ++ph) {
-        double rk = dmax_ref(ar[ph], 1e-14) / dmax_ref(a[ph], 1e-14);
-        double ck2 = m.gamma[ph] * (p + m.pInf[ph]) / rk;
-        s += a[ph] / (rk * dmax_ref(ck2, 1e-14));
-    }
-    double c2 = 1.0 / (rho * dmax_ref(s, 1e-30));
-    return sqrt(dmax_ref(c2, 0.0));
 }
 
 // Per-block minimum of the acoustic (advective when `advective`) cell time
@@ -93,7 +73,7 @@ __global__ void __launch_bounds__(kRedThreads) dt_cells_kernel(DtArgs A) {
                     a[ph] = A.alpha[ph][id];
                     ar[ph] = A.alphaRho[ph][id];
                 }
-                c = mix_sound(A.mix, A.rho[id], A.pres[id], a, ar);
+                c = mix_sound(A.mix, A.rho[id], A.pres[id], a, ar, A.mix.nph);
             } else {
                 double pinf = A.eos.stiffened ? A.eos.pInf : 0.0;
                 c = sqrt(A.eos.gamma * dmax_ref(A.pres[id] + pinf, 1e-14) / dmax_ref(A.rho[id], 1e-14));
@@ -171,7 +151,7 @@ struct C2PArgs {
     Eos eos;
     Mix mix;
     const T* cons[5 + 2 * kMaxPhases];  // rho, mom[dim], E, alphaRho[nph], alpha[nph]
-    double* rho;                         // written in the multiphase form only
+    double* rho;                         // SolutionState::rho seen by the primitive side
     double* vel[3];
     double* pres;
 };

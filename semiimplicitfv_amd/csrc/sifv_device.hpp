@@ -185,43 +185,49 @@ struct Recon<3, T> {
 
 // ---------------------------------------------------------------------------
 // Riemann fluxes along axis D of a DIM-dimensional problem.  Face states are
-// primitive (rho, u[DIM], p) with gammaEff/piInfEff (constant per face for a
-// single phase).  Formulas: computeHLLCFlux (src/HLLCSolver.cpp:7-161),
-// computeRusanovFlux (src/RusanovSolver.cpp:7-83), computeLFFlux
-// (src/LFSolver.cpp:8-75), helpers include/RiemannSolver.hpp:58-78.
+// primitive (rho, u[DIM], p, sigma) with their effective gamma / piInf (the
+// EOS constants for a single phase, the mixture values of the reconstructed
+// volume fractions otherwise, MixtureEOS::effectiveGammaAndPiInf) and NPH
+// reconstructed volume fractions.  Formulas: computeHLLCFlux
+// (src/HLLCSolver.cpp:7-161), computeRusanovFlux (src/RusanovSolver.cpp:7-83),
+// computeLFFlux (src/LFSolver.cpp:8-75), helpers include/RiemannSolver.hpp:58-78.
 // Output F[0]=mass, F[1..DIM]=momentum, F[DIM+1]=energy, F[DIM+2]=pressure
-// flux, F[DIM+3]=face velocity.
+// flux, F[DIM+3]=face velocity, F[DIM+4+ph]=volume-fraction flux.
 // ---------------------------------------------------------------------------
-template <int DIM, typename T>
+template <int DIM, int NPH, typename T>
 struct FaceState {
-    T rho, u[DIM], p, sig;
+    T rho, u[DIM], p, sig, g, pi;
+    T a[NPH > 0 ? NPH : 1];
 };
 
-template <int DIM, typename T>
-__device__ __forceinline__ T sound_direct(const FaceState<DIM, T>& w, T g, T pi) {
-    return sqrt(g * tmax(w.p + pi, T(1e-14)) / tmax(w.rho, T(1e-14)));
+template <int NPH>
+constexpr int flux_len(int dim) { return dim + 4 + NPH; }
+
+template <int DIM, int NPH, typename T>
+__device__ __forceinline__ T sound_direct(const FaceState<DIM, NPH, T>& w) {
+    return sqrt(w.g * tmax(w.p + w.pi, T(1e-14)) / tmax(w.rho, T(1e-14)));
 }
 
-template <int DIM, typename T>
-__device__ __forceinline__ T rhoE_state(const FaceState<DIM, T>& w, T g, T pi) {
+template <int DIM, int NPH, typename T>
+__device__ __forceinline__ T rhoE_state(const FaceState<DIM, NPH, T>& w) {
     T q = w.u[0] * w.u[0];
 #pragma unroll
     for (int d = 1; d < DIM; ++d) q += w.u[d] * w.u[d];
     T ke = T(0.5) * w.rho * q;
-    return (w.p + g * pi) / (g - T(1.0)) + ke;
+    return (w.p + w.g * w.pi) / (w.g - T(1.0)) + ke;
 }
 
 enum { RS_LF = 0, RS_RUSANOV = 1, RS_HLLC = 2 };
 
-template <int RS, int D, int DIM, typename T>
-__device__ __forceinline__ void riemann(const FaceState<DIM, T>& L, const FaceState<DIM, T>& R, T g,
-                                        T pi, bool incP, bool igr, T* F) {
+template <int RS, int D, int DIM, int NPH, typename T>
+__device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const FaceState<DIM, NPH, T>& R,
+                                        bool incP, bool igr, T* F) {
     T uL = L.u[D], uR = R.u[D];
     if (RS == RS_HLLC) {
-        T EL = rhoE_state(L, g, pi), ER = rhoE_state(R, g, pi);
+        T EL = rhoE_state(L), ER = rhoE_state(R);
         T sL, sR, sS;
         if (incP) {
-            T cL = sound_direct(L, g, pi), cR = sound_direct(R, g, pi);
+            T cL = sound_direct(L), cR = sound_direct(R);
             sL = tmin(uL - cL, uR - cR);
             sR = tmax(uL + cL, uR + cR);
             sS = (R.p - L.p + L.rho * uL * (sL - uL) - R.rho * uR * (sR - uR)) /
@@ -232,9 +238,10 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, T>& L, const FaceSt
             sS = T(0.5) * (uL + uR);
         }
         if (sL >= T(0) || sR <= T(0)) {
-            const FaceState<DIM, T>& W = (sL >= T(0)) ? L : R;
-            T un = (sL >= T(0)) ? uL : uR;
-            T E = (sL >= T(0)) ? EL : ER;
+            const bool lft = sL >= T(0);
+            const FaceState<DIM, NPH, T>& W = lft ? L : R;
+            T un = lft ? uL : uR;
+            T E = lft ? EL : ER;
             F[0] = W.rho * un;
 #pragma unroll
             for (int i = 0; i < DIM; ++i) F[1 + i] = W.rho * W.u[i] * un;
@@ -245,9 +252,11 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, T>& L, const FaceSt
             }
             F[DIM + 2] = W.p * un;
             F[DIM + 3] = un;
+#pragma unroll
+            for (int ph = 0; ph < NPH; ++ph) F[DIM + 4 + ph] = W.a[ph] * un;
         } else {
             const bool left = sS >= T(0);
-            const FaceState<DIM, T>& W = left ? L : R;
+            const FaceState<DIM, NPH, T>& W = left ? L : R;
             T un = left ? uL : uR;
             T s = left ? sL : sR;
             T E = left ? EL : ER;
@@ -271,15 +280,17 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, T>& L, const FaceSt
             }
             F[DIM + 2] = W.p * sS;
             F[DIM + 3] = sS;
+#pragma unroll
+            for (int ph = 0; ph < NPH; ++ph) F[DIM + 4 + ph] = W.a[ph] * sS;
         }
     } else {
         // Rusanov and LF share the central + dissipation form.
-        T EL = rhoE_state(L, g, pi), ER = rhoE_state(R, g, pi);
+        T EL = rhoE_state(L), ER = rhoE_state(R);
         T S;
         if (RS == RS_RUSANOV) {
             T aL = fabs(uL), aR = fabs(uR);
             if (incP) {
-                T cL = sound_direct(L, g, pi), cR = sound_direct(R, g, pi);
+                T cL = sound_direct(L), cR = sound_direct(R);
                 S = tmax(aL + cL, aR + cR);
             } else {
                 S = tmax(aL, aR);
@@ -311,7 +322,7 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, T>& L, const FaceSt
             qL = sqrt(qL);
             qR = sqrt(qR);
             if (incP) {
-                T cL = sound_direct(L, g, pi), cR = sound_direct(R, g, pi);
+                T cL = sound_direct(L), cR = sound_direct(R);
                 S = tmax(qL, qR) + tmax(cL, cR);
             } else {
                 S = tmax(qL, qR);
@@ -333,6 +344,9 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, T>& L, const FaceSt
         }
         F[DIM + 3] = T(0.5) * (uL + uR);
         F[DIM + 2] = T(0.5) * (L.p * uL + R.p * uR) - T(0.5) * S * (R.p - L.p);
+#pragma unroll
+        for (int ph = 0; ph < NPH; ++ph)
+            F[DIM + 4 + ph] = T(0.5) * (L.a[ph] * uL + R.a[ph] * uR) - T(0.5) * S * (R.a[ph] - L.a[ph]);
     }
 }
 

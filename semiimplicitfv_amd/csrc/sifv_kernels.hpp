@@ -1,0 +1,28 @@
+// sifv_kernels.hpp - dispatch tables of the instantiated stage kernels.
+// Instantiations are spread over several translation units
+// (sifv_kernels_*.hip) so they compile in parallel.
+#pragma once
+
+#include "sifv_stage.hpp"
+
+namespace sifv {
+
+template <typename T>
+using StageFn = void (*)(StageParams<T>);
+
+// 1D/2D: xy is the full stage kernel and z is null; 3D: xy writes the x+y
+// part of the right-hand side and z completes the stage.
+template <typename T>
+struct StageKernels {
+    StageFn<T> xy = nullptr;
+    StageFn<T> z = nullptr;
+};
+
+// R = reconstruction radius (1, 2, 3), rs = Riemann solver (RS_*), nph = 0 or 2.
+StageKernels<double> stage_kernels_1d(int R, int rs, int nph, bool semi);
+StageKernels<double> stage_kernels_2d(int R, int rs, int nph, bool semi);
+StageKernels<double> stage_kernels_3d(int R, int rs, int nph);
+StageKernels<double> stage_kernels_3d_semi(int R, int rs, int nph);
+StageKernels<float> stage_kernels_fp32(int dim, int R, int rs);
+
+}  // namespace sifv

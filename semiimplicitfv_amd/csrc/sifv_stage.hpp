@@ -1,48 +1,84 @@
 // sifv_stage.hpp - fused cons->prim + ghost mapping + reconstruction +
-// Riemann flux + flux divergence + SSP-RK stage update for 1D and 2D grids.
+// Riemann flux + flux divergence + RK stage update.
 //
 // Replaces, per RK stage, the reference's chain
 //   convertConservativeToPrimitiveVariables (src/SolutionState.cpp:187-205)
+//     / MixtureEOS::convertConservativeToPrimitive (src/MixtureEOS.cpp:99-143)
 //   -> applyBoundaryConditions(PRIM)          (src/RectilinearMesh.cpp:498-524)
-//   -> Reconstructor::reconstructX/Y          (src/Reconstruction.cpp:204-432)
-//   -> computeRHS face loops                   (src/ExplicitSolver.cpp:270-419)
-//   -> stage update                            (src/ExplicitSolver.cpp:127-188)
-// with one kernel that reads each conservative value once from HBM.
+//   -> Reconstructor::reconstructX/Y/Z        (src/Reconstruction.cpp:204-541)
+//   -> computeRHS face loops and sources      (src/ExplicitSolver.cpp:270-496,
+//                                              src/SemiImplicitSolver.cpp:238-482)
+//   -> stage update                            (src/ExplicitSolver.cpp:127-188,
+//                                              src/SemiImplicitSolver.cpp:159-224)
 //
-// Work decomposition (wave64): one wave owns a strip of 62 consecutive
-// columns plus one halo column on each side (lane l <-> column X0-1+l) and
-// marches upward through `rows` consecutive rows.  The y-stencil of each
-// lane's column lives in a register window that slides one row per
-// iteration; the x-stencil is exchanged through LDS.  Each lane reconstructs
-// its own cell once per direction (both face states from one stencil), so
-// every face flux is computed exactly once per strip.  Ghost cells are never
-// materialised: stencil indices outside the domain are mapped to their
-// source cells with the boundary rule and velocity sign flips applied.
+// Work decomposition (wave64).  1D/2D (`stage_xy_kernel`, MODE_FULL): one
+// wave owns a strip of 62 consecutive columns plus one halo column on each
+// side (lane l <-> column X0-1+l) and marches upward through `rows` rows.
+// The y-stencil of each lane's column lives in a register window that slides
+// one row per iteration; the x-stencil is exchanged through LDS.  Each lane
+// reconstructs its own cell once per direction (both face states from one
+// stencil), so every face flux is computed once per strip.
+// 3D: the same kernel runs on each z-plane (MODE_XY) and writes the x+y part
+// of the right-hand side; `stage_z_kernel` then marches 64-column strips in z
+// with a z-window, adds the z part in the reference's accumulation order
+// (x, then y, then z) and applies the stage update.
+// Ghost cells are never materialised: stencil indices outside the domain are
+// mapped to their source cells with the boundary rule and velocity sign
+// flips applied.
 #pragma once
 
 #include "sifv_device.hpp"
 
 namespace sifv {
 
-constexpr int kOwn = 62;  // owned columns per wave
+constexpr int kOwn = 62;  // owned columns per wave in the x-marching kernels
+
+enum { MODE_FULL = 0, MODE_XY = 1 };
+
+// Mixture parameters (MultiPhaseParams, include/SimulationConfig.hpp:43-52).
+struct Mix {
+    int nph;
+    double gamma[kMaxPhases], pInf[kMaxPhases];
+    double alphaMin;
+};
+
+// Wood sound speed, MixtureEOS::mixtureSoundSpeed (src/MixtureEOS.cpp:48-63).
+__device__ __forceinline__ double mix_sound(const Mix& m, double rho, double p, const double* a,
+                                            const double* ar, int nph) {
+    double s = 0.0;
+    for (int ph = 0; ph < nph; ++ph) {
+        double rk = dmax_ref(ar[ph], 1e-14) / dmax_ref(a[ph], 1e-14);
+        double ck2 = m.gamma[ph] * (p + m.pInf[ph]) / rk;
+        s += a[ph] / (rk * dmax_ref(ck2, 1e-14));
+    }
+    double c2 = 1.0 / (rho * dmax_ref(s, 1e-30));
+    return sqrt(dmax_ref(c2, 0.0));
+}
 
 template <typename T>
 struct StageParams {
     Grid g;
     Eos eos;
-    const T* src[5];      // stencil source: rho, mom[DIM], E
-    const T* u0[5];       // RK backup, read pointwise (unused when c2 == 0)
-    T* dst[5];            // stage output
-    const double* sigma;  // entropic pressure (LF with IGR only)
+    Mix mix;
+    size_t total;         // padded cells per field
+    const T* src;         // stencil source buffer (field q at q*total)
+    const T* u0;          // RK backup buffer, read pointwise when c2 != 0
+    T* dst;               // stage output buffer
+    T* part;              // 3D: x+y part of the right-hand side (NCV+2 fields)
+    const double* sigma;  // entropic pressure (LF with IGR)
     double c1, c2, c3, c4;
-    int rk1;              // RKOrder == 1: U += dt * R
+    int rk1;              // RKOrder == 1 form (U += dt R)
     const double* dt;     // device-resident time step
     const int* active;    // device flag; 0 skips the stage (time loop finished)
-    double* dtPartial;    // per-block min of min_d dx_d/(|u_d|+c) of the new state, or null
+    double* dtPartial;    // per-block min of the new state's acoustic cell dt, or null
     double eps;
-    int incP, igr, weno;
-    int rows;             // rows marched per wave
-    int nseg;             // y segments per strip
+    int igr, weno;
+    int rows;             // rows (y) or planes (z) marched per wave
+    int nseg;             // segments per strip
+    int nstrip;           // strips per row / plane
+    double* pres;         // semi-implicit: stage-start pressure (written)
+    double* vel[3];       // semi-implicit: star velocity (written)
+    double* pAdv;         // semi-implicit: advected pressure (written)
 };
 
 // Logical block id with all blocks of one XCD contiguous (blocks are
@@ -55,287 +91,574 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return (x < rem) ? x * (per + 1) + q : rem * (per + 1) + (x - rem) * per + q;
 }
 
-// Load one cell's conservative state (through the ghost mapping) and return
-// primitive (rho, u[DIM], p) in FP64.
-template <int DIM, typename T>
-__device__ __forceinline__ void load_prim(const StageParams<T>& P, int i, int j, double* w) {
-    const Grid& g = P.g;
-    bool fnx = false, ftx = false, fny = false, fty = false;
-    int ci = i < -g.ngx ? -g.ngx : (i > g.nx - 1 + g.ngx ? g.nx - 1 + g.ngx : i);
-    int si = map_axis(ci, g.nx, g.bc[0], g.bc[1], fnx, ftx);
-    int sj = 0;
-    if (DIM >= 2) {
-        int cj = j < -g.ngy ? -g.ngy : (j > g.ny - 1 + g.ngy ? g.ny - 1 + g.ngy : j);
-        sj = map_axis(cj, g.ny, g.bc[2], g.bc[3], fny, fty);
+// Primitive state of one cell (FP64) plus its partial densities.
+template <int DIM, int NPH>
+struct Cell {
+    double rho, u[DIM], p;
+    double a[NPH > 0 ? NPH : 1];
+    double ar[NPH > 0 ? NPH : 1];
+};
+
+__device__ __forceinline__ int clampi(int c, int lo, int hi) { return c < lo ? lo : (c > hi ? hi : c); }
+
+// cons -> prim of the conservative values of one cell.  Single phase:
+// toPrimitive of the ideal/stiffened gas (rho passed through unclamped, as
+// the reconstruction reads SolutionState::rho).  Multiphase:
+// MixtureEOS::convertConservativeToPrimitive (rho = max(sum alphaRho, 1e-14)).
+template <int DIM, int NPH>
+__device__ __forceinline__ void prim_of(const Eos& e, const Mix& m, const double* U, Cell<DIM, NPH>& c) {
+    if constexpr (NPH == 0) {
+        double u[DIM], mom[DIM], p;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) mom[d] = U[1 + d];
+        cons_to_prim<DIM>(e, U[0], mom, U[DIM + 1], u, p);
+        c.rho = U[0];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) c.u[d] = u[d];
+        c.p = p;
+    } else {
+        double r = 0.0;
+#pragma unroll
+        for (int ph = 0; ph < NPH; ++ph) r += U[DIM + 2 + ph];
+        double rs = dmax_ref(r, 1e-14);
+        c.rho = rs;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) c.u[d] = U[1 + d] / rs;
+        double ke = 0.5 * rs * c.u[0] * c.u[0];
+#pragma unroll
+        for (int d = 1; d < DIM; ++d) ke += 0.5 * rs * c.u[d] * c.u[d];
+        double sInv = 0.0, sPi = 0.0;
+#pragma unroll
+        for (int ph = 0; ph < NPH; ++ph) {
+            double a = U[DIM + 2 + NPH + ph];
+            double gm1 = m.gamma[ph] - 1.0;
+            sInv += a / gm1;
+            sPi += a * m.gamma[ph] * m.pInf[ph] / gm1;
+            c.a[ph] = a;
+            c.ar[ph] = U[DIM + 2 + ph];
+        }
+        c.p = ((U[DIM + 1] - ke) - sPi) / sInv;
     }
-    size_t id = cell_index(g, si, sj, 0);
-    double rho = (double)P.src[0][id];
-    double mom[DIM];
+}
+
+// Load one cell (i, j, k) through the ghost mapping (with velocity flips).
+template <int DIM, int NPH, typename T>
+__device__ __forceinline__ void load_cell(const StageParams<T>& P, const T* buf, int i, int j, int k,
+                                          Cell<DIM, NPH>& c) {
+    constexpr int NCV = DIM + 2 + 2 * NPH;
+    const Grid& g = P.g;
+    bool fn[3] = {false, false, false}, ft[3] = {false, false, false};
+    int si = map_axis(clampi(i, -g.ngx, g.nx - 1 + g.ngx), g.nx, g.bc[0], g.bc[1], fn[0], ft[0]);
+    int sj = 0, sk = 0;
+    if (DIM >= 2) sj = map_axis(clampi(j, -g.ngy, g.ny - 1 + g.ngy), g.ny, g.bc[2], g.bc[3], fn[1], ft[1]);
+    if (DIM >= 3) sk = map_axis(clampi(k, -g.ngz, g.nz - 1 + g.ngz), g.nz, g.bc[4], g.bc[5], fn[2], ft[2]);
+    size_t id = cell_index(g, si, sj, sk);
+    double U[NCV];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) mom[d] = (double)P.src[1 + d][id];
-    double E = (double)P.src[DIM + 1][id];
-    double u[DIM], p;
-    cons_to_prim<DIM>(P.eos, rho, mom, E, u, p);
-    w[0] = rho;
-    // velocity sign flips of wall rules: component d flips with the normal
-    // flag of axis d and the tangential flags of the other axes
-    bool fl[2] = {fnx != fty, ftx != fny};
+    for (int q = 0; q < NCV; ++q) U[q] = (double)buf[(size_t)q * P.total + id];
+    prim_of<DIM, NPH>(P.eos, P.mix, U, c);
+    // velocity component d flips with the normal flag of axis d and the
+    // tangential flags of the other axes (onion-peel composition)
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) w[1 + d] = fl[d] ? -u[d] : u[d];
-    w[DIM + 1] = p;
+    for (int d = 0; d < DIM; ++d) {
+        bool f = fn[d];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e)
+            if (e != d) f = f != ft[e];
+        if (f) c.u[d] = -c.u[d];
+    }
 }
 
 template <int DIM, typename T>
-__device__ __forceinline__ double load_sigma(const StageParams<T>& P, int i, int j) {
+__device__ __forceinline__ double load_sigma(const StageParams<T>& P, int i, int j, int k) {
     const Grid& g = P.g;
-    int ci = i < -g.ngx ? -g.ngx : (i > g.nx - 1 + g.ngx ? g.nx - 1 + g.ngx : i);
     // sigma ghosts are refreshed by the IGR solve's scalar fill
-    int si = map_axis_scalar(ci, g.nx, g.sbc[0], g.sbc[1]);
-    int cj = j < -g.ngy ? -g.ngy : (j > g.ny - 1 + g.ngy ? g.ny - 1 + g.ngy : j);
-    int sj = DIM >= 2 ? map_axis_scalar(cj, g.ny, g.sbc[2], g.sbc[3]) : 0;
-    return P.sigma[cell_index(g, si, sj, 0)];
+    int si = map_axis_scalar(clampi(i, -g.ngx, g.nx - 1 + g.ngx), g.nx, g.sbc[0], g.sbc[1]);
+    int sj = DIM >= 2 ? map_axis_scalar(clampi(j, -g.ngy, g.ny - 1 + g.ngy), g.ny, g.sbc[2], g.sbc[3]) : 0;
+    int sk = DIM >= 3 ? map_axis_scalar(clampi(k, -g.ngz, g.nz - 1 + g.ngz), g.nz, g.sbc[4], g.sbc[5]) : 0;
+    return P.sigma[cell_index(g, si, sj, sk)];
 }
 
-template <int DIM, int R, int RS, typename T>
+// Reconstructed variables of a cell: rho, u[DIM], p, alpha[NPH].
+template <int DIM, int NPH>
+constexpr int n_recon() { return DIM + 2 + NPH; }
+
+template <int DIM, int NPH, typename T>
+__device__ __forceinline__ void recon_vars(const Cell<DIM, NPH>& c, T* v) {
+    v[0] = (T)c.rho;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) v[1 + d] = (T)c.u[d];
+    v[DIM + 1] = (T)c.p;
+#pragma unroll
+    for (int ph = 0; ph < NPH; ++ph) v[DIM + 2 + ph] = (T)c.a[ph];
+}
+
+// Face state from reconstructed values; effective gamma/piInf from the
+// reconstructed volume fractions (MixtureEOS::effectiveGammaAndPiInf,
+// src/MixtureEOS.cpp:18-31) or the single-phase EOS constants
+// (src/Reconstruction.cpp:309-314).
+template <int DIM, int NPH, typename T>
+__device__ __forceinline__ void face_state(const StageParams<T>& P, const T* v, T sig,
+                                           FaceState<DIM, NPH, T>& w) {
+    w.rho = v[0];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) w.u[d] = v[1 + d];
+    w.p = v[DIM + 1];
+    w.sig = sig;
+    if constexpr (NPH == 0) {
+        w.g = (T)P.eos.gamma;
+        w.pi = (T)(P.eos.stiffened ? P.eos.pInf : 0.0);
+    } else {
+        T sInv = T(0), sPi = T(0);
+#pragma unroll
+        for (int ph = 0; ph < NPH; ++ph) {
+            w.a[ph] = v[DIM + 2 + ph];
+            T gk = (T)P.mix.gamma[ph];
+            T gm1 = gk - T(1.0);
+            sInv += w.a[ph] / gm1;
+            sPi += w.a[ph] * gk * (T)P.mix.pInf[ph] / gm1;
+        }
+        w.g = T(1.0) + T(1.0) / sInv;
+        w.pi = (w.g - T(1.0)) / w.g * sPi;
+    }
+}
+
+// Face contribution vector: [mass, mom[DIM], energy, alphaRho flux[NPH],
+// alpha flux[NPH], pressure flux, face velocity] (conservative order, then
+// the advected-pressure flux and the divergence velocity).
+template <int DIM, int NPH>
+constexpr int n_fc() { return DIM + 2 + 2 * NPH + 2; }
+
+template <int DIM, int NPH, typename T>
+__device__ __forceinline__ void face_contrib(const T* F, double rhoL, double rhoR, const double* arL,
+                                             const double* arR, T* fc) {
+    fc[0] = F[0];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) fc[1 + d] = F[1 + d];
+    fc[DIM + 1] = F[DIM + 1];
+    if constexpr (NPH > 0) {
+        // upwind partial-density flux (src/ExplicitSolver.cpp:307-326)
+        const bool up = F[0] >= T(0);
+        double ru = dmax_ref(up ? rhoL : rhoR, 1e-14);
+#pragma unroll
+        for (int ph = 0; ph < NPH; ++ph) fc[DIM + 2 + ph] = (T)(((up ? arL[ph] : arR[ph]) / ru) * (double)F[0]);
+#pragma unroll
+        for (int ph = 0; ph < NPH; ++ph) fc[DIM + 2 + NPH + ph] = F[DIM + 4 + ph];
+    }
+    fc[DIM + 2 + 2 * NPH] = F[DIM + 2];
+    fc[DIM + 3 + 2 * NPH] = F[DIM + 3];
+}
+
+// acc += c*lo - c*hi over conservative + pressure slots; the divergence slot
+// takes -c*lo + c*hi (src/ExplicitSolver.cpp:312-351: divU[L] += c v, divU[R] -= c v).
+template <int DIM, int NPH, typename T>
+__device__ __forceinline__ void accumulate(T* acc, const T* lo, const T* hi, T c, bool first) {
+    constexpr int NF = n_fc<DIM, NPH>();
+#pragma unroll
+    for (int q = 0; q < NF - 1; ++q) {
+        T a = first ? c * lo[q] : acc[q] + c * lo[q];
+        acc[q] = a - c * hi[q];
+    }
+    T d = first ? -(c * lo[NF - 1]) : acc[NF - 1] - c * lo[NF - 1];
+    acc[NF - 1] = d + c * hi[NF - 1];
+}
+
+// Sources and stage update of one owned cell; returns the acoustic cell dt
+// of the new state when requested.  acc holds the complete flux part.
+template <int DIM, int NPH, bool SEMI, typename T>
+__device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id, T* acc, const Cell<DIM, NPH>& cell,
+                                              const double* h, double dt, bool wantDt) {
+    constexpr int NCV = DIM + 2 + 2 * NPH;
+    constexpr int QP = NCV, QD = NCV + 1;
+    // sources: alpha div(u) (src/ExplicitSolver.cpp:486-496), p div(u)
+    // (src/SemiImplicitSolver.cpp:472-482)
+#pragma unroll
+    for (int ph = 0; ph < NPH; ++ph) acc[DIM + 2 + NPH + ph] = acc[DIM + 2 + NPH + ph] + (T)cell.a[ph] * acc[QD];
+    if constexpr (SEMI) acc[QP] = acc[QP] + (T)cell.p * acc[QD];
+    double un[NCV];
+    double U0v[NCV];
+    const bool use0 = P.c2 != 0.0;
+#pragma unroll
+    for (int q = 0; q < NCV; ++q) U0v[q] = use0 ? (double)P.u0[(size_t)q * P.total + id] : 0.0;
+#pragma unroll
+    for (int q = 0; q < NCV; ++q) {
+        double U = (double)P.src[(size_t)q * P.total + id];
+        double Rq = (double)acc[q];
+        if (SEMI && q == DIM + 1) {
+            un[q] = U;  // total energy is rebuilt by the correction step
+            continue;
+        }
+        if (P.rk1 && (!SEMI || q >= DIM + 2)) {
+            un[q] = U + dt * Rq;
+        } else {
+            double a = P.c1 * U;
+            if (use0) a = a + P.c2 * U0v[q];
+            un[q] = (a + (P.c3 * dt) * Rq) / P.c4;
+        }
+    }
+    if constexpr (NPH > 0) {
+        // post-update clamp / normalisation (src/ExplicitSolver.cpp:169-185)
+        double rs = 0.0;
+#pragma unroll
+        for (int ph = 0; ph < NPH; ++ph) {
+            un[DIM + 2 + ph] = dmax_ref(un[DIM + 2 + ph], 1e-14);
+            rs += un[DIM + 2 + ph];
+        }
+        un[0] = rs;
+#pragma unroll
+        for (int ph = 0; ph < NPH; ++ph) un[DIM + 2 + NPH + ph] = dmax_ref(un[DIM + 2 + NPH + ph], P.mix.alphaMin);
+        double as = 0.0;
+#pragma unroll
+        for (int ph = 0; ph < NPH; ++ph) as += un[DIM + 2 + NPH + ph];
+#pragma unroll
+        for (int ph = 0; ph < NPH; ++ph) un[DIM + 2 + NPH + ph] /= as;
+    }
+#pragma unroll
+    for (int q = 0; q < NCV; ++q)
+        if (!(SEMI && q == DIM + 1)) P.dst[(size_t)q * P.total + id] = (T)un[q];
+    if constexpr (SEMI) {
+        // advected pressure (src/SemiImplicitSolver.cpp:168,185-186); the
+        // stage-0 pressure is the conversion of the backup state
+        double p0 = 0.0;
+        if (use0) {
+            Cell<DIM, NPH> c0;
+            prim_of<DIM, NPH>(P.eos, P.mix, U0v, c0);
+            p0 = c0.p;
+        }
+        double a = P.c1 * cell.p + P.c2 * p0;
+        P.pAdv[id] = (a + (P.c3 * dt) * (double)acc[QP]) / P.c4;
+        P.pres[id] = cell.p;
+        double rs = dmax_ref(un[0], 1e-14);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) P.vel[d][id] = un[1 + d] / rs;
+        return 1.0e300;
+    }
+    if (!wantDt) return 1.0e300;
+    Cell<DIM, NPH> c;
+    double Uq[NCV];
+#pragma unroll
+    for (int q = 0; q < NCV; ++q) Uq[q] = (double)(T)un[q];
+    prim_of<DIM, NPH>(P.eos, P.mix, Uq, c);
+    double cs;
+    if constexpr (NPH == 0) {
+        double pinf = P.eos.stiffened ? P.eos.pInf : 0.0;
+        cs = sqrt(P.eos.gamma * dmax_ref(c.p + pinf, 1e-14) / dmax_ref(c.rho, 1e-14));
+    } else {
+        cs = mix_sound(P.mix, c.rho, c.p, c.a, c.ar, NPH);
+    }
+    double dc = h[0] / (fabs(c.u[0]) + cs);
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) dc = dmin_ref(dc, h[d] / (fabs(c.u[d]) + cs));
+    return dc;
+}
+
+template <int DIM, int NPH, int R, typename T>
+__device__ __forceinline__ void recon_window(const StageParams<T>& P, const Cell<DIM, NPH>* win, const T* sigwin,
+                                             int first, bool useSig, T eps, bool weno,
+                                             FaceState<DIM, NPH, T>& minusS, FaceState<DIM, NPH, T>& plusS) {
+    constexpr int NV = n_recon<DIM, NPH>();
+    constexpr int S = 2 * R - 1;
+    T st[S], mn[NV], pl[NV], vv[S][NV];
+#pragma unroll
+    for (int s = 0; s < S; ++s) recon_vars<DIM, NPH, T>(win[first + s], vv[s]);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) st[s] = vv[s][v];
+        Recon<R, T>::run(st, eps, weno, mn[v], pl[v]);
+    }
+    T sm = T(0), sl = T(0);
+    if (useSig) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) st[s] = sigwin[first + s];
+        Recon<R, T>::run(st, eps, weno, sm, sl);
+    }
+    face_state<DIM, NPH, T>(P, mn, sm, minusS);
+    face_state<DIM, NPH, T>(P, pl, sl, plusS);
+}
+
+// ===========================================================================
+// x + y kernel (1D/2D full stage; 3D x-y part on one plane)
+// ===========================================================================
+template <int DIM, int R, int RS, int NPH, bool SEMI, int MODE, typename T>
 __global__ void __launch_bounds__(64) stage_xy_kernel(StageParams<T> P) {
-    constexpr int NP = DIM + 2;  // primitive variables rho, u[DIM], p
-    constexpr int NC = DIM + 2;  // conservative variables
-    constexpr int E = R - 1;     // extra LDS columns per side
-    constexpr int S = 2 * R - 1; // stencil width
-    constexpr int WLO = (R >= 2) ? (2 - R) : 0;          // window start, relative to r
-    constexpr int WS = R - WLO + 1;                      // window rows
-    constexpr int WROW = -WLO;                           // window slot of row r
-    constexpr int WST = (R >= 2) ? 0 : 1;                // first stencil slot of row r+1
+    constexpr int NV = n_recon<DIM, NPH>();   // reconstructed variables
+    constexpr int NF = n_fc<DIM, NPH>();      // face contribution length
+    constexpr int NFL = DIM + 4 + NPH;        // Riemann output length
+    constexpr int E = R - 1;                  // extra LDS columns per side
+    constexpr int S = 2 * R - 1;              // stencil width
+    constexpr int WLO = (R >= 2) ? (2 - R) : 0;
+    constexpr int WS = R - WLO + 1;           // window rows
+    constexpr int WROW = -WLO;                // window slot of row r
+    constexpr int WST = (R >= 2) ? 0 : 1;     // first stencil slot of row r+1
+    constexpr bool YDIR = DIM >= 2;
 
     if (P.active && *P.active == 0) return;
     const Grid& g = P.g;
     const int lane = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int strip = blk / P.nseg;
-    const int seg = blk - strip * P.nseg;
+    const int perPlane = P.nstrip * P.nseg;
+    const int kz = (MODE == MODE_XY) ? blk / perPlane : 0;
+    const int rem = blk - kz * perPlane;
+    const int strip = rem / P.nseg;
+    const int seg = rem - strip * P.nseg;
     const int X0 = strip * kOwn;
     const int col = X0 - 1 + lane;
-    const int Y0 = (DIM >= 2) ? seg * P.rows : 0;
-    const int Y1 = (DIM >= 2) ? ((Y0 + P.rows < g.ny) ? Y0 + P.rows : g.ny) : 1;
+    const int Y0 = YDIR ? seg * P.rows : 0;
+    const int Y1 = YDIR ? ((Y0 + P.rows < g.ny) ? Y0 + P.rows : g.ny) : 1;
     const bool owned = lane >= 1 && lane <= kOwn && col < g.nx;
     const bool useSig = (RS == RS_LF) && P.igr;
+    const bool incP = !SEMI;
+    const bool igr = P.igr != 0;
 
-    __shared__ T sp[NP + 1][kWave + 2 * E];
+    __shared__ T sp[NV + 1][kWave + 2 * E];
 
-    // extra column handled by this lane (lanes 0..2E-1)
     int ecol = 0;
     if (lane < E) ecol = X0 - 1 - E + lane;
     else if (lane < 2 * E) ecol = X0 + kWave - 1 + (lane - E);
 
     const T eps = (T)P.eps;
-    const T gam = (T)P.eos.gamma;
-    const T pinf = (T)(P.eos.stiffened ? P.eos.pInf : 0.0);
-    const bool incP = P.incP != 0;
     const bool weno = P.weno != 0;
     const double dt = *P.dt;
-
-    // geometry of the lane's column
-    const int cc = col < g.nx ? col : g.nx - 1;
-    const double dxc = g.dxw[(cc < 0 ? 0 : cc) + g.ngx];
-
+    const int cc = col < g.nx ? (col < 0 ? 0 : col) : g.nx - 1;
+    const double dxc = g.dxw[cc + g.ngx];
+    const double dzk = (DIM >= 3) ? g.dzw[kz + g.ngz] : 1.0;
     double dtMin = 1.0e300;
 
-    // y machinery (DIM == 2)
-    T win[WS > 0 ? WS : 1][NP];
-    T sigwin[WS > 0 ? WS : 1];
-    FaceState<DIM, T> plusPrev;
-    T Glo[NC];
+    Cell<DIM, NPH> win[WS];
+    T sigwin[WS];
+    FaceState<DIM, NPH, T> plusPrev;
+    T Glo[NF];
 #pragma unroll
-    for (int q = 0; q < NC; ++q) Glo[q] = T(0);
+    for (int q = 0; q < NF; ++q) Glo[q] = T(0);
 
     int rStart = Y0;
-    if (DIM >= 2) {
+    if (YDIR) {
         rStart = Y0 - 2;
-        // preload the window as it stands after iteration rStart-1 (slot 0
-        // is slid out before it is read, so it is left empty)
+        // the window as it stands after iteration rStart-1 (slot 0 is slid
+        // out before it is read)
 #pragma unroll
         for (int w = 1; w < WS; ++w) {
-            double pr[NP];
             const int row = rStart - 1 + WLO + w;
-            load_prim<DIM>(P, col, row, pr);
-#pragma unroll
-            for (int v = 0; v < NP; ++v) win[w][v] = (T)pr[v];
-            sigwin[w] = useSig ? (T)load_sigma<DIM>(P, col, row) : T(0);
+            load_cell<DIM, NPH>(P, P.src, col, row, kz, win[w]);
+            sigwin[w] = useSig ? (T)load_sigma<DIM>(P, col, row, kz) : T(0);
         }
     }
 
     for (int r = rStart; r < Y1; ++r) {
-        T rhs[NC];
+        T Ghi[NF];
 #pragma unroll
-        for (int q = 0; q < NC; ++q) rhs[q] = T(0);
-        double cy = 0.0;
-        if (DIM >= 2) {
-            // slide the window and load row r+R
+        for (int q = 0; q < NF; ++q) Ghi[q] = T(0);
+        double cy = 0.0, dyr = 1.0;
+        Cell<DIM, NPH> cur;
+        T sigc = T(0);
+        if (YDIR) {
 #pragma unroll
             for (int w = 0; w < WS - 1; ++w) {
-#pragma unroll
-                for (int v = 0; v < NP; ++v) win[w][v] = win[w + 1][v];
+                win[w] = win[w + 1];
                 sigwin[w] = sigwin[w + 1];
             }
-            {
-                double pr[NP];
-                load_prim<DIM>(P, col, r + R, pr);
-#pragma unroll
-                for (int v = 0; v < NP; ++v) win[WS - 1][v] = (T)pr[v];
-                sigwin[WS - 1] = useSig ? (T)load_sigma<DIM>(P, col, r + R) : T(0);
-            }
+            load_cell<DIM, NPH>(P, P.src, col, r + R, kz, win[WS - 1]);
+            sigwin[WS - 1] = useSig ? (T)load_sigma<DIM>(P, col, r + R, kz) : T(0);
             // y reconstruction of row r+1
-            FaceState<DIM, T> mY, pY;
-            {
-                T st[S], mn, pl;
-#pragma unroll
-                for (int v = 0; v < NP; ++v) {
-#pragma unroll
-                    for (int s = 0; s < S; ++s) st[s] = win[WST + s][v];
-                    Recon<R, T>::run(st, eps, weno, mn, pl);
-                    if (v == 0) { mY.rho = mn; pY.rho = pl; }
-                    else if (v == NP - 1) { mY.p = mn; pY.p = pl; }
-                    else { mY.u[v - 1] = mn; pY.u[v - 1] = pl; }
-                }
-                if (useSig) {
-#pragma unroll
-                    for (int s = 0; s < S; ++s) st[s] = sigwin[WST + s];
-                    Recon<R, T>::run(st, eps, weno, mn, pl);
-                    mY.sig = mn;
-                    pY.sig = pl;
-                } else {
-                    mY.sig = T(0);
-                    pY.sig = T(0);
-                }
-            }
-            T Ghi[NC];
-#pragma unroll
-            for (int q = 0; q < NC; ++q) Ghi[q] = T(0);
+            FaceState<DIM, NPH, T> mY, pY;
+            recon_window<DIM, NPH, R, T>(P, win, sigwin, WST, useSig, eps, weno, mY, pY);
             if (r >= Y0 - 1) {
-                T F[DIM + 4];
-                riemann<RS, 1, DIM, T>(plusPrev, mY, gam, pinf, incP, P.igr != 0, F);
-#pragma unroll
-                for (int q = 0; q < NC; ++q) Ghi[q] = F[q];
+                T F[NFL];
+                riemann<RS, 1, DIM, NPH, T>(plusPrev, mY, incP, igr, F);
+                // cells of face r+1/2: rows r (slot WROW) and r+1 (slot WROW+1)
+                face_contrib<DIM, NPH, T>(F, win[WROW].rho, win[WROW + 1].rho, win[WROW].ar,
+                                          win[WROW + 1].ar, Ghi);
             }
             plusPrev = pY;
             if (r < Y0) {
 #pragma unroll
-                for (int q = 0; q < NC; ++q) Glo[q] = Ghi[q];
+                for (int q = 0; q < NF; ++q) Glo[q] = Ghi[q];
                 continue;  // warm-up rows: no x phase, no output
             }
-            // y contribution (added after the x contribution below)
-            const double dyr = g.dyw[r + g.ngy];
-            cy = (dxc * 1.0) / ((dxc * dyr) * 1.0);
-#pragma unroll
-            for (int q = 0; q < NC; ++q) rhs[q] = Ghi[q];  // stash G_{r+1/2}
+            dyr = g.dyw[r + g.ngy];
+            cy = (dxc * dzk) / ((dxc * dyr) * dzk);
+            cur = win[WROW];
+            sigc = sigwin[WROW];
+        } else {
+            load_cell<DIM, NPH>(P, P.src, col, r, kz, cur);
+            if (useSig) sigc = (T)load_sigma<DIM>(P, col, r, kz);
+            dyr = g.dyw[0];
         }
 
         // ---------------- x phase on row r ----------------
-        double w0[NP];
-        T sig0 = T(0);
-        if (DIM >= 2) {
-#pragma unroll
-            for (int v = 0; v < NP; ++v) w0[v] = (double)win[WROW][v];
-            sig0 = sigwin[WROW];
-        } else {
-            load_prim<DIM>(P, col, r, w0);
-            if (useSig) sig0 = (T)load_sigma<DIM>(P, col, r);
-        }
         __syncthreads();  // previous iteration's LDS reads are complete
+        {
+            T vv[NV];
+            recon_vars<DIM, NPH, T>(cur, vv);
 #pragma unroll
-        for (int v = 0; v < NP; ++v) sp[v][lane + E] = (T)w0[v];
-        if (useSig) sp[NP][lane + E] = sig0;
+            for (int v = 0; v < NV; ++v) sp[v][lane + E] = vv[v];
+            if (useSig) sp[NV][lane + E] = sigc;
+        }
         if (E > 0 && lane < 2 * E) {
-            double we[NP];
-            load_prim<DIM>(P, ecol, r, we);
-            int pos = lane < E ? lane : kWave + lane;
+            Cell<DIM, NPH> ce;
+            load_cell<DIM, NPH>(P, P.src, ecol, r, kz, ce);
+            T vv[NV];
+            recon_vars<DIM, NPH, T>(ce, vv);
+            const int pos = lane < E ? lane : kWave + lane;
 #pragma unroll
-            for (int v = 0; v < NP; ++v) sp[v][pos] = (T)we[v];
-            if (useSig) sp[NP][pos] = (T)load_sigma<DIM>(P, ecol, r);
+            for (int v = 0; v < NV; ++v) sp[v][pos] = vv[v];
+            if (useSig) sp[NV][pos] = (T)load_sigma<DIM>(P, ecol, r, kz);
         }
         __syncthreads();
-        FaceState<DIM, T> mX, pX;
+        FaceState<DIM, NPH, T> mX, pX;
         {
-            T st[S], mn, pl;
+            T st[S], mn[NV], pl[NV];
 #pragma unroll
-            for (int v = 0; v < NP; ++v) {
+            for (int v = 0; v < NV; ++v) {
 #pragma unroll
                 for (int s = 0; s < S; ++s) st[s] = sp[v][lane + s];
-                Recon<R, T>::run(st, eps, weno, mn, pl);
-                if (v == 0) { mX.rho = mn; pX.rho = pl; }
-                else if (v == NP - 1) { mX.p = mn; pX.p = pl; }
-                else { mX.u[v - 1] = mn; pX.u[v - 1] = pl; }
+                Recon<R, T>::run(st, eps, weno, mn[v], pl[v]);
             }
+            T sm = T(0), sl = T(0);
             if (useSig) {
 #pragma unroll
-                for (int s = 0; s < S; ++s) st[s] = sp[NP][lane + s];
-                Recon<R, T>::run(st, eps, weno, mn, pl);
-                mX.sig = mn;
-                pX.sig = pl;
-            } else {
-                mX.sig = T(0);
-                pX.sig = T(0);
+                for (int s = 0; s < S; ++s) st[s] = sp[NV][lane + s];
+                Recon<R, T>::run(st, eps, weno, sm, sl);
             }
+            face_state<DIM, NPH, T>(P, mn, sm, mX);
+            face_state<DIM, NPH, T>(P, pl, sl, pX);
         }
         // face col+1/2: left = plus of this lane, right = minus of lane+1
-        FaceState<DIM, T> rX;
+        FaceState<DIM, NPH, T> rX;
         rX.rho = __shfl_down(mX.rho, 1, kWave);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) rX.u[d] = __shfl_down(mX.u[d], 1, kWave);
         rX.p = __shfl_down(mX.p, 1, kWave);
         rX.sig = __shfl_down(mX.sig, 1, kWave);
-        T Fx[DIM + 4];
-        riemann<RS, 0, DIM, T>(pX, rX, gam, pinf, incP, P.igr != 0, Fx);
-        T Fl[NC];
+        rX.g = __shfl_down(mX.g, 1, kWave);
+        rX.pi = __shfl_down(mX.pi, 1, kWave);
 #pragma unroll
-        for (int q = 0; q < NC; ++q) Fl[q] = __shfl_up(Fx[q], 1, kWave);
+        for (int ph = 0; ph < NPH; ++ph) rX.a[ph] = __shfl_down(mX.a[ph], 1, kWave);
+        T Fx[NFL];
+        riemann<RS, 0, DIM, NPH, T>(pX, rX, incP, igr, Fx);
+        T FCx[NF];
+        {
+            double rhoR = __shfl_down(cur.rho, 1, kWave);
+            double arR[NPH > 0 ? NPH : 1];
+#pragma unroll
+            for (int ph = 0; ph < NPH; ++ph) arR[ph] = __shfl_down(cur.ar[ph], 1, kWave);
+            face_contrib<DIM, NPH, T>(Fx, cur.rho, rhoR, cur.ar, arR, FCx);
+        }
+        T FCl[NF];
+#pragma unroll
+        for (int q = 0; q < NF; ++q) FCl[q] = __shfl_up(FCx[q], 1, kWave);
 
         if (owned) {
-            const double dyr = (DIM >= 2) ? g.dyw[r + g.ngy] : g.dyw[0];
-            const double cx = (dyr * 1.0) / ((dxc * dyr) * 1.0);
-            T res[NC];
+            const double cx = (dyr * dzk) / ((dxc * dyr) * dzk);
+            T acc[NF];
+            accumulate<DIM, NPH, T>(acc, FCl, FCx, (T)cx, true);
+            if (YDIR) accumulate<DIM, NPH, T>(acc, Glo, Ghi, (T)cy, false);
+            const size_t id = cell_index(g, col, r, kz);
+            if (MODE == MODE_XY) {
 #pragma unroll
-            for (int q = 0; q < NC; ++q) {
-                T acc = (T)cx * Fl[q];
-                acc = acc - (T)cx * Fx[q];
-                if (DIM >= 2) {
-                    acc = acc + (T)cy * Glo[q];
-                    acc = acc - (T)cy * rhs[q];
-                }
-                res[q] = acc;
-            }
-            size_t id = cell_index(g, col, r, 0);
-            double un[NC];
-#pragma unroll
-            for (int q = 0; q < NC; ++q) {
-                double U = (double)P.src[q][id];
-                double Rq = (double)res[q];
-                if (P.rk1) {
-                    un[q] = U + dt * Rq;
-                } else {
-                    double a = P.c1 * U;
-                    if (P.c2 != 0.0) a = a + P.c2 * (double)P.u0[q][id];
-                    un[q] = (a + (P.c3 * dt) * Rq) / P.c4;
-                }
-                P.dst[q][id] = (T)un[q];
-            }
-            if (P.dtPartial) {
-                double rho = (double)(T)un[0];
-                double mom[DIM], u[DIM], p;
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) mom[d] = (double)(T)un[1 + d];
-                cons_to_prim<DIM>(P.eos, rho, mom, (double)(T)un[DIM + 1], u, p);
-                double pinfd = P.eos.stiffened ? P.eos.pInf : 0.0;
-                double c = sqrt(P.eos.gamma * dmax_ref(p + pinfd, 1e-14) / dmax_ref(rho, 1e-14));
-                double dc = dxc / (fabs(u[0]) + c);
-                if (DIM >= 2) dc = dmin_ref(dc, dyr / (fabs(u[1]) + c));
+                for (int q = 0; q < NF; ++q) P.part[(size_t)q * P.total + id] = acc[q];
+            } else {
+                const double h[3] = {dxc, dyr, dzk};
+                double dc = finish_cell<DIM, NPH, SEMI, T>(P, id, acc, cur, h, dt, P.dtPartial != nullptr);
                 dtMin = dmin_ref(dtMin, dc);
             }
         }
-        if (DIM >= 2) {
+        if (YDIR) {
 #pragma unroll
-            for (int q = 0; q < NC; ++q) Glo[q] = rhs[q];
+            for (int q = 0; q < NF; ++q) Glo[q] = Ghi[q];
         }
+    }
+    if (MODE == MODE_FULL && P.dtPartial) {
+        double m = wave_min(dtMin);
+        if (lane == 0) P.dtPartial[blockIdx.x] = m;
+    }
+}
+
+// ===========================================================================
+// 3D z pass: 64 columns per wave, marching in z with a register window
+// ===========================================================================
+template <int R, int RS, int NPH, bool SEMI, typename T>
+__global__ void __launch_bounds__(64) stage_z_kernel(StageParams<T> P) {
+    constexpr int DIM = 3;
+    constexpr int NF = n_fc<DIM, NPH>();
+    constexpr int NFL = DIM + 4 + NPH;
+    constexpr int WLO = (R >= 2) ? (2 - R) : 0;
+    constexpr int WS = R - WLO + 1;
+    constexpr int WROW = -WLO;
+    constexpr int WST = (R >= 2) ? 0 : 1;
+
+    if (P.active && *P.active == 0) return;
+    const Grid& g = P.g;
+    const int lane = threadIdx.x;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    // blk -> (row j, strip, segment), segment fastest
+    const int perRow = P.nstrip * P.nseg;
+    const int j = blk / perRow;
+    const int rem = blk - j * perRow;
+    const int strip = rem / P.nseg;
+    const int seg = rem - strip * P.nseg;
+    const int col = strip * kWave + lane;
+    const int Z0 = seg * P.rows;
+    const int Z1 = (Z0 + P.rows < g.nz) ? Z0 + P.rows : g.nz;
+    const bool owned = col < g.nx;
+    const bool useSig = (RS == RS_LF) && P.igr;
+    const bool incP = !SEMI;
+    const bool igr = P.igr != 0;
+    const T eps = (T)P.eps;
+    const bool weno = P.weno != 0;
+    const double dt = *P.dt;
+    const int cc = owned ? col : g.nx - 1;
+    const double dxc = g.dxw[cc + g.ngx];
+    const double dyj = g.dyw[j + g.ngy];
+    double dtMin = 1.0e300;
+
+    Cell<DIM, NPH> win[WS];
+    T sigwin[WS];
+    FaceState<DIM, NPH, T> plusPrev;
+    T Hlo[NF];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) Hlo[q] = T(0);
+    const int rStart = Z0 - 2;
+#pragma unroll
+    for (int w = 1; w < WS; ++w) {
+        const int kk = rStart - 1 + WLO + w;
+        load_cell<DIM, NPH>(P, P.src, cc, j, kk, win[w]);
+        sigwin[w] = useSig ? (T)load_sigma<DIM>(P, cc, j, kk) : T(0);
+    }
+    for (int r = rStart; r < Z1; ++r) {
+#pragma unroll
+        for (int w = 0; w < WS - 1; ++w) {
+            win[w] = win[w + 1];
+            sigwin[w] = sigwin[w + 1];
+        }
+        load_cell<DIM, NPH>(P, P.src, cc, j, r + R, win[WS - 1]);
+        sigwin[WS - 1] = useSig ? (T)load_sigma<DIM>(P, cc, j, r + R) : T(0);
+        FaceState<DIM, NPH, T> mZ, pZ;
+        recon_window<DIM, NPH, R, T>(P, win, sigwin, WST, useSig, eps, weno, mZ, pZ);
+        T Hhi[NF];
+#pragma unroll
+        for (int q = 0; q < NF; ++q) Hhi[q] = T(0);
+        if (r >= Z0 - 1) {
+            T F[NFL];
+            riemann<RS, 2, DIM, NPH, T>(plusPrev, mZ, incP, igr, F);
+            face_contrib<DIM, NPH, T>(F, win[WROW].rho, win[WROW + 1].rho, win[WROW].ar, win[WROW + 1].ar, Hhi);
+        }
+        plusPrev = pZ;
+        if (r >= Z0 && owned) {
+            const double dzr = g.dzw[r + g.ngz];
+            const double cz = (dxc * dyj) / ((dxc * dyj) * dzr);
+            const size_t id = cell_index(g, col, j, r);
+            T acc[NF];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) acc[q] = P.part[(size_t)q * P.total + id];
+            accumulate<DIM, NPH, T>(acc, Hlo, Hhi, (T)cz, false);
+            const double h[3] = {dxc, dyj, dzr};
+            double dc = finish_cell<DIM, NPH, SEMI, T>(P, id, acc, win[WROW], h, dt, P.dtPartial != nullptr);
+            dtMin = dmin_ref(dtMin, dc);
+        }
+#pragma unroll
+        for (int q = 0; q < NF; ++q) Hlo[q] = Hhi[q];
     }
     if (P.dtPartial) {
         double m = wave_min(dtMin);

@@ -1,6 +1,6 @@
 """Writes the golden fixtures of tests/golden/ from the compiled reference.
 
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py [generator ...]
 
 Every expected output here comes from the unmodified reference sources
 (oracle/_ref/libsifv_ref.so, built from the reference checkout by
@@ -23,7 +23,7 @@ from semiimplicitfv_amd import (BoundaryCondition, IdealGasEOS, ReconstructionOr
 from semiimplicitfv_amd.cases import _fill  # noqa: E402
 from semiimplicitfv_amd.timeloop import run_time_loop  # noqa: E402
 from tests.golden.problems import (bubble_3d, jacobi_problem, liquid_gas_tube, random_ghost_case,  # noqa: E402
-                                   recon_lines, riemann_pairs)
+                                   recon_lines, riemann_pairs, two_phase_cases)
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 
@@ -140,13 +140,33 @@ def g_liquid_gas():
     save("liquid_gas_1d.npz", **out)
 
 
+def g_two_phase():
+    """Interior fields, time steps and Jacobi counts of the seeded water/air
+    cases (explicit and semi-implicit, 1D-3D)."""
+    out = {}
+    for key, (build, rs, nsteps, semi) in two_phase_cases().items():
+        cfg, mesh, st, eos = build()
+        s = po.CpuSolver(mesh, cfg, rs, eos, "ref")
+        dts, its = [], []
+        for k in range(nsteps):
+            cfg.step = k
+            if semi:
+                dt, it = s.semi_step(st, cfg)
+                its.append(it)
+            else:
+                dt = s.explicit_step(st, cfg)
+            dts.append(dt)
+        out[f"{key}_dts"] = np.array(dts)
+        out[f"{key}_iters"] = np.array(its, dtype=np.int64)
+        for fid in st.field_ids():
+            out[f"{key}_f{fid}"] = np.ascontiguousarray(mesh.interior(st.field(fid)))
+    save("two_phase_ref.npz", **out)
+
+
+GENERATORS = {"recon": g_recon, "riemann": g_riemann, "sod": g_sod, "vortex": g_vortex, "ghost": g_ghost,
+              "jacobi": g_jacobi, "semi": g_semi, "liquid_gas": g_liquid_gas, "two_phase": g_two_phase}
+
 if __name__ == "__main__":
     assert po.available("ref"), "build oracle/_ref first (make -C oracle)"
-    g_recon()
-    g_riemann()
-    g_sod()
-    g_vortex()
-    g_ghost()
-    g_jacobi()
-    g_semi()
-    g_liquid_gas()
+    for name in sys.argv[1:] or GENERATORS:
+        GENERATORS[name]()

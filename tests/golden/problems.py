@@ -231,3 +231,94 @@ def liquid_gas_tube(n: int = 200):
     I(st.rhoE)[...] = E.reshape(1, 1, n)
     eos = IdealGasEOS(1.4, 287.0, cfg)
     return cfg, mesh, st, eos, HLLCSolver(eos)
+
+
+def make_two_phase_case(dim, n, bcs=None, periods=(0, 0, 0), recon=2, rk=3, seed=0, semi=False, p_jump=True,
+                        cfl=0.4, nonuniform=False):
+    """Water/air state with a smooth (tanh) interface, random-ish velocity
+    and an optional pressure jump.  semi=True configures the semi-implicit
+    solver (dt <= 2e-5, tolerance 1e-6, up to 400 Jacobi sweeps)."""
+    rng = np.random.default_rng(seed)
+    cfg = SimulationConfig(dim=dim, nGhost=3, RKOrder=rk, reconOrder=recon, semiImplicit=semi)
+    cfg.multiPhaseParams.nPhases = 2
+    cfg.multiPhaseParams.phases = [WATER, AIR]
+    cfg.multiPhaseParams.alphaMin = 1e-8
+    if semi:
+        cfg.semiImplicitParams.cfl = cfl
+        cfg.semiImplicitParams.maxDt = 2e-5
+        cfg.semiImplicitParams.pressureTol = 1e-6
+        cfg.semiImplicitParams.maxPressureIters = 400
+    else:
+        cfg.explicitParams.cfl = cfl
+        cfg.explicitParams.maxDt = 1.0
+    cfg.validate()
+    nodes = []
+    for d in range(3):
+        m = n[d] if d < dim else 1
+        if d >= dim:
+            nodes.append(np.array([0.0, 1.0]))
+            continue
+        if nonuniform:
+            w = 1.0 + 0.5 * rng.random(m)
+            x = np.concatenate([[0.0], np.cumsum(w)])
+            x = x / x[-1]
+        else:
+            x = np.linspace(0.0, 1.0, m + 1)
+        nodes.append(x)
+    mesh = RectilinearMesh(cfg, nodes[0], nodes[1], nodes[2], periods=periods)
+    if bcs is not None:
+        for f, b in enumerate(bcs):
+            mesh.setBoundaryCondition(f, b)
+    st = SolutionState()
+    st.allocate(mesh.totalCells(), cfg)
+    xc, yc, zc = mesh.centroids()
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    h = 1.0 / max(n[:dim])
+    s = (X - 0.45) + (0.3 * (Y - 0.5) if dim >= 2 else 0.0) + (0.2 * (Z - 0.5) if dim >= 3 else 0.0)
+    amin = cfg.multiPhaseParams.alphaMin
+    aw = np.clip(0.5 * (1.0 - np.tanh(s / (2.0 * h))), amin, 1.0 - amin)
+    aa = 1.0 - aw
+    ar0 = aw * (1000.0 * (1.0 + 0.01 * np.sin(2 * np.pi * Y)))
+    ar1 = aa * (1.2 * (1.0 + 0.05 * np.cos(2 * np.pi * X)))
+    rho = ar0 + ar1
+    p = 1e5 * (1.0 + 0.2 * np.cos(2 * np.pi * (X + Y + Z)))
+    if p_jump:
+        p = np.where(s < 0.0, p * 20.0, p)
+    u = [(0.5 * np.sin(2 * np.pi * (Y + 0.1 * d)) + 0.2 * (d + 1)) * (1.0 + 0.01 * rng.random(X.shape))
+         for d in range(dim)]
+    ke = sum(0.5 * rho * u[d] * u[d] for d in range(dim))
+    E = _mixture_energy(p, [aw, aa], ke, cfg.multiPhaseParams.phases)
+    I = mesh.interior
+    I(st.alphaRho[0])[...] = ar0
+    I(st.alphaRho[1])[...] = ar1
+    I(st.alpha[0])[...] = aw
+    I(st.alpha[1])[...] = aa
+    I(st.rho)[...] = rho
+    I(st.pres)[...] = p
+    for d, (m, v) in enumerate(zip([st.rhoU, st.rhoV, st.rhoW], [st.velU, st.velV, st.velW])):
+        if d < dim:
+            I(m)[...] = rho * u[d]
+            I(v)[...] = u[d]
+    I(st.rhoE)[...] = E
+    eos = StiffenedGasEOS(4.4, 6.0e8, 1.0, cfg)
+    return cfg, mesh, st, eos
+
+
+_TP_BCS = [BoundaryCondition.Outflow, BoundaryCondition.SlipWall, BoundaryCondition.Symmetry,
+           BoundaryCondition.NoSlipWall, BoundaryCondition.Outflow, BoundaryCondition.SlipWall]
+
+
+def two_phase_cases():
+    """Seeded water/air cases whose reference outputs are two_phase_ref.npz:
+    key -> (builder, Riemann kind (0 LF, 1 Rusanov, 2 HLLC), steps, semi)."""
+    out = {}
+    sizes = {1: (120,), 2: (40, 23), 3: (14, 10, 9)}
+    for dim, rs in ((1, 2), (1, 1), (2, 2), (2, 0), (3, 2), (3, 1)):
+        out[f"x{dim}_rs{rs}"] = (
+            (lambda dim=dim: make_two_phase_case(dim, sizes[dim], bcs=_TP_BCS, seed=dim,
+                                                 periods=(0, 0, 1) if dim == 3 else (0, 0, 0))), rs, 5, False)
+    ssizes = {1: (100,), 2: (36, 24), 3: (12, 10, 9)}
+    for dim in (1, 2, 3):
+        out[f"s{dim}"] = ((lambda dim=dim: make_two_phase_case(dim, ssizes[dim], bcs=[BoundaryCondition.SlipWall] * 6,
+                                                               seed=10 + dim, semi=True, p_jump=False)), 2, 3, True)
+    return out

@@ -1,0 +1,150 @@
+"""GPU parity of the semi-implicit step (advective stage kernels, pressure
+set-up, Jacobi sweeps, correction) and of the standalone Jacobi solve,
+against the oracle and the reference's own golden output.
+
+Bar: relative L1 and Linf <= 1e-12 after N steps against the FP64 CPU
+oracle, identical time steps and Jacobi iteration counts.  Water/air
+states are ill-conditioned (gamma*pInf ~ 2.6e9 beside pressures ~ 1e5):
+the compiled reference and the oracle already differ there by up to ~2e-11
+through rounding alone, so each field's bar is max(1e-12, 3 x that
+measured reference-vs-oracle distance), applied against both the oracle
+and the reference's own output.  All calls go through the C ABI.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from semiimplicitfv_amd import (BoundaryCondition, HLLCSolver, IdealGasEOS, JacobiPressureSolver, LFSolver,
+                                RusanovSolver, SemiImplicitSolver, _abi)
+from tests.golden import problems
+from tests.helpers import make_semi_case, noise_floor_bar, rel_errs
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+RS = {"hllc": HLLCSolver, "rusanov": RusanovSolver, "lf": LFSolver}
+
+
+def _pair(oracle, cfg, mesh, st, eos, rsolver, nsteps, rows=0):
+    sg, sc = st.copy(), st.copy()
+    gpu = SemiImplicitSolver(mesh, rsolver, JacobiPressureSolver(), eos, None, cfg)
+    if rows:
+        gpu.set_tile(rows)
+    cpu = oracle.CpuSolver(mesh, cfg, rsolver.kind(), eos, "c")
+    gpu.upload(sg)
+    hist = []
+    for s in range(nsteps):
+        cfg.step = s
+        dg = gpu.step_resident(cfg)
+        dc, ic = cpu.semi_step(sc, cfg)
+        hist.append((dg, dc, gpu.lastPressureIterations(), ic))
+    gpu.download(sg)
+    gpu.close()
+    return sg, sc, hist
+
+
+def _check(mesh, sg, sc, hist, tol=TOL):
+    for dg, dc, ig, ic in hist:
+        assert abs(dg - dc) <= 1e-13 * dc, (dg, dc)
+        assert ig == ic, (ig, ic)
+    for fid in sc.field_ids():
+        l1, li = rel_errs(mesh.interior(sg.field(fid)), mesh.interior(sc.field(fid)))
+        assert l1 <= tol and li <= tol, f"field {fid}: rel L1 {l1:.3e}, rel Linf {li:.3e}"
+
+
+@pytest.mark.parametrize("rk", [1, 2, 3])
+def test_semi_1d_rk_orders(gpu, oracle_c, rk):
+    cfg, mesh, st, eos = make_semi_case(1, (96,), rk=rk, seed=rk, nonuniform=True,
+                                        bcs=[BoundaryCondition.SlipWall, BoundaryCondition.Outflow] + [0] * 4)
+    sg, sc, hist = _pair(oracle_c, cfg, mesh, st, eos, HLLCSolver(eos), 6)
+    _check(mesh, sg, sc, hist)
+
+
+@pytest.mark.parametrize("recon,rsname", [(2, "hllc"), (2, "rusanov"), (2, "lf"), (1, "hllc"), (0, "rusanov"),
+                                          (5, "lf")])
+def test_semi_2d_recon_riemann(gpu, oracle_c, recon, rsname):
+    cfg, mesh, st, eos = make_semi_case(2, (40, 33), recon=recon, seed=recon + 5, periods=(1, 0, 0),
+                                        bcs=[0, 0, BoundaryCondition.NoSlipWall, BoundaryCondition.Symmetry, 0, 0])
+    sg, sc, hist = _pair(oracle_c, cfg, mesh, st, eos, RS[rsname](eos), 4)
+    _check(mesh, sg, sc, hist)
+
+
+def test_semi_2d_multi_strip(gpu, oracle_c):
+    cfg, mesh, st, eos = make_semi_case(2, (130, 41), seed=4, periods=(0, 1, 0), nonuniform=True,
+                                        bcs=[BoundaryCondition.SlipWall, BoundaryCondition.Outflow, 0, 0, 0, 0])
+    sg, sc, hist = _pair(oracle_c, cfg, mesh, st, eos, HLLCSolver(eos), 3, rows=3)
+    _check(mesh, sg, sc, hist)
+
+
+def test_semi_3d_single_phase(gpu, oracle_c):
+    cfg, mesh, st, eos = make_semi_case(3, (13, 10, 11), seed=2, periods=(0, 0, 1),
+                                        bcs=[BoundaryCondition.SlipWall, BoundaryCondition.Outflow,
+                                             BoundaryCondition.Symmetry, BoundaryCondition.NoSlipWall, 0, 0])
+    sg, sc, hist = _pair(oracle_c, cfg, mesh, st, eos, HLLCSolver(eos), 3)
+    _check(mesh, sg, sc, hist)
+
+
+def _check_floor(mesh, sg, sc, ref_fields):
+    for fid in sc.field_ids():
+        ref, orc, got = ref_fields(fid), mesh.interior(sc.field(fid)), mesh.interior(sg.field(fid))
+        b1, bi = noise_floor_bar(ref, orc)
+        for other in (orc, ref):
+            l1, li = rel_errs(got, other)
+            assert l1 <= b1 and li <= bi, f"field {fid}: rel L1 {l1:.3e} (bar {b1:.1e}), Linf {li:.3e} (bar {bi:.1e})"
+
+
+@pytest.mark.parametrize("key", ["s1", "s2", "s3"])
+def test_semi_two_phase(gpu, oracle_c, key):
+    """Seeded water/air states, 3 semi-implicit steps (1D runs into the
+    400-sweep cap): same dt and Jacobi counts as oracle and reference,
+    fields within the rounding-floor bar."""
+    d = np.load(os.path.join(GOLDEN, "two_phase_ref.npz"))
+    build, rs, nsteps, _ = problems.two_phase_cases()[key]
+    cfg, mesh, st, eos = build()
+    sg, sc, hist = _pair(oracle_c, cfg, mesh, st, eos, HLLCSolver(eos), nsteps)
+    for k, (dg, dc, ig, ic) in enumerate(hist):
+        assert abs(dg - dc) <= 1e-13 * dc and abs(dg - d[f"{key}_dts"][k]) <= 1e-13 * dc
+        assert ig == ic == d[f"{key}_iters"][k]
+    _check_floor(mesh, sg, sc, lambda fid: d[f"{key}_f{fid}"])
+
+
+def test_bubble_3d_golden(gpu, oracle_c):
+    """Air bubble in water (unit cube, 12^3, slip walls), 3 semi-implicit
+    steps: oracle, and the reference's own output (iteration counts, dt)."""
+    d = np.load(os.path.join(GOLDEN, "bubble3d_semi.npz"))
+    cfg, mesh, st, eos, rs = problems.bubble_3d(12, perturb=True)
+    sg, sc, hist = _pair(oracle_c, cfg, mesh, st, eos, rs, 3)
+    for k, (dg, dc, ig, ic) in enumerate(hist):
+        assert ig == ic == d["iters"][k]
+        assert abs(dg - dc) <= 1e-15 * dc and abs(dg - d["dts"][k]) <= 1e-15 * d["dts"][k]
+    _check_floor(mesh, sg, sc, lambda fid: mesh.interior(d[f"f{fid}"]))
+
+
+@pytest.mark.parametrize("tol,maxit", [(1e-6, 1000), (1e-300, 7), (1e-2, 1000)])
+def test_jacobi_standalone(gpu, oracle_c, tol, maxit):
+    """sifv_jacobi_solve: iteration count, residual history and the whole
+    array (ghosts as the reference leaves them) against the oracle."""
+    cfg, mesh, rho, rhoc2, rhs, p0, dt = problems.jacobi_problem()
+    eos = IdealGasEOS(1.4, 1.0, cfg)
+    cfg.semiImplicitParams.maxPressureIters = maxit
+    g = SemiImplicitSolver(mesh, HLLCSolver(eos), JacobiPressureSolver(), eos, None, cfg)
+    js = JacobiPressureSolver(g)
+    pg = p0.copy()
+    hg = np.zeros(maxit)
+    ig = js.solve(mesh, rho, rhoc2, rhs, pg, dt, tol, maxit, history=hg)
+    cpu = oracle_c.CpuSolver(mesh, cfg, 2, eos, "c")
+    pc = p0.copy()
+    hc = np.zeros(maxit)
+    ic = cpu.jacobi(rho, rhoc2, rhs, pc, dt, tol, maxit, hc)
+    assert ig == ic
+    assert np.abs(hg[:ig] - hc[:ic]).max() <= 1e-9 * max(hc[0], 1e-300) + 1e-300
+    l1, li = rel_errs(pg, pc)
+    assert l1 <= 1e-14 and li <= 1e-14, (l1, li)
+    if tol == 1e-6:
+        dgold = np.load(os.path.join(GOLDEN, "jacobi_3d.npz"))
+        assert ig == int(dgold["it_tol1e-06"][0])
+        l1, li = rel_errs(mesh.interior(pg), mesh.interior(dgold["p_tol1e-06"]))
+        assert l1 <= 1e-14 and li <= 1e-14
+    g.close()
