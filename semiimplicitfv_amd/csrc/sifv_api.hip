@@ -161,6 +161,13 @@ struct sifv_handle {
     int residCap = 0;
     double* hostResid = nullptr;
     int lastIters = 0;
+    // IGR: right-hand side, diagonal, padded rho; wavefront tiles and flags
+    double *igrRhs = nullptr, *igrDiag = nullptr, *igrRho = nullptr;
+    int* igrOrder = nullptr;
+    int* igrFlags = nullptr;   // [ntiles] tile epochs, [ntiles] error flag
+    int igrNtx = 0, igrNty = 0, igrTj = 1, igrTiles = 0;
+    int igrEpoch = 0;
+    bool sigPrim = false;
 
     size_t tsize() const { return precision == SIFV_MIXED ? sizeof(float) : sizeof(double); }
     Grid grid() const {
@@ -195,6 +202,8 @@ struct sifv_handle {
 };
 
 namespace {
+
+int run_igr_any(sifv_handle* h, int src, const int* active);
 
 int ensure_partials(sifv_handle* h, int n) {
     if (n <= h->npartialCap) return SIFV_OK;
@@ -298,6 +307,7 @@ int launch_stage(sifv_handle* h, int src, int u0, int dst, const double* coef, b
     P.eps = h->cfg.wenoEps;
     P.igr = h->cfg.useIGR ? 1 : 0;
     P.weno = (h->cfg.reconOrder <= SIFV_WENO5) ? 1 : 0;
+    P.sigPrim = h->sigPrim ? 1 : 0;
     P.pres = h->pres;
     for (int d = 0; d < 3; ++d) P.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
     P.pAdv = h->jac[4];
@@ -511,6 +521,8 @@ int enqueue_explicit_step(sifv_handle* h, double targetDt, bool loop, double end
     const int* active = loop ? h->flags : nullptr;
     auto stage = [&](int s, int src, int u0, int dst) {
         bool last = s == ord - 1;
+        int irc = run_igr_any(h, src, active);
+        if (irc) return irc;
         return h->precision == SIFV_MIXED
                    ? launch_stage<float>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active, false)
                    : launch_stage<double>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active, false);
@@ -634,6 +646,8 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
     }
     const int nb = elementwise_blocks(h);
     for (int s = 0; s < ord; ++s) {
+        rc = run_igr_any(h, src[s], nullptr);
+        if (rc) return rc;
         rc = launch_stage<double>(h, src[s], u0[s], dst[s], kRK[ord - 1][s], ord == 1, false, nullptr, true);
         if (rc) return rc;
         double* rho = h->consField<double>(dst[s], 0);
@@ -697,6 +711,121 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
     return SIFV_OK;
 }
 
+// Wavefront tiling of the IGR sweep: 64 columns x tj rows of one plane.
+// tj minimises the critical path (tile diagonals x steps per tile).
+int setup_igr(sifv_handle* h) {
+    const int ntx = (h->nx + kWave - 1) / kWave;
+    long best = -1;
+    int bestTj = 1;
+    for (int tj = 1; tj <= 256; tj *= 2) {
+        if (h->dim == 1 && tj > 1) break;
+        const int nty = (h->ny + tj - 1) / tj;
+        long cost = (long)(ntx + nty + h->nz - 2) * (tj + kWave - 1);
+        if (best < 0 || cost < best) { best = cost; bestTj = tj; }
+        if (tj >= h->ny) break;
+    }
+    h->igrTj = bestTj;
+    h->igrNtx = ntx;
+    h->igrNty = (h->ny + bestTj - 1) / bestTj;
+    h->igrTiles = ntx * h->igrNty * h->nz;
+    std::vector<int> order(h->igrTiles);
+    for (int t = 0; t < h->igrTiles; ++t) order[t] = t;
+    auto diag = [&](int t) { return t % ntx + (t / ntx) % h->igrNty + t / (ntx * h->igrNty); };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return diag(a) < diag(b); });
+    const size_t nb = sizeof(double) * h->total;
+    HIPCHK(hipMalloc(&h->igrRhs, nb));
+    HIPCHK(hipMalloc(&h->igrDiag, nb));
+    HIPCHK(hipMalloc(&h->igrRho, nb));
+    HIPCHK(hipMemset(h->igrRhs, 0, nb));
+    HIPCHK(hipMemset(h->igrDiag, 0, nb));
+    HIPCHK(hipMemset(h->igrRho, 0, nb));
+    HIPCHK(hipMalloc(&h->igrOrder, sizeof(int) * h->igrTiles));
+    HIPCHK(hipMemcpy(h->igrOrder, order.data(), sizeof(int) * h->igrTiles, hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&h->igrFlags, sizeof(int) * (h->igrTiles + 1)));
+    HIPCHK(hipMemset(h->igrFlags, 0, sizeof(int) * (h->igrTiles + 1)));
+    h->igrEpoch = 0;
+    return SIFV_OK;
+}
+
+// IGR solve at the start of an RK stage on conservative buffer `src`
+// (ExplicitSolver::step, src/ExplicitSolver.cpp:117-121 -> IGRSolver::
+// solveEntropicPressure, src/IGR.cpp:102-158): IGRWarmStartIters sweeps at
+// config.step == 0, IGRIters otherwise.
+template <typename T>
+int run_igr(sifv_handle* h, int src, const int* active) {
+    const auto& ip = h->cfg.igrParams;
+    const int iters = h->cfg.step == 0 ? ip.IGRWarmStartIters : ip.IGRIters;
+    h->sigPrim = iters == 0;
+    if (iters == 0) return SIFV_OK;
+    IgrPreArgs<T> A;
+    A.P.g = h->grid();
+    A.P.eos = h->eosd();
+    A.P.mix = h->mix();
+    A.P.total = h->total;
+    A.P.src = static_cast<const T*>(h->cons[src]);
+    // alpha = alphaCoeff * dx(0)^2 (src/IGR.cpp:45)
+    double dx0 = 0.0;
+    HIPCHK(hipMemcpy(&dx0, h->d_dx + h->ngx, sizeof(double), hipMemcpyDeviceToHost));
+    A.alpha = ip.alphaCoeff * dx0 * dx0;
+    A.rhs = h->igrRhs;
+    A.diag = h->igrDiag;
+    A.rho = h->igrRho;
+    IgrPreFn<T> pre = igr_pre_kernel_for<T>(h->dim, h->nph);
+    if (!pre) return fail(SIFV_ERR_UNSUPPORTED, "no IGR kernel for this configuration");
+    void* pargs[] = {&A};
+    int rc = timed_launch(h, reinterpret_cast<const void*>(pre), dim3(elementwise_blocks(h)), dim3(256), pargs);
+    if (rc) return rc;
+    double* f[1];
+    int comp[1] = {0};
+    f[0] = h->igrRho;
+    rc = fill_dirs(h, f, comp, 1, true);
+    if (rc) return rc;
+    // 1-deep sigma ghosts as the stage-start fill leaves them
+    f[0] = h->sigma;
+    rc = fill_dirs(h, f, comp, 1, true);
+    if (rc) return rc;
+    IgrSweepArgs S;
+    S.g = h->grid();
+    S.sg = h->sigma;
+    S.rho = h->igrRho;
+    S.rhs = h->igrRhs;
+    S.diag = h->igrDiag;
+    S.alpha = A.alpha;
+    S.ntx = h->igrNtx;
+    S.nty = h->igrNty;
+    S.tj = h->igrTj;
+    S.ntiles = h->igrTiles;
+    S.order = h->igrOrder;
+    S.flags = h->igrFlags;
+    S.err = h->igrFlags + h->igrTiles;
+    S.active = active;
+    IgrSweepFn sweep = igr_sweep_kernel_for(h->dim);
+    const int nblk = std::min(h->igrTiles, 512);
+    for (int it = 0; it < iters; ++it) {
+        S.epoch = ++h->igrEpoch;
+        void* sargs[] = {&S};
+        rc = timed_launch(h, reinterpret_cast<const void*>(sweep), dim3(nblk), dim3(kWave), sargs);
+        if (rc) return rc;
+        rc = fill_dirs(h, f, comp, 1, true);
+        if (rc) return rc;
+    }
+    return SIFV_OK;
+}
+
+int run_igr_any(sifv_handle* h, int src, const int* active) {
+    if (!h->cfg.useIGR) return SIFV_OK;
+    return h->precision == SIFV_MIXED ? run_igr<float>(h, src, active) : run_igr<double>(h, src, active);
+}
+
+// The IGR wavefront raises its error flag only when a wait ran out of budget.
+int check_igr(sifv_handle* h) {
+    if (!h->cfg.useIGR) return SIFV_OK;
+    int e = 0;
+    HIPCHK(hipMemcpy(&e, h->igrFlags + h->igrTiles, sizeof(int), hipMemcpyDeviceToHost));
+    if (e) return fail(SIFV_ERR_HIP, "IGR wavefront wait exceeded its budget");
+    return SIFV_OK;
+}
+
 int check_step_config(sifv_handle* h, const sifv_config* cfg) {
     if (cfg) {
         // per-call fields of SimulationConfig (step/time, and parameters the
@@ -736,7 +865,6 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
         if (mesh->bc[f] < 0 || mesh->bc[f] > 4) return fail(SIFV_ERR_INVALID, "unknown boundary condition");
     if (cfg->multiPhaseParams.nPhases != 0 && cfg->multiPhaseParams.nPhases != 2)
         return fail(SIFV_ERR_UNSUPPORTED, "the GPU path carries single-phase or two-phase states");
-    if (cfg->useIGR) return fail(SIFV_ERR_UNSUPPORTED, "IGR is not on the GPU path yet");
     if (precision == SIFV_MIXED && (cfg->semiImplicit || cfg->multiPhaseParams.nPhases > 0))
         return fail(SIFV_ERR_UNSUPPORTED, "mixed precision is available for the single-phase explicit path only");
 
@@ -828,6 +956,13 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
         CK(hipMalloc(&h->part, h->tsize() * h->total * h->nfc));
     }
 #undef CK
+    if (cfg->useIGR) {
+        rc = setup_igr(h);
+        if (rc) {
+            sifv_destroy(h);
+            return rc;
+        }
+    }
     h->dtNextValid = false;
     h->primFresh = true;
     *out = h;
@@ -853,6 +988,11 @@ int sifv_destroy(sifv_handle* h) {
     hipFree(h->scal);
     hipFree(h->flags);
     hipFree(h->part);
+    hipFree(h->igrRhs);
+    hipFree(h->igrDiag);
+    hipFree(h->igrRho);
+    hipFree(h->igrOrder);
+    hipFree(h->igrFlags);
     for (int q = 0; q < 6; ++q) hipFree(h->jac[q]);
     hipFree(h->resid);
     if (h->hostResid) hipHostFree(h->hostResid);
@@ -956,6 +1096,8 @@ int sifv_explicit_step(sifv_handle* h, const sifv_config* cfg, double targetDt, 
     HIPCHK(hipStreamSynchronize(h->stream));
     rc = harvest_timing(h);
     if (rc) return rc;
+    rc = check_igr(h);
+    if (rc) return rc;
     if (dt_out) *dt_out = h->hostPinned[0];
     if (!std::isfinite(h->hostPinned[0])) return fail(SIFV_ERR_NONFINITE, "non-finite time step");
     return SIFV_OK;
@@ -969,15 +1111,22 @@ int sifv_explicit_advance(sifv_handle* h, const sifv_config* cfg, int nsteps, do
     check_step_config(h, cfg);
     hipLaunchKernelGGL(loop_reset_kernel, dim3(1), dim3(1), 0, h->stream, h->scal, h->flags, *time_inout);
     HIPCHK(hipGetLastError());
+    // runTimeLoop advances config.step once per step (src/RKTimeStepping.cpp:266);
+    // only the IGR warm start reads it
+    const int step0 = h->cfg.step;
     for (int s = 0; s < nsteps; ++s) {
+        h->cfg.step = step0 + s;
         int rc = enqueue_explicit_step(h, -1.0, true, endTime);
         if (rc) return rc;
     }
+    h->cfg.step = step0;
     HIPCHK(hipMemcpyAsync(h->hostPinned, h->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     int* hflags = reinterpret_cast<int*>(h->hostPinned + 2);
     HIPCHK(hipMemcpyAsync(hflags, h->flags, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     int rc = harvest_timing(h);
+    if (rc) return rc;
+    rc = check_igr(h);
     if (rc) return rc;
     *time_inout = h->hostPinned[1];
     if (steps_done) *steps_done = hflags[1];
@@ -1102,6 +1251,8 @@ int sifv_semi_implicit_step(sifv_handle* h, const sifv_config* cfg, double targe
     HIPCHK(hipMemcpyAsync(h->hostPinned, h->scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     rc = harvest_timing(h);
+    if (rc) return rc;
+    rc = check_igr(h);
     if (rc) return rc;
     if (dt_out) *dt_out = h->hostPinned[0];
     if (pressure_iters) *pressure_iters = h->lastIters;

@@ -3,6 +3,7 @@
 // (sifv_kernels_*.hip) so they compile in parallel.
 #pragma once
 
+#include "sifv_igr.hpp"
 #include "sifv_stage.hpp"
 
 namespace sifv {
@@ -24,5 +25,18 @@ StageKernels<double> stage_kernels_2d(int R, int rs, int nph, bool semi);
 StageKernels<double> stage_kernels_3d(int R, int rs, int nph);
 StageKernels<double> stage_kernels_3d_semi(int R, int rs, int nph);
 StageKernels<float> stage_kernels_fp32(int dim, int R, int rs);
+
+// IGR: pre-pass (rhs, diagonal, rho) per dimension / phase count, and the
+// wavefront Gauss-Seidel sweep per dimension.
+template <typename T>
+using IgrPreFn = void (*)(IgrPreArgs<T>);
+using IgrSweepFn = void (*)(IgrSweepArgs);
+template <typename T>
+IgrPreFn<T> igr_pre_kernel_for(int dim, int nph);
+template <>
+IgrPreFn<double> igr_pre_kernel_for<double>(int dim, int nph);
+template <>
+IgrPreFn<float> igr_pre_kernel_for<float>(int dim, int nph);
+IgrSweepFn igr_sweep_kernel_for(int dim);
 
 }  // namespace sifv

@@ -73,6 +73,7 @@ struct StageParams {
     double* dtPartial;    // per-block min of the new state's acoustic cell dt, or null
     double eps;
     int igr, weno;
+    int sigPrim;          // sigma ghosts follow the PRIM rule (no IGR sweep ran this stage)
     int rows;             // rows (y) or planes (z) marched per wave
     int nseg;             // segments per strip
     int nstrip;           // strips per row / plane
@@ -172,10 +173,22 @@ __device__ __forceinline__ void load_cell(const StageParams<T>& P, const T* buf,
 template <int DIM, typename T>
 __device__ __forceinline__ double load_sigma(const StageParams<T>& P, int i, int j, int k) {
     const Grid& g = P.g;
-    // sigma ghosts are refreshed by the IGR solve's scalar fill
-    int si = map_axis_scalar(clampi(i, -g.ngx, g.nx - 1 + g.ngx), g.nx, g.sbc[0], g.sbc[1]);
-    int sj = DIM >= 2 ? map_axis_scalar(clampi(j, -g.ngy, g.ny - 1 + g.ngy), g.ny, g.sbc[2], g.sbc[3]) : 0;
-    int sk = DIM >= 3 ? map_axis_scalar(clampi(k, -g.ngz, g.nz - 1 + g.ngz), g.nz, g.sbc[4], g.sbc[5]) : 0;
+    i = clampi(i, -g.ngx, g.nx - 1 + g.ngx);
+    if (DIM >= 2) j = clampi(j, -g.ngy, g.ny - 1 + g.ngy);
+    if (DIM >= 3) k = clampi(k, -g.ngz, g.nz - 1 + g.ngz);
+    int si, sj = 0, sk = 0;
+    if (P.sigPrim) {
+        // ghosts as the stage-start PRIM fill leaves them (no sign flips for a scalar)
+        bool fn = false, ft = false;
+        si = map_axis(i, g.nx, g.bc[0], g.bc[1], fn, ft);
+        if (DIM >= 2) sj = map_axis(j, g.ny, g.bc[2], g.bc[3], fn, ft);
+        if (DIM >= 3) sk = map_axis(k, g.nz, g.bc[4], g.bc[5], fn, ft);
+    } else {
+        // ghosts refreshed by the IGR solve's scalar fill
+        si = map_axis_scalar(i, g.nx, g.sbc[0], g.sbc[1]);
+        if (DIM >= 2) sj = map_axis_scalar(j, g.ny, g.sbc[2], g.sbc[3]);
+        if (DIM >= 3) sk = map_axis_scalar(k, g.nz, g.sbc[4], g.sbc[5]);
+    }
     return P.sigma[cell_index(g, si, sj, sk)];
 }
 

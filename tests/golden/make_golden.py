@@ -23,7 +23,7 @@ from semiimplicitfv_amd import (BoundaryCondition, IdealGasEOS, ReconstructionOr
 from semiimplicitfv_amd.cases import _fill  # noqa: E402
 from semiimplicitfv_amd.timeloop import run_time_loop  # noqa: E402
 from tests.golden.problems import (bubble_3d, jacobi_problem, liquid_gas_tube, random_ghost_case,  # noqa: E402
-                                   recon_lines, riemann_pairs, two_phase_cases)
+                                   igr_cases, recon_lines, riemann_pairs, two_phase_cases)
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 
@@ -140,11 +140,9 @@ def g_liquid_gas():
     save("liquid_gas_1d.npz", **out)
 
 
-def g_two_phase():
-    """Interior fields, time steps and Jacobi counts of the seeded water/air
-    cases (explicit and semi-implicit, 1D-3D)."""
+def _run_cases(table, name):
     out = {}
-    for key, (build, rs, nsteps, semi) in two_phase_cases().items():
+    for key, (build, rs, nsteps, semi) in table.items():
         cfg, mesh, st, eos = build()
         s = po.CpuSolver(mesh, cfg, rs, eos, "ref")
         dts, its = [], []
@@ -160,11 +158,23 @@ def g_two_phase():
         out[f"{key}_iters"] = np.array(its, dtype=np.int64)
         for fid in st.field_ids():
             out[f"{key}_f{fid}"] = np.ascontiguousarray(mesh.interior(st.field(fid)))
-    save("two_phase_ref.npz", **out)
+    save(name, **out)
+
+
+def g_two_phase():
+    """Interior fields, time steps and Jacobi counts of the seeded water/air
+    cases (explicit and semi-implicit, 1D-3D)."""
+    _run_cases(two_phase_cases(), "two_phase_ref.npz")
+
+
+def g_igr():
+    """The same for the IGR cases (explicit 1D-3D, semi-implicit 2D)."""
+    _run_cases(igr_cases(), "igr_ref.npz")
 
 
 GENERATORS = {"recon": g_recon, "riemann": g_riemann, "sod": g_sod, "vortex": g_vortex, "ghost": g_ghost,
-              "jacobi": g_jacobi, "semi": g_semi, "liquid_gas": g_liquid_gas, "two_phase": g_two_phase}
+              "jacobi": g_jacobi, "semi": g_semi, "liquid_gas": g_liquid_gas, "two_phase": g_two_phase,
+              "igr": g_igr}
 
 if __name__ == "__main__":
     assert po.available("ref"), "build oracle/_ref first (make -C oracle)"

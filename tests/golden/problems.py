@@ -8,6 +8,7 @@ import numpy as np
 from semiimplicitfv_amd import (BoundaryCondition, HLLCSolver, IdealGasEOS, PhaseEOS, RectilinearMesh,
                                 ReconstructionOrder, SimulationConfig, SolutionState, StiffenedGasEOS,
                                 uniform_mesh)
+from semiimplicitfv_amd.cases import _fill
 
 WATER = PhaseEOS(4.4, 6.0e8)
 AIR = PhaseEOS(1.4, 0.0)
@@ -322,3 +323,79 @@ def two_phase_cases():
         out[f"s{dim}"] = ((lambda dim=dim: make_two_phase_case(dim, ssizes[dim], bcs=[BoundaryCondition.SlipWall] * 6,
                                                                seed=10 + dim, semi=True, p_jump=False)), 2, 3, True)
     return out
+
+
+def make_case(dim, n, bcs=None, periods=(0, 0, 0), recon=2, rk=3, seed=0, nonuniform=False, eos_kind="ideal",
+              ng=None, cfl=0.5, shock=True):
+    """A random but well-posed state: smooth background + optional jump."""
+    rng = np.random.default_rng(seed)
+    cfg = SimulationConfig(dim=dim, nGhost=ng if ng else 3, RKOrder=rk, reconOrder=recon)
+    cfg.explicitParams.cfl = cfl
+    cfg.explicitParams.maxDt = 1.0
+    cfg.validate()
+    nodes = []
+    for d in range(3):
+        m = n[d] if d < dim else 1
+        if d >= dim:
+            nodes.append(np.array([0.0, 1.0]))
+            continue
+        if nonuniform:
+            w = 1.0 + 0.5 * rng.random(m)
+            x = np.concatenate([[0.0], np.cumsum(w)])
+            x = x / x[-1]
+        else:
+            x = np.linspace(0.0, 1.0, m + 1)
+        nodes.append(x)
+    mesh = RectilinearMesh(cfg, nodes[0], nodes[1], nodes[2], periods=periods)
+    if bcs is not None:
+        for f, b in enumerate(bcs):
+            mesh.setBoundaryCondition(f, b)
+    st = SolutionState()
+    st.allocate(mesh.totalCells(), cfg)
+    if eos_kind == "stiff":
+        eos = StiffenedGasEOS(4.4, 6e3, 1.0, cfg)
+    else:
+        eos = IdealGasEOS(1.4, 1.0, cfg)
+    xc, yc, zc = mesh.centroids()
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * X + 0.3) * np.cos(2 * np.pi * Y) + 0.1 * np.cos(2 * np.pi * Z)
+    p = 1.0 + 0.3 * np.cos(2 * np.pi * (X + Y + Z))
+    if shock:
+        jump = (X + 0.3 * Y + 0.2 * Z) < 0.45
+        rho = np.where(jump, rho * 1.8, rho)
+        p = np.where(jump, p * 3.0, p)
+    u = [0.3 * np.sin(2 * np.pi * (Y + 0.1 * d)) + 0.1 * (d + 1) for d in range(dim)]
+    _fill(mesh, st, eos, rho, u, p)
+    return cfg, mesh, st, eos
+
+
+def igr_cases():
+    """Seeded IGR cases whose reference outputs are igr_ref.npz:
+    key -> (builder, Riemann kind (0 LF, 1 Rusanov, 2 HLLC), steps, semi).
+    Step 0 runs IGRWarmStartIters sweeps, later steps IGRIters."""
+    def build(dim, n, rs_semi, warm=6, iters=3, alpha=1.0, seed=0, periods=(0, 0, 0), bcs=None, semi=False):
+        cfg, mesh, st, eos = make_case(dim, n, bcs=bcs, periods=periods, seed=seed, nonuniform=True, cfl=0.4)
+        cfg.useIGR = True
+        cfg.igrParams.alphaCoeff = alpha
+        cfg.igrParams.IGRIters = iters
+        cfg.igrParams.IGRWarmStartIters = warm
+        if semi:
+            cfg.semiImplicit = True
+            cfg.semiImplicitParams.cfl = 0.5
+            cfg.semiImplicitParams.maxDt = 1e-2
+            cfg.semiImplicitParams.pressureTol = 1e-9
+            cfg.semiImplicitParams.maxPressureIters = 300
+        cfg.validate()
+        return cfg, mesh, st, eos
+
+    walls = [BoundaryCondition.SlipWall, BoundaryCondition.Outflow, BoundaryCondition.NoSlipWall,
+             BoundaryCondition.Symmetry, BoundaryCondition.SlipWall, BoundaryCondition.Outflow]
+    return {
+        "e1_lf": (lambda: build(1, (150,), 0, seed=1, bcs=walls), 0, 4, False),
+        "e1_hllc_p": (lambda: build(1, (90,), 2, seed=2, periods=(1, 0, 0)), 2, 3, False),
+        "e2_lf": (lambda: build(2, (70, 21), 0, seed=3, periods=(1, 0, 0), bcs=walls), 0, 3, False),
+        "e2_rus": (lambda: build(2, (33, 40), 1, seed=4, bcs=walls), 1, 3, False),
+        "e2_lf_w0": (lambda: build(2, (26, 19), 0, warm=0, seed=5, bcs=walls), 0, 3, False),
+        "e3_lf": (lambda: build(3, (12, 10, 9), 0, seed=6, periods=(0, 0, 1), bcs=walls), 0, 3, False),
+        "s2_lf": (lambda: build(2, (30, 22), 0, seed=7, bcs=walls, semi=True), 0, 3, True),
+    }
