@@ -190,6 +190,56 @@ int sifv_synchronize(sifv_handle* h);
  * 0 restores the built-in choice.  Used by the autotuner. */
 int sifv_set_tile(sifv_handle* h, int rows_per_wave);
 
+/* ---- discrete adjoint ----------------------------------------------------- */
+/* No counterpart in the reference (it has no derivatives).  The transpose of
+ * the explicit solver's step map for doubly periodic 2D meshes with WENO5,
+ * Rusanov, SSP-RK3, a fixed time step (explicitParams.constDt > 0), single-
+ * phase ideal gas, FP64.  sifv_adjoint_forward runs nsteps steps from the
+ * resident state (as sifv_explicit_step would) and keeps checkpoints every
+ * checkpoint_every steps; sifv_adjoint_backward recomputes each segment from
+ * its checkpoint and returns grad = (dU^N/dU^0)^T seed and
+ * dgamma = seed . dU^N/dgamma.  seed / grad: 4 fields (rho, rhoU, rhoV, rhoE)
+ * x totalCells, padded layout, device pointers (the _host form takes host
+ * arrays); grad is 0 on ghost cells. */
+int sifv_adjoint_forward(sifv_handle* h, const sifv_config* cfg, int nsteps, int checkpoint_every);
+int sifv_adjoint_backward(sifv_handle* h, const double* seed, double* grad, double* dgamma);
+int sifv_adjoint_backward_host(sifv_handle* h, const double* seed, double* grad, double* dgamma);
+/* Device pointer of the resident conservative state (field q at q*totalCells). */
+int sifv_state_device_ptr(sifv_handle* h, double** ptr);
+
+/* ---- batched 1D runs ------------------------------------------------------ */
+/* No counterpart in the reference, which runs one problem per process: many
+ * independent 1D explicit runs (each what ExplicitSolver::step driven by
+ * runTimeLoop without output does on its own 1D mesh with outflow ends,
+ * src/ExplicitSolver.cpp:68-203, src/RKTimeStepping.cpp:241-356) advanced to
+ * one end time in a single persistent launch.  Tubes have uneven cell counts;
+ * fields are packed tube after tube (interior cells only, no ghosts).
+ * nodes: ncells[t]+1 node coordinates per tube, packed.  cfg: dim 1, WENO5,
+ * RKOrder 2 or 3, single phase, explicit, no IGR (cfl, maxDt, constDt,
+ * wenoEps are read).  precision: SIFV_FP64 or SIFV_MIXED. */
+typedef struct sifv_ensemble sifv_ensemble;
+int sifv_ensemble_create(int ntubes, const int* ncells, const double* nodes, const sifv_config* cfg, int riemann,
+                         const sifv_eos* eos, int precision, sifv_ensemble** out);
+int sifv_ensemble_destroy(sifv_ensemble* e);
+int sifv_ensemble_total_cells(const sifv_ensemble* e, size_t* n);
+/* fields: SIFV_F_RHO, SIFV_F_RHOU, SIFV_F_RHOE (state), SIFV_F_VELU, SIFV_F_PRES
+ * (initial primitives: the first time step reads them, as the reference's
+ * first step reads SolutionState's primitive arrays).  Download of VELU/PRES
+ * converts the current state. */
+int sifv_ensemble_upload(sifv_ensemble* e, int field, const double* host, size_t n);
+int sifv_ensemble_download(sifv_ensemble* e, int field, double* host, size_t n);
+/* Tile shape: workgroup size (64..1024, one cell per thread per tile) and
+ * resident workgroups per CU. */
+int sifv_ensemble_set_shape(sifv_ensemble* e, int block, int wg_per_cu);
+/* Advances every tube from time 0 to endTime (at most maxSteps steps each);
+ * steps/times (ntubes entries, may be NULL) receive each tube's step count and
+ * final time, kernel_ms the launch's duration. */
+int sifv_ensemble_run(sifv_ensemble* e, double endTime, int maxSteps, int* steps, double* times, double* kernel_ms);
+/* Runs every candidate shape on the current state (restored after each
+ * trial, at most maxSteps steps per tube) and keeps the fastest. */
+int sifv_ensemble_autotune(sifv_ensemble* e, double endTime, int maxSteps, int* block_out, int* wg_out,
+                           double* best_ms);
+
 #ifdef __cplusplus
 }
 #endif

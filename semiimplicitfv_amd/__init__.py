@@ -12,10 +12,11 @@ from .mesh import RectilinearMesh, runtime_linspace, uniform_mesh
 from .state import (HLLCSolver, IdealGasEOS, LFSolver, RusanovSolver, SolutionState, StiffenedGasEOS)
 from .solvers import ExplicitSolver, JacobiPressureSolver, SemiImplicitSolver
 from . import _abi
+from .ensemble import RiemannEnsemble
 
 __all__ = [
     "BoundaryCondition", "ExplicitParams", "IGRParams", "MultiPhaseParams", "PhaseEOS", "ReconstructionOrder",
     "RiemannSolverType", "SemiImplicitParams", "SimulationConfig", "VarSet", "RectilinearMesh", "runtime_linspace",
     "uniform_mesh", "HLLCSolver", "IdealGasEOS", "LFSolver", "RusanovSolver", "SolutionState", "StiffenedGasEOS",
-    "ExplicitSolver", "JacobiPressureSolver", "SemiImplicitSolver",
+    "ExplicitSolver", "JacobiPressureSolver", "SemiImplicitSolver", "RiemannEnsemble",
 ]

@@ -127,6 +127,18 @@ def _declare(lib):
         "sifv_timing_read": (i, [vp, P(d), P(C.c_long), i]),
         "sifv_synchronize": (i, [vp]),
         "sifv_set_tile": (i, [vp, i]),
+        "sifv_adjoint_forward": (i, [vp, P(Config_C), i, i]),
+        "sifv_adjoint_backward": (i, [vp, vp, vp, P(d)]),
+        "sifv_adjoint_backward_host": (i, [vp, P(d), P(d), P(d)]),
+        "sifv_state_device_ptr": (i, [vp, P(vp)]),
+        "sifv_ensemble_create": (i, [i, P(i), P(d), P(Config_C), i, P(Eos_C), i, P(vp)]),
+        "sifv_ensemble_destroy": (i, [vp]),
+        "sifv_ensemble_total_cells": (i, [vp, P(sz)]),
+        "sifv_ensemble_upload": (i, [vp, i, P(d), sz]),
+        "sifv_ensemble_download": (i, [vp, i, P(d), sz]),
+        "sifv_ensemble_set_shape": (i, [vp, i, i]),
+        "sifv_ensemble_run": (i, [vp, d, i, P(i), P(d), P(d)]),
+        "sifv_ensemble_autotune": (i, [vp, d, i, P(i), P(i), P(d)]),
     }
     for name, (res, args) in sigs.items():
         if hasattr(lib, name):

@@ -19,6 +19,7 @@
 #include "sifv_aux.hpp"
 #include "sifv_kernels.hpp"
 #include "sifv_semi.hpp"
+#include "sifv_adjoint.hpp"
 
 using namespace sifv;
 
@@ -123,6 +124,11 @@ struct Timing {
 
 }  // namespace
 
+namespace sifv {
+// Error message of the last failing call, shared with the other C-ABI units.
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace sifv
+
 struct sifv_handle {
     int dim, nx, ny, nz, ng, ngx, ngy, ngz, nxT, nyT, nzT;
     size_t total;
@@ -168,6 +174,15 @@ struct sifv_handle {
     int igrNtx = 0, igrNty = 0, igrTj = 1, igrTiles = 0;
     int igrEpoch = 0;
     bool sigPrim = false;
+    bool unifY = false;       // all interior y (and z) widths are the same double
+    // discrete adjoint: checkpoints of the forward run and work arrays
+    std::vector<double*> adjCk;   // states at steps 0, k, 2k, ...
+    std::vector<double*> adjSeg;  // states of one segment during the backward sweep
+    double* adjWork = nullptr;    // W (4), pbP (8), pbM (8), contrib (40), U1, U2, L1, L2, lam (4 each)
+    double* adjGpart = nullptr;
+    double* adjGbar = nullptr;
+    int adjN = 0, adjK = 0, adjNb = 0;
+    double adjDt = 0.0;
 
     size_t tsize() const { return precision == SIFV_MIXED ? sizeof(float) : sizeof(double); }
     Grid grid() const {
@@ -288,19 +303,22 @@ int timed_launch(sifv_handle* h, const void* fn, dim3 grid, dim3 block, void** a
 // One RK stage of the fused kernel(s).  semi: advective fluxes + star update
 // (writes pres / vel / pAdv) instead of the explicit update.
 template <typename T>
-int launch_stage(sifv_handle* h, int src, int u0, int dst, const double* coef, bool rk1, bool last,
-                 const int* active, bool semi) {
+int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const double* coef, bool rk1, bool last,
+                   const int* active, bool semi) {
     StageParams<T> P;
     P.g = h->grid();
     P.eos = h->eosd();
     P.mix = h->mix();
     P.total = h->total;
-    P.src = static_cast<const T*>(h->cons[src]);
-    P.u0 = static_cast<const T*>(h->cons[u0 >= 0 ? u0 : src]);
-    P.dst = static_cast<T*>(h->cons[dst]);
+    P.src = srcp;
+    P.u0 = u0p ? u0p : srcp;
+    P.dst = dstp;
     P.part = static_cast<T*>(h->part);
     P.sigma = h->sigma;
     P.c1 = coef[0]; P.c2 = coef[1]; P.c3 = coef[2]; P.c4 = coef[3];
+    P.c4inv = 1.0 / coef[3];
+    P.c4pow2 = (coef[3] == 1.0 || coef[3] == 2.0 || coef[3] == 4.0) ? 1 : 0;
+    P.unifY = h->unifY ? 1 : 0;
     P.rk1 = rk1 ? 1 : 0;
     P.dt = h->scal;
     P.active = active;
@@ -350,6 +368,14 @@ int launch_stage(sifv_handle* h, int src, int u0, int dst, const double* coef, b
         if (wantDt) h->npartialValid = nbz;
     }
     return SIFV_OK;
+}
+
+template <typename T>
+int launch_stage(sifv_handle* h, int src, int u0, int dst, const double* coef, bool rk1, bool last,
+                 const int* active, bool semi) {
+    return launch_stage_p<T>(h, static_cast<const T*>(h->cons[src]),
+                             u0 >= 0 ? static_cast<const T*>(h->cons[u0]) : nullptr, static_cast<T*>(h->cons[dst]),
+                             coef, rk1, last, active, semi);
 }
 
 int harvest_timing(sifv_handle* h) {
@@ -826,6 +852,71 @@ int check_igr(sifv_handle* h) {
     return SIFV_OK;
 }
 
+// ---- discrete adjoint ------------------------------------------------------
+int adj_check(const sifv_handle* h) {
+    const sifv_config& c = h->cfg;
+    if (h->dim != 2 || !(h->bc[0] == BK_WRAP && h->bc[2] == BK_WRAP))
+        return fail(SIFV_ERR_UNSUPPORTED, "the adjoint covers doubly periodic 2D meshes");
+    if (c.reconOrder != SIFV_WENO5 || h->riemann != SIFV_RUSANOV || c.RKOrder != 3)
+        return fail(SIFV_ERR_UNSUPPORTED, "the adjoint covers WENO5 + Rusanov + SSP-RK3");
+    if (h->nph != 0 || h->eos.kind != SIFV_EOS_IDEAL || h->precision != SIFV_FP64 || c.useIGR || c.semiImplicit)
+        return fail(SIFV_ERR_UNSUPPORTED, "the adjoint covers the single-phase ideal-gas FP64 explicit path");
+    if (!(c.explicitParams.constDt > 0.0))
+        return fail(SIFV_ERR_UNSUPPORTED, "the adjoint needs a fixed time step (explicitParams.constDt > 0)");
+    return SIFV_OK;
+}
+
+double* adj_slot(sifv_handle* h, int k) { return h->adjWork + (size_t)k * 4 * h->total; }
+
+// vjp of R at U against mu: out = sum_m a_m X_m + s * (dR/dU)^T mu, gbar += s * d(mu.R)/dgamma
+int adj_vjp(sifv_handle* h, const double* U, const double* mu, double s, double* out, const double* const* X,
+            const double* a, int nX) {
+    const size_t T = h->total;
+    AdjArgs A;
+    A.g = h->grid();
+    A.total = T;
+    A.gamma = h->eos.gamma;
+    A.eps = h->cfg.wenoEps;
+    A.U = U;
+    A.mu = mu;
+    A.W = h->adjWork;
+    A.pbP = h->adjWork + 4 * T;
+    A.pbM = h->adjWork + 12 * T;
+    A.contrib = h->adjWork + 20 * T;
+    A.gpart = h->adjGpart;
+    A.s = s;
+    A.out = out;
+    for (int m = 0; m < 3; ++m) {
+        A.X[m] = m < nX ? X[m] : nullptr;
+        A.a[m] = m < nX ? a[m] : 0.0;
+    }
+    A.nX = nX;
+    const int nb = h->adjNb;
+    A.gpartOff = 0;
+    hipLaunchKernelGGL(adj_prim_kernel, dim3(nb), dim3(256), 0, h->stream, A);
+    hipLaunchKernelGGL(adj_face_kernel<0>, dim3(nb), dim3(256), 0, h->stream, A);
+    A.gpartOff = nb;
+    hipLaunchKernelGGL(adj_face_kernel<1>, dim3(nb), dim3(256), 0, h->stream, A);
+    hipLaunchKernelGGL(adj_recon_kernel, dim3(nb), dim3(256), 0, h->stream, A);
+    A.gpartOff = 2 * nb;
+    hipLaunchKernelGGL(adj_gather_kernel, dim3(nb), dim3(256), 0, h->stream, A);
+    hipLaunchKernelGGL(adj_gsum_kernel, dim3(1), dim3(64), 0, h->stream, (const double*)h->adjGpart, 3 * nb,
+                       h->adjGbar);
+    HIPCHK(hipGetLastError());
+    return SIFV_OK;
+}
+
+// One forward RK3 step from `un` into `next` on explicit pointers, keeping the
+// stage states in the adjoint work slots (U1 = slot 15, U2 = slot 16).
+int adj_stages(sifv_handle* h, const double* un, double* next) {
+    double* U1 = adj_slot(h, 15);
+    double* U2 = adj_slot(h, 16);
+    int rc = launch_stage_p<double>(h, un, nullptr, U1, kRK[2][0], false, false, nullptr, false);
+    if (!rc) rc = launch_stage_p<double>(h, U1, un, U2, kRK[2][1], false, false, nullptr, false);
+    if (!rc && next) rc = launch_stage_p<double>(h, U2, un, next, kRK[2][2], false, false, nullptr, false);
+    return rc;
+}
+
 int check_step_config(sifv_handle* h, const sifv_config* cfg) {
     if (cfg) {
         // per-call fields of SimulationConfig (step/time, and parameters the
@@ -914,6 +1005,9 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
     for (int i = 0; i < h->nxT; ++i) dx[i] = xe[i + 1] - xe[i];
     for (int j = 0; j < h->nyT; ++j) dy[j] = ye[j + 1] - ye[j];
     for (int k = 0; k < h->nzT; ++k) dz[k] = ze[k + 1] - ze[k];
+    h->unifY = true;
+    for (int j = 1; j < h->ny; ++j) h->unifY = h->unifY && dy[h->ngy + j] == dy[h->ngy];
+    for (int k = 1; k < h->nz; ++k) h->unifY = h->unifY && dz[h->ngz + k] == dz[h->ngz];
 
     auto bail = [&](int code) {
         sifv_destroy(h);
@@ -988,6 +1082,11 @@ int sifv_destroy(sifv_handle* h) {
     hipFree(h->scal);
     hipFree(h->flags);
     hipFree(h->part);
+    for (double* c : h->adjCk) hipFree(c);
+    for (double* c : h->adjSeg) hipFree(c);
+    hipFree(h->adjWork);
+    hipFree(h->adjGpart);
+    hipFree(h->adjGbar);
     hipFree(h->igrRhs);
     hipFree(h->igrDiag);
     hipFree(h->igrRho);
@@ -1295,6 +1394,141 @@ int sifv_jacobi_solve(sifv_handle* h, const double* rho, const double* rhoc2, co
     if (iters) *iters = n;
     if (residual_history)
         for (int t = 0; t < n; ++t) residual_history[t] = h->hostResid[t];
+    return SIFV_OK;
+}
+
+// Runs nsteps explicit steps with the fixed dt from the resident state and
+// keeps the state every checkpoint_every steps for the backward sweep.
+int sifv_adjoint_forward(sifv_handle* h, const sifv_config* cfg, int nsteps, int checkpoint_every) {
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    if (nsteps < 1 || checkpoint_every < 1) return fail(SIFV_ERR_INVALID, "nsteps and checkpoint_every must be >= 1");
+    check_step_config(h, cfg);
+    int rc = adj_check(h);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * 4 * h->total;
+    const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
+    while ((int)h->adjCk.size() < nck) {
+        double* p = nullptr;
+        HIPCHK(hipMalloc(&p, bytes));
+        h->adjCk.push_back(p);
+    }
+    h->adjN = nsteps;
+    h->adjK = checkpoint_every;
+    h->adjDt = h->cfg.explicitParams.constDt;
+    for (int n = 0; n < nsteps; ++n) {
+        if (n % checkpoint_every == 0)
+            HIPCHK(hipMemcpyAsync(h->adjCk[n / checkpoint_every], h->cons[h->cur], bytes, hipMemcpyDeviceToDevice,
+                                  h->stream));
+        rc = enqueue_explicit_step(h, -1.0, false, 0.0);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return harvest_timing(h);
+}
+
+// Transposes the recorded run: grad = (dU^N/dU^0)^T seed, dgamma = seed .
+// dU^N/dgamma.  seed and grad are device arrays of 4 fields x totalCells
+// (rho, rhoU, rhoV, rhoE; ghosts ignored, written as 0).  The segment between
+// two checkpoints is recomputed forward, then each step is transposed stage
+// by stage (SSP-RK3: U1 = U + dt R(U); U2 = (U1 + 3U + dt R(U1))/4;
+// U' = (2U2 + U + 2dt R(U2))/3).
+int sifv_adjoint_backward(sifv_handle* h, const double* seed, double* grad, double* dgamma) {
+    if (!h || !seed || !grad) return fail(SIFV_ERR_INVALID, "null argument");
+    if (h->adjN < 1) return fail(SIFV_ERR_INVALID, "no recorded forward run (call sifv_adjoint_forward first)");
+    int rc = adj_check(h);
+    if (rc) return rc;
+    const size_t T = h->total;
+    const size_t bytes = sizeof(double) * 4 * T;
+    h->adjNb = (int)std::min<size_t>(((size_t)h->nx * h->ny + 255) / 256, 4096);
+    if (!h->adjWork) {
+        HIPCHK(hipMalloc(&h->adjWork, sizeof(double) * 80 * T));
+        HIPCHK(hipMemset(h->adjWork, 0, sizeof(double) * 80 * T));
+        HIPCHK(hipMalloc(&h->adjGpart, sizeof(double) * 3 * 4096));
+        HIPCHK(hipMalloc(&h->adjGbar, sizeof(double)));
+    }
+    while ((int)h->adjSeg.size() < h->adjK) {
+        double* p = nullptr;
+        HIPCHK(hipMalloc(&p, bytes));
+        h->adjSeg.push_back(p);
+    }
+    HIPCHK(hipMemsetAsync(h->adjGbar, 0, sizeof(double), h->stream));
+    HIPCHK(hipMemcpyAsync(h->scal, &h->adjDt, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    double* lam = adj_slot(h, 19);
+    double* L1 = adj_slot(h, 17);
+    double* L2 = adj_slot(h, 18);
+    HIPCHK(hipMemcpyAsync(lam, seed, bytes, hipMemcpyDeviceToDevice, h->stream));
+    const double dt = h->adjDt;
+    const int nseg = (h->adjN + h->adjK - 1) / h->adjK;
+    for (int sgm = nseg - 1; sgm >= 0; --sgm) {
+        const int n0 = sgm * h->adjK, n1 = std::min(h->adjN, n0 + h->adjK);
+        HIPCHK(hipMemcpyAsync(h->adjSeg[0], h->adjCk[sgm], bytes, hipMemcpyDeviceToDevice, h->stream));
+        for (int n = n0 + 1; n < n1; ++n) {
+            rc = adj_stages(h, h->adjSeg[n - 1 - n0], h->adjSeg[n - n0]);
+            if (rc) return rc;
+        }
+        for (int n = n1 - 1; n >= n0; --n) {
+            const double* Un = h->adjSeg[n - n0];
+            rc = adj_stages(h, Un, nullptr);
+            if (rc) return rc;
+            const double* U1 = adj_slot(h, 15);
+            const double* U2 = adj_slot(h, 16);
+            {
+                const double* X[1] = {lam};
+                const double a[1] = {2.0 / 3.0};
+                rc = adj_vjp(h, U2, lam, 2.0 * dt / 3.0, L2, X, a, 1);
+            }
+            if (!rc) {
+                const double* X[1] = {L2};
+                const double a[1] = {0.25};
+                rc = adj_vjp(h, U1, L2, 0.25 * dt, L1, X, a, 1);
+            }
+            if (!rc) {
+                const double* X[3] = {lam, L2, L1};
+                const double a[3] = {1.0 / 3.0, 0.75, 1.0};
+                rc = adj_vjp(h, Un, L1, dt, lam, X, a, 3);
+            }
+            if (rc) return rc;
+        }
+    }
+    HIPCHK(hipMemsetAsync(grad, 0, bytes, h->stream));
+    // interior cells carry the gradient; ghosts stay 0
+    for (int q = 0; q < 4; ++q)
+        hipLaunchKernelGGL(copy_interior_kernel, dim3(elementwise_blocks(h)), dim3(256), 0, h->stream, h->grid(),
+                           (const double*)(lam + q * T), grad + q * T);
+    HIPCHK(hipGetLastError());
+    double gb = 0.0;
+    HIPCHK(hipMemcpyAsync(h->hostPinned, h->adjGbar, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    gb = h->hostPinned[0];
+    if (dgamma) *dgamma = gb;
+    return SIFV_OK;
+}
+
+// Host-array form of sifv_adjoint_backward.
+int sifv_adjoint_backward_host(sifv_handle* h, const double* seed, double* grad, double* dgamma) {
+    if (!h || !seed || !grad) return fail(SIFV_ERR_INVALID, "null argument");
+    const size_t bytes = sizeof(double) * 4 * h->total;
+    double *ds = nullptr, *dg = nullptr;
+    HIPCHK(hipMalloc(&ds, bytes));
+    if (hipMalloc(&dg, bytes) != hipSuccess) {
+        hipFree(ds);
+        return fail(SIFV_ERR_NOMEM, "adjoint staging buffers");
+    }
+    int rc = SIFV_OK;
+    if (hipMemcpy(ds, seed, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = fail(SIFV_ERR_HIP, "seed upload");
+    if (!rc) rc = sifv_adjoint_backward(h, ds, dg, dgamma);
+    if (!rc && hipMemcpy(grad, dg, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(SIFV_ERR_HIP, "grad download");
+    hipFree(ds);
+    hipFree(dg);
+    return rc;
+}
+
+// Device pointer of the resident conservative state (4 fields x totalCells
+// for 2D), for callers that keep tensors on the device.
+int sifv_state_device_ptr(sifv_handle* h, double** ptr) {
+    if (!h || !ptr) return fail(SIFV_ERR_INVALID, "null argument");
+    if (h->precision != SIFV_FP64) return fail(SIFV_ERR_UNSUPPORTED, "device state pointer needs an FP64 handle");
+    *ptr = static_cast<double*>(h->cons[h->cur]);
     return SIFV_OK;
 }
 

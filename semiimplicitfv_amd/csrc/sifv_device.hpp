@@ -89,10 +89,11 @@ template <int DIM>
 __device__ __forceinline__ void cons_to_prim(const Eos& e, double rho, const double* mom, double E,
                                              double* u, double& p) {
     double rs = dmax_ref(rho, 1e-14);
+    const double ri = 1.0 / rs;
     double ke = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        u[d] = mom[d] / rs;
+        u[d] = mom[d] * ri;
         ke += u[d] * u[d];
     }
     ke *= 0.5 * rs;
@@ -122,6 +123,44 @@ struct Recon<1, T> {
     }
 };
 
+// WENO weights a_k = d_k / (eps + beta_k)^2, normalised.  FP64: the weights
+// are cleared of their denominators, w_k = d_k prod_{m != k} Q_m / sum, with
+// Q_m = (eps + beta_m)^2, and both faces share one division
+// 1/(den_minus * den_plus) (a second form with one division per face takes
+// over should that product leave the FP64 range).  FP32: hardware reciprocals.
+__device__ __forceinline__ void weno_finish(double nP, double dP, double nM, double dM, double& plus,
+                                            double& minus) {
+    const double pr = dP * dM;
+    if (pr <= 1.0e290) {
+        const double inv = 1.0 / pr;
+        plus = (nP * dM) * inv;
+        minus = (nM * dP) * inv;
+    } else {
+        plus = nP / dP;
+        minus = nM / dM;
+    }
+}
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Division / reciprocal / square root in the working precision: IEEE FP64,
+// hardware (1-ulp) FP32 instructions.
+template <typename T>
+__device__ __forceinline__ T t_rcp(T x) {
+    if constexpr (sizeof(T) == 8) return T(1.0) / x;
+    else return frcp(x);
+}
+template <typename T>
+__device__ __forceinline__ T t_div(T a, T b) {
+    if constexpr (sizeof(T) == 8) return a / b;
+    else return a * frcp(b);
+}
+template <typename T>
+__device__ __forceinline__ T t_sqrt(T x) {
+    if constexpr (sizeof(T) == 8) return sqrt(x);
+    else return __builtin_amdgcn_sqrtf(x);
+}
+
 template <typename T>
 struct Recon<2, T> {
     __device__ __forceinline__ static void run(const T* v, T eps, bool weno, T& minus, T& plus) {
@@ -130,19 +169,28 @@ struct Recon<2, T> {
         T pl1 = h * v[1] + h * v[2];
         T pr0 = h * v[0] + h * v[1];
         T pr1 = th * v[1] - h * v[2];
+        const T c13 = T(1.0) / T(3.0), c23 = T(2.0) / T(3.0);
         if (!weno) {
-            plus = (T(1.0) / T(3.0)) * pl0 + (T(2.0) / T(3.0)) * pl1;
-            minus = (T(2.0) / T(3.0)) * pr0 + (T(1.0) / T(3.0)) * pr1;
+            plus = c13 * pl0 + c23 * pl1;
+            minus = c23 * pr0 + c13 * pr1;
             return;
         }
         T b0 = (v[1] - v[0]) * (v[1] - v[0]);
         T b1 = (v[2] - v[1]) * (v[2] - v[1]);
-        T q0 = T(1.0) / ((eps + b0) * (eps + b0));
-        T q1 = T(1.0) / ((eps + b1) * (eps + b1));
-        T aL0 = (T(1.0) / T(3.0)) * q0, aL1 = (T(2.0) / T(3.0)) * q1;
-        T aR0 = (T(2.0) / T(3.0)) * q0, aR1 = (T(1.0) / T(3.0)) * q1;
-        plus = (aL0 * pl0 + aL1 * pl1) / (aL0 + aL1);
-        minus = (aR0 * pr0 + aR1 * pr1) / (aR0 + aR1);
+        T Q0 = (eps + b0) * (eps + b0);
+        T Q1 = (eps + b1) * (eps + b1);
+        if constexpr (sizeof(T) == 8) {
+            // plus: a = (1/3)/Q0, (2/3)/Q1; minus: (2/3)/Q0, (1/3)/Q1
+            const T nP = c13 * Q1 * pl0 + c23 * Q0 * pl1, dP = c13 * Q1 + c23 * Q0;
+            const T nM = c23 * Q1 * pr0 + c13 * Q0 * pr1, dM = c23 * Q1 + c13 * Q0;
+            weno_finish(nP, dP, nM, dM, plus, minus);
+        } else {
+            T q0 = frcp(Q0), q1 = frcp(Q1);
+            T aL0 = c13 * q0, aL1 = c23 * q1;
+            T aR0 = c23 * q0, aR1 = c13 * q1;
+            plus = (aL0 * pl0 + aL1 * pl1) * frcp(aL0 + aL1);
+            minus = (aR0 * pr0 + aR1 * pr1) * frcp(aR0 + aR1);
+        }
     }
 };
 
@@ -173,13 +221,22 @@ struct Recon<3, T> {
         T b1 = c1312 * t1 * t1 + c14 * s1 * s1;
         T b2 = c1312 * t2 * t2 + c14 * s2 * s2;
         T e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
-        T q0 = T(1.0) / (e0 * e0);
-        T q1 = T(1.0) / (e1 * e1);
-        T q2 = T(1.0) / (e2 * e2);
-        T aL0 = d0 * q0, aL1 = d1 * q1, aL2 = d2 * q2;
-        T aR0 = d0 * q2, aR1 = d1 * q1, aR2 = d2 * q0;
-        plus = (aL0 * pl0 + aL1 * pl1 + aL2 * pl2) / (aL0 + aL1 + aL2);
-        minus = (aR0 * pr0 + aR1 * pr1 + aR2 * pr2) / (aR0 + aR1 + aR2);
+        T Q0 = e0 * e0, Q1 = e1 * e1, Q2 = e2 * e2;
+        if constexpr (sizeof(T) == 8) {
+            // plus: a_k = d_k / Q_k; minus: d0/Q2, d1/Q1, d2/Q0
+            const T P12 = Q1 * Q2, P02 = Q0 * Q2, P01 = Q0 * Q1;
+            const T wP0 = d0 * P12, wP1 = d1 * P02, wP2 = d2 * P01;
+            const T wM0 = d0 * P01, wM1 = d1 * P02, wM2 = d2 * P12;
+            const T nP = wP0 * pl0 + wP1 * pl1 + wP2 * pl2, dP = wP0 + wP1 + wP2;
+            const T nM = wM0 * pr0 + wM1 * pr1 + wM2 * pr2, dM = wM0 + wM1 + wM2;
+            weno_finish(nP, dP, nM, dM, plus, minus);
+        } else {
+            T q0 = frcp(Q0), q1 = frcp(Q1), q2 = frcp(Q2);
+            T aL0 = d0 * q0, aL1 = d1 * q1, aL2 = d2 * q2;
+            T aR0 = d0 * q2, aR1 = d1 * q1, aR2 = d2 * q0;
+            plus = (aL0 * pl0 + aL1 * pl1 + aL2 * pl2) * frcp(aL0 + aL1 + aL2);
+            minus = (aR0 * pr0 + aR1 * pr1 + aR2 * pr2) * frcp(aR0 + aR1 + aR2);
+        }
     }
 };
 
@@ -197,6 +254,7 @@ struct Recon<3, T> {
 template <int DIM, int NPH, typename T>
 struct FaceState {
     T rho, u[DIM], p, sig, g, pi;
+    T gm1i;  // 1 / (g - 1)
     T a[NPH > 0 ? NPH : 1];
 };
 
@@ -205,7 +263,17 @@ constexpr int flux_len(int dim) { return dim + 4 + NPH; }
 
 template <int DIM, int NPH, typename T>
 __device__ __forceinline__ T sound_direct(const FaceState<DIM, NPH, T>& w) {
-    return sqrt(w.g * tmax(w.p + w.pi, T(1e-14)) / tmax(w.rho, T(1e-14)));
+    return t_sqrt(t_div(w.g * tmax(w.p + w.pi, T(1e-14)), tmax(w.rho, T(1e-14))));
+}
+
+// Both sound speeds of a face with one division.
+template <int DIM, int NPH, typename T>
+__device__ __forceinline__ void sound_pair(const FaceState<DIM, NPH, T>& L, const FaceState<DIM, NPH, T>& R,
+                                           T& cL, T& cR) {
+    const T rL = tmax(L.rho, T(1e-14)), rR = tmax(R.rho, T(1e-14));
+    const T inv = t_rcp(rL * rR);
+    cL = t_sqrt(L.g * tmax(L.p + L.pi, T(1e-14)) * (rR * inv));
+    cR = t_sqrt(R.g * tmax(R.p + R.pi, T(1e-14)) * (rL * inv));
 }
 
 template <int DIM, int NPH, typename T>
@@ -214,7 +282,7 @@ __device__ __forceinline__ T rhoE_state(const FaceState<DIM, NPH, T>& w) {
 #pragma unroll
     for (int d = 1; d < DIM; ++d) q += w.u[d] * w.u[d];
     T ke = T(0.5) * w.rho * q;
-    return (w.p + w.g * w.pi) / (w.g - T(1.0)) + ke;
+    return (w.p + w.g * w.pi) * w.gm1i + ke;
 }
 
 enum { RS_LF = 0, RS_RUSANOV = 1, RS_HLLC = 2 };
@@ -227,11 +295,12 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
         T EL = rhoE_state(L), ER = rhoE_state(R);
         T sL, sR, sS;
         if (incP) {
-            T cL = sound_direct(L), cR = sound_direct(R);
+            T cL, cR;
+            sound_pair(L, R, cL, cR);
             sL = tmin(uL - cL, uR - cR);
             sR = tmax(uL + cL, uR + cR);
-            sS = (R.p - L.p + L.rho * uL * (sL - uL) - R.rho * uR * (sR - uR)) /
-                 (L.rho * (sL - uL) - R.rho * (sR - uR));
+            sS = t_div(R.p - L.p + L.rho * uL * (sL - uL) - R.rho * uR * (sR - uR),
+                       L.rho * (sL - uL) - R.rho * (sR - uR));
         } else {
             sL = tmin(uL, uR);
             sR = tmax(uL, uR);
@@ -260,7 +329,20 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
             T un = left ? uL : uR;
             T s = left ? sL : sR;
             T E = left ? EL : ER;
-            T rS = W.rho * (s - un) / (s - sS);
+            const T A = s - un, B = s - sS;
+            T rS, e, pOverRhoA = T(0);
+            if (incP) {
+                // s - un and s - sS are nonzero here (sound speed > 0, sS strictly
+                // between the outer waves): one division for all three quotients
+                const T rA = W.rho * A;
+                const T inv = t_rcp(rA * B);
+                rS = rA * (rA * inv);
+                e = E * (A * B * inv);
+                pOverRhoA = W.p * (B * inv);
+            } else {
+                rS = t_div(W.rho * A, B);
+                e = t_div(E, W.rho);
+            }
             F[0] = W.rho * un + s * (rS - W.rho);
 #pragma unroll
             for (int i = 0; i < DIM; ++i) {
@@ -270,9 +352,8 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
                 if (incP && i == D) base = base + W.p;
                 F[1 + i] = base + s * (ruS - W.rho * ui);
             }
-            T e = E / W.rho;
             if (incP) {
-                T ES = rS * (e + (sS - un) * (sS + W.p / (W.rho * (s - un))));
+                T ES = rS * (e + (sS - un) * (sS + pOverRhoA));
                 F[DIM + 1] = (E + W.p) * un + s * (ES - E);
             } else {
                 T ES = rS * (e + (sS - un) * sS);
@@ -290,7 +371,8 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
         if (RS == RS_RUSANOV) {
             T aL = fabs(uL), aR = fabs(uR);
             if (incP) {
-                T cL = sound_direct(L), cR = sound_direct(R);
+                T cL, cR;
+                sound_pair(L, R, cL, cR);
                 S = tmax(aL + cL, aR + cR);
             } else {
                 S = tmax(aL, aR);
@@ -319,10 +401,11 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
                 qL += L.u[d] * L.u[d];
                 qR += R.u[d] * R.u[d];
             }
-            qL = sqrt(qL);
-            qR = sqrt(qR);
+            qL = t_sqrt(qL);
+            qR = t_sqrt(qR);
             if (incP) {
-                T cL = sound_direct(L), cR = sound_direct(R);
+                T cL, cR;
+                sound_pair(L, R, cL, cR);
                 S = tmax(qL, qR) + tmax(cL, cR);
             } else {
                 S = tmax(qL, qR);

@@ -67,6 +67,8 @@ struct StageParams {
     T* part;              // 3D: x+y part of the right-hand side (NCV+2 fields)
     const double* sigma;  // entropic pressure (LF with IGR)
     double c1, c2, c3, c4;
+    double c4inv;         // 1/c4, used when c4 is a power of two (then x*c4inv == x/c4 exactly)
+    int c4pow2;
     int rk1;              // RKOrder == 1 form (U += dt R)
     const double* dt;     // device-resident time step
     const int* active;    // device flag; 0 skips the stage (time loop finished)
@@ -74,6 +76,7 @@ struct StageParams {
     double eps;
     int igr, weno;
     int sigPrim;          // sigma ghosts follow the PRIM rule (no IGR sweep ran this stage)
+    int unifY;            // every interior row (and plane) width is the same double
     int rows;             // rows (y) or planes (z) marched per wave
     int nseg;             // segments per strip
     int nstrip;           // strips per row / plane
@@ -221,6 +224,7 @@ __device__ __forceinline__ void face_state(const StageParams<T>& P, const T* v, 
     if constexpr (NPH == 0) {
         w.g = (T)P.eos.gamma;
         w.pi = (T)(P.eos.stiffened ? P.eos.pInf : 0.0);
+        w.gm1i = (T)(1.0 / (P.eos.gamma - 1.0));
     } else {
         T sInv = T(0), sPi = T(0);
 #pragma unroll
@@ -233,6 +237,7 @@ __device__ __forceinline__ void face_state(const StageParams<T>& P, const T* v, 
         }
         w.g = T(1.0) + T(1.0) / sInv;
         w.pi = (w.g - T(1.0)) / w.g * sPi;
+        w.gm1i = sInv;
     }
 }
 
@@ -306,7 +311,8 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
         } else {
             double a = P.c1 * U;
             if (use0) a = a + P.c2 * U0v[q];
-            un[q] = (a + (P.c3 * dt) * Rq) / P.c4;
+            const double b = a + (P.c3 * dt) * Rq;
+            un[q] = P.c4pow2 ? b * P.c4inv : b / P.c4;
         }
     }
     if constexpr (NPH > 0) {
@@ -437,6 +443,11 @@ __global__ void __launch_bounds__(64) stage_xy_kernel(StageParams<T> P) {
     const double dxc = g.dxw[cc + g.ngx];
     const double dzk = (DIM >= 3) ? g.dzw[kz + g.ngz] : 1.0;
     double dtMin = 1.0e300;
+    // face coefficients faceArea / cellVolume (src/ExplicitSolver.cpp:305-340);
+    // constant along the march when every row has the same width
+    const double dyU = YDIR ? g.dyw[g.ngy] : g.dyw[0];
+    const double cxU = (dyU * dzk) / ((dxc * dyU) * dzk);
+    const double cyU = (dxc * dzk) / ((dxc * dyU) * dzk);
 
     Cell<DIM, NPH> win[WS];
     T sigwin[WS];
@@ -490,7 +501,7 @@ __global__ void __launch_bounds__(64) stage_xy_kernel(StageParams<T> P) {
                 continue;  // warm-up rows: no x phase, no output
             }
             dyr = g.dyw[r + g.ngy];
-            cy = (dxc * dzk) / ((dxc * dyr) * dzk);
+            cy = P.unifY ? cyU : (dxc * dzk) / ((dxc * dyr) * dzk);
             cur = win[WROW];
             sigc = sigwin[WROW];
         } else {
@@ -544,8 +555,16 @@ __global__ void __launch_bounds__(64) stage_xy_kernel(StageParams<T> P) {
         for (int d = 0; d < DIM; ++d) rX.u[d] = __shfl_down(mX.u[d], 1, kWave);
         rX.p = __shfl_down(mX.p, 1, kWave);
         rX.sig = __shfl_down(mX.sig, 1, kWave);
-        rX.g = __shfl_down(mX.g, 1, kWave);
-        rX.pi = __shfl_down(mX.pi, 1, kWave);
+        if constexpr (NPH == 0) {
+            // single phase: the EOS constants are the same in every lane
+            rX.g = mX.g;
+            rX.pi = mX.pi;
+            rX.gm1i = mX.gm1i;
+        } else {
+            rX.g = __shfl_down(mX.g, 1, kWave);
+            rX.pi = __shfl_down(mX.pi, 1, kWave);
+            rX.gm1i = __shfl_down(mX.gm1i, 1, kWave);
+        }
 #pragma unroll
         for (int ph = 0; ph < NPH; ++ph) rX.a[ph] = __shfl_down(mX.a[ph], 1, kWave);
         T Fx[NFL];
@@ -563,7 +582,7 @@ __global__ void __launch_bounds__(64) stage_xy_kernel(StageParams<T> P) {
         for (int q = 0; q < NF; ++q) FCl[q] = __shfl_up(FCx[q], 1, kWave);
 
         if (owned) {
-            const double cx = (dyr * dzk) / ((dxc * dyr) * dzk);
+            const double cx = P.unifY ? cxU : (dyr * dzk) / ((dxc * dyr) * dzk);
             T acc[NF];
             accumulate<DIM, NPH, T>(acc, FCl, FCx, (T)cx, true);
             if (YDIR) accumulate<DIM, NPH, T>(acc, Glo, Ghi, (T)cy, false);
@@ -625,6 +644,7 @@ __global__ void __launch_bounds__(64) stage_z_kernel(StageParams<T> P) {
     const double dxc = g.dxw[cc + g.ngx];
     const double dyj = g.dyw[j + g.ngy];
     double dtMin = 1.0e300;
+    const double czU = (dxc * dyj) / ((dxc * dyj) * g.dzw[g.ngz]);
 
     Cell<DIM, NPH> win[WS];
     T sigwin[WS];
@@ -660,7 +680,7 @@ __global__ void __launch_bounds__(64) stage_z_kernel(StageParams<T> P) {
         plusPrev = pZ;
         if (r >= Z0 && owned) {
             const double dzr = g.dzw[r + g.ngz];
-            const double cz = (dxc * dyj) / ((dxc * dyj) * dzr);
+            const double cz = P.unifY ? czU : (dxc * dyj) / ((dxc * dyj) * dzr);
             const size_t id = cell_index(g, col, j, r);
             T acc[NF];
 #pragma unroll

@@ -141,6 +141,32 @@ class ExplicitSolver(_GpuSolverBase):
         return t.value, s.value
 
 
+    # ---- discrete adjoint (no reference counterpart) ----------------------
+    def adjoint_forward(self, config: SimulationConfig, nsteps: int, checkpoint_every: int = 20):
+        """Runs nsteps fixed-dt steps from the resident state, keeping
+        checkpoints for adjoint_backward."""
+        cc = config.to_c()
+        _abi.check(self._lib.sifv_adjoint_forward(self._h, C.byref(cc), int(nsteps), int(checkpoint_every)))
+
+    def adjoint_backward(self, seed):
+        """seed: 4 padded arrays (rho, rhoU, rhoV, rhoE), the adjoint of the
+        final state.  -> (4 padded gradient arrays w.r.t. the state at the
+        start of adjoint_forward, d/dgamma)."""
+        n = self.mesh.totalCells()
+        s = np.ascontiguousarray(np.concatenate([np.asarray(a, dtype=np.float64).ravel() for a in seed]))
+        if s.size != 4 * n:
+            raise ValueError("seed must hold 4 fields of totalCells values")
+        g = np.empty(4 * n)
+        dg = C.c_double()
+        _abi.check(self._lib.sifv_adjoint_backward_host(self._h, _abi.dptr(s), _abi.dptr(g), C.byref(dg)))
+        return [g[q * n:(q + 1) * n] for q in range(4)], dg.value
+
+    def state_device_ptr(self) -> int:
+        p = C.c_void_p()
+        _abi.check(self._lib.sifv_state_device_ptr(self._h, C.byref(p)))
+        return p.value
+
+
 class SemiImplicitSolver(_GpuSolverBase):
     """SemiImplicitSolver (include/SemiImplicitSolver.hpp:22-96) with the
     Jacobi pressure solver, on the GPU."""

@@ -23,7 +23,7 @@ from semiimplicitfv_amd import (BoundaryCondition, IdealGasEOS, ReconstructionOr
 from semiimplicitfv_amd.cases import _fill  # noqa: E402
 from semiimplicitfv_amd.timeloop import run_time_loop  # noqa: E402
 from tests.golden.problems import (bubble_3d, jacobi_problem, liquid_gas_tube, random_ghost_case,  # noqa: E402
-                                   igr_cases, recon_lines, riemann_pairs, two_phase_cases)
+                                   ensemble_sample, igr_cases, recon_lines, riemann_pairs, two_phase_cases)
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 
@@ -172,9 +172,31 @@ def g_igr():
     _run_cases(igr_cases(), "igr_ref.npz")
 
 
+def g_ensemble():
+    """Shock tubes of the batched ensemble sample run to t = 0.1 one by one
+    (runTimeLoop dt rule): step counts, final times and interior fields."""
+    from semiimplicitfv_amd import ensemble
+
+    p, idx = ensemble_sample()
+    cfg, eos, rs, tubes = ensemble.build_tubes(p, idx)
+    out = {"rows": np.array(idx)}
+    for k, (mesh, st) in enumerate(tubes):
+        s = po.CpuSolver(mesh, cfg, rs.kind(), eos, "ref")
+        t, n = 0.0, 0
+        while t < 0.1:
+            cfg.step = n
+            t += s.explicit_step(st, cfg, 0.1 - t)
+            n += 1
+        out[f"t{k}_steps"] = np.array([n])
+        out[f"t{k}_time"] = np.array([t])
+        for fid in (0, 1, 4, 5, 8):
+            out[f"t{k}_f{fid}"] = np.ascontiguousarray(mesh.interior(st.field(fid)).ravel())
+    save("ensemble_ref.npz", **out)
+
+
 GENERATORS = {"recon": g_recon, "riemann": g_riemann, "sod": g_sod, "vortex": g_vortex, "ghost": g_ghost,
               "jacobi": g_jacobi, "semi": g_semi, "liquid_gas": g_liquid_gas, "two_phase": g_two_phase,
-              "igr": g_igr}
+              "igr": g_igr, "ensemble": g_ensemble}
 
 if __name__ == "__main__":
     assert po.available("ref"), "build oracle/_ref first (make -C oracle)"

@@ -399,3 +399,14 @@ def igr_cases():
         "e3_lf": (lambda: build(3, (12, 10, 9), 0, seed=6, periods=(0, 0, 1), bcs=walls), 0, 3, False),
         "s2_lf": (lambda: build(2, (30, 22), 0, seed=7, bcs=walls, semi=True), 0, 3, True),
     }
+
+
+def ensemble_sample(k: int = 16, nmax: int = 1024, seed: int = 3):
+    """Rows of the batched Riemann ensemble table (cases.riemann_ensemble_params)
+    with at most nmax cells, drawn without replacement; returns (table, rows)."""
+    from semiimplicitfv_amd import cases
+
+    p = cases.riemann_ensemble_params()
+    small = np.flatnonzero(p["ncells"] <= nmax)
+    rng = np.random.default_rng(seed)
+    return p, sorted(rng.choice(small, size=k, replace=False).tolist())

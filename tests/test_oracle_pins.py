@@ -365,3 +365,24 @@ def test_igr_golden(oracle_c, key):
     for fid in st.field_ids():
         l1, li = rel_errs(mesh.interior(st.field(fid)), d[f"{key}_f{fid}"])
         assert l1 <= 1e-13 and li <= 1e-12, (fid, l1, li)
+
+
+def test_ensemble_tubes_golden(oracle_c):
+    """Shock tubes of the batched ensemble sample, each run to t = 0.1 with
+    the runTimeLoop dt rule: oracle against the reference's own output."""
+    from semiimplicitfv_amd import ensemble
+
+    d = load("ensemble_ref.npz")
+    p, idx = problems.ensemble_sample()
+    cfg, eos, rs, tubes = ensemble.build_tubes(p, idx)
+    for k, (mesh, st) in enumerate(tubes[:6]):
+        s = oracle_c.CpuSolver(mesh, cfg, rs.kind(), eos, "c")
+        t, n = 0.0, 0
+        while t < 0.1:
+            cfg.step = n
+            t += s.explicit_step(st, cfg, 0.1 - t)
+            n += 1
+        assert n == d[f"t{k}_steps"][0] and t == d[f"t{k}_time"][0]
+        for fid in (0, 1, 4, 5, 8):
+            l1, li = rel_errs(mesh.interior(st.field(fid)).ravel(), d[f"t{k}_f{fid}"])
+            assert l1 <= 1e-12 and li <= 1e-11, (k, fid, l1, li)
