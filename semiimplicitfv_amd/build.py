@@ -27,6 +27,36 @@ FLAGS = [
 ]
 
 
+def build_torch_extension(verbose: bool = True) -> str:
+    """Builds the thin PyTorch extension _sifv_torch (C++, torch API only) in
+    place, linked against libsifv_hip.so."""
+    import contextlib
+    import io
+
+    from setuptools import setup
+    from torch.utils.cpp_extension import BuildExtension, CppExtension
+
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    ext = CppExtension("semiimplicitfv_amd._sifv_torch", [os.path.join(CSRC, "torch_adjoint.cpp")],
+                       include_dirs=[inc], library_dirs=[HERE], libraries=["sifv_hip"],
+                       extra_compile_args=["-O2"], extra_link_args=["-Wl,-rpath,$ORIGIN"])
+    t0 = time.time()
+    cwd = os.getcwd()
+    os.chdir(os.path.dirname(HERE))
+    try:
+        out = io.StringIO()
+        with contextlib.redirect_stdout(out), contextlib.redirect_stderr(out):
+            setup(name="sifv_torch", ext_modules=[ext], cmdclass={"build_ext": BuildExtension},
+                  script_args=["-q", "build_ext", "--inplace", "--build-temp", os.path.join(HERE, "_build"),
+                               "--build-lib", os.path.join(HERE, "_build", "lib")])
+    finally:
+        os.chdir(cwd)
+    if verbose:
+        print(f"built the torch extension in {time.time() - t0:.1f} s")
+    import glob as _g
+    return (_g.glob(os.path.join(HERE, "_sifv_torch*.so")) or [""])[0]
+
+
 def build(verbose: bool = True) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
@@ -59,3 +89,4 @@ def build(verbose: bool = True) -> str:
 
 if __name__ == "__main__":
     build()
+    build_torch_extension()

@@ -29,6 +29,19 @@
 
 #include "sifv_device.hpp"
 
+#ifndef SIFV_PREFETCH_ROW_D
+#define SIFV_PREFETCH_ROW_D true
+#endif
+#ifndef SIFV_PRELOAD_UPDATE_D
+#define SIFV_PRELOAD_UPDATE_D false
+#endif
+#ifndef SIFV_PREFETCH_ROW_F
+#define SIFV_PREFETCH_ROW_F true
+#endif
+#ifndef SIFV_PRELOAD_UPDATE_F
+#define SIFV_PRELOAD_UPDATE_F true
+#endif
+
 namespace sifv {
 
 constexpr int kOwn = 62;  // owned columns per wave in the x-marching kernels
@@ -143,6 +156,53 @@ __device__ __forceinline__ void prim_of(const Eos& e, const Mix& m, const double
         }
         c.p = ((U[DIM + 1] - ke) - sPi) / sInv;
     }
+}
+
+// Raw conservative values of one mapped cell plus its velocity sign flips
+// (bit d: negate u[d]); split from the conversion so that the load of the
+// next row can be issued one march iteration ahead of its use.
+template <int DIM, int NPH, typename T>
+struct RawCell {
+    T U[DIM + 2 + 2 * NPH];
+    unsigned flip;
+};
+
+template <int DIM, int NPH, typename T>
+__device__ __forceinline__ void load_raw(const StageParams<T>& P, const T* buf, int i, int j, int k,
+                                         RawCell<DIM, NPH, T>& rc) {
+    constexpr int NCV = DIM + 2 + 2 * NPH;
+    const Grid& g = P.g;
+    bool fn[3] = {false, false, false}, ft[3] = {false, false, false};
+    int si = map_axis(clampi(i, -g.ngx, g.nx - 1 + g.ngx), g.nx, g.bc[0], g.bc[1], fn[0], ft[0]);
+    int sj = 0, sk = 0;
+    if (DIM >= 2) sj = map_axis(clampi(j, -g.ngy, g.ny - 1 + g.ngy), g.ny, g.bc[2], g.bc[3], fn[1], ft[1]);
+    if (DIM >= 3) sk = map_axis(clampi(k, -g.ngz, g.nz - 1 + g.ngz), g.nz, g.bc[4], g.bc[5], fn[2], ft[2]);
+    size_t id = cell_index(g, si, sj, sk);
+#pragma unroll
+    for (int q = 0; q < NCV; ++q) rc.U[q] = buf[(size_t)q * P.total + id];
+    unsigned fl = 0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        bool f = fn[d];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e)
+            if (e != d) f = f != ft[e];
+        fl |= f ? (1u << d) : 0u;
+    }
+    rc.flip = fl;
+}
+
+template <int DIM, int NPH, typename T>
+__device__ __forceinline__ void raw_to_cell(const StageParams<T>& P, const RawCell<DIM, NPH, T>& rc,
+                                            Cell<DIM, NPH>& c) {
+    constexpr int NCV = DIM + 2 + 2 * NPH;
+    double U[NCV];
+#pragma unroll
+    for (int q = 0; q < NCV; ++q) U[q] = (double)rc.U[q];
+    prim_of<DIM, NPH>(P.eos, P.mix, U, c);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+        if (rc.flip & (1u << d)) c.u[d] = -c.u[d];
 }
 
 // Load one cell (i, j, k) through the ghost mapping (with velocity flips).
@@ -285,7 +345,8 @@ __device__ __forceinline__ void accumulate(T* acc, const T* lo, const T* hi, T c
 // of the new state when requested.  acc holds the complete flux part.
 template <int DIM, int NPH, bool SEMI, typename T>
 __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id, T* acc, const Cell<DIM, NPH>& cell,
-                                              const double* h, double dt, bool wantDt) {
+                                              const double* h, double dt, bool wantDt, const T* Upre = nullptr,
+                                              const T* U0pre = nullptr) {
     constexpr int NCV = DIM + 2 + 2 * NPH;
     constexpr int QP = NCV, QD = NCV + 1;
     // sources: alpha div(u) (src/ExplicitSolver.cpp:486-496), p div(u)
@@ -297,10 +358,11 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
     double U0v[NCV];
     const bool use0 = P.c2 != 0.0;
 #pragma unroll
-    for (int q = 0; q < NCV; ++q) U0v[q] = use0 ? (double)P.u0[(size_t)q * P.total + id] : 0.0;
+    for (int q = 0; q < NCV; ++q)
+        U0v[q] = use0 ? (double)(U0pre ? U0pre[q] : P.u0[(size_t)q * P.total + id]) : 0.0;
 #pragma unroll
     for (int q = 0; q < NCV; ++q) {
-        double U = (double)P.src[(size_t)q * P.total + id];
+        double U = (double)(Upre ? Upre[q] : P.src[(size_t)q * P.total + id]);
         double Rq = (double)acc[q];
         if (SEMI && q == DIM + 1) {
             un[q] = U;  // total energy is rebuilt by the correction step
@@ -396,6 +458,22 @@ __device__ __forceinline__ void recon_window(const StageParams<T>& P, const Cell
     face_state<DIM, NPH, T>(P, pl, sl, plusS);
 }
 
+// Latency hiding in the y-march: the next window row can be loaded one
+// iteration ahead (kPrefetchRow) and the update operands of the current row
+// (U and U0 at the cell) at the top of the iteration (kPreloadUpdate).  Each
+// costs registers; they are enabled where the kernel stays within its
+// occupancy step (measured on the 4096^2 vortex).
+template <typename T>
+struct MarchTuning {
+    static constexpr bool kPrefetchRow = SIFV_PREFETCH_ROW_D;
+    static constexpr bool kPreloadUpdate = SIFV_PRELOAD_UPDATE_D;
+};
+template <>
+struct MarchTuning<float> {
+    static constexpr bool kPrefetchRow = SIFV_PREFETCH_ROW_F;
+    static constexpr bool kPreloadUpdate = SIFV_PRELOAD_UPDATE_F;
+};
+
 // ===========================================================================
 // x + y kernel (1D/2D full stage; 3D x-y part on one plane)
 // ===========================================================================
@@ -468,6 +546,12 @@ __global__ void __launch_bounds__(64) stage_xy_kernel(StageParams<T> P) {
             sigwin[w] = useSig ? (T)load_sigma<DIM>(P, col, row, kz) : T(0);
         }
     }
+    // raw values of the row entering the window next iteration
+    constexpr bool PF = MarchTuning<T>::kPrefetchRow;
+    constexpr bool PU = MarchTuning<T>::kPreloadUpdate && MODE == MODE_FULL;
+    RawCell<DIM, NPH, T> nextRow;
+    if (YDIR && PF) load_raw<DIM, NPH, T>(P, P.src, col, rStart + R, kz, nextRow);
+    T preU[DIM + 2 + 2 * NPH], preU0[DIM + 2 + 2 * NPH];
 
     for (int r = rStart; r < Y1; ++r) {
         T Ghi[NF];
@@ -482,7 +566,12 @@ __global__ void __launch_bounds__(64) stage_xy_kernel(StageParams<T> P) {
                 win[w] = win[w + 1];
                 sigwin[w] = sigwin[w + 1];
             }
-            load_cell<DIM, NPH>(P, P.src, col, r + R, kz, win[WS - 1]);
+            if constexpr (PF) {
+                raw_to_cell<DIM, NPH, T>(P, nextRow, win[WS - 1]);
+                if (r + 1 < Y1) load_raw<DIM, NPH, T>(P, P.src, col, r + 1 + R, kz, nextRow);
+            } else {
+                load_cell<DIM, NPH>(P, P.src, col, r + R, kz, win[WS - 1]);
+            }
             sigwin[WS - 1] = useSig ? (T)load_sigma<DIM>(P, col, r + R, kz) : T(0);
             // y reconstruction of row r+1
             FaceState<DIM, NPH, T> mY, pY;
@@ -501,6 +590,14 @@ __global__ void __launch_bounds__(64) stage_xy_kernel(StageParams<T> P) {
                 continue;  // warm-up rows: no x phase, no output
             }
             dyr = g.dyw[r + g.ngy];
+            if (PU && owned) {
+                const size_t pid = cell_index(g, col, r, kz);
+#pragma unroll
+                for (int q = 0; q < DIM + 2 + 2 * NPH; ++q) {
+                    preU[q] = P.src[(size_t)q * P.total + pid];
+                    preU0[q] = P.c2 != 0.0 ? P.u0[(size_t)q * P.total + pid] : T(0);
+                }
+            }
             cy = P.unifY ? cyU : (dxc * dzk) / ((dxc * dyr) * dzk);
             cur = win[WROW];
             sigc = sigwin[WROW];
@@ -592,7 +689,9 @@ __global__ void __launch_bounds__(64) stage_xy_kernel(StageParams<T> P) {
                 for (int q = 0; q < NF; ++q) P.part[(size_t)q * P.total + id] = acc[q];
             } else {
                 const double h[3] = {dxc, dyr, dzk};
-                double dc = finish_cell<DIM, NPH, SEMI, T>(P, id, acc, cur, h, dt, P.dtPartial != nullptr);
+                double dc = (YDIR && PU)
+                                ? finish_cell<DIM, NPH, SEMI, T>(P, id, acc, cur, h, dt, P.dtPartial != nullptr, preU, preU0)
+                                : finish_cell<DIM, NPH, SEMI, T>(P, id, acc, cur, h, dt, P.dtPartial != nullptr);
                 dtMin = dmin_ref(dtMin, dc);
             }
         }
