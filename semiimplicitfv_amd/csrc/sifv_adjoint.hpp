@@ -384,13 +384,14 @@ __global__ void __launch_bounds__(256) adj_gather_kernel(AdjArgs A) {
     if (threadIdx.x == 0) A.gpart[A.gpartOff + blockIdx.x] = A.s * bs;
 }
 
-// gbar[0] += sum of partials[0..n) in index order.
-__global__ void adj_gsum_kernel(const double* part, int n, double* gbar) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double s = 0.0;
-        for (int q = 0; q < n; ++q) s += part[q];
-        gbar[0] += s;
-    }
+// gbar[0] += sum of partials[0..n): one block, fixed strided order per
+// thread and a fixed tree, so the result does not depend on timing.
+__global__ void __launch_bounds__(256) adj_gsum_kernel(const double* part, int n, double* gbar) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) s += part[q];
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) gbar[0] += s;
 }
 
 }  // namespace sifv

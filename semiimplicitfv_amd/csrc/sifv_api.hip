@@ -900,7 +900,7 @@ int adj_vjp(sifv_handle* h, const double* U, const double* mu, double s, double*
     hipLaunchKernelGGL(adj_recon_kernel, dim3(nb), dim3(256), 0, h->stream, A);
     A.gpartOff = 2 * nb;
     hipLaunchKernelGGL(adj_gather_kernel, dim3(nb), dim3(256), 0, h->stream, A);
-    hipLaunchKernelGGL(adj_gsum_kernel, dim3(1), dim3(64), 0, h->stream, (const double*)h->adjGpart, 3 * nb,
+    hipLaunchKernelGGL(adj_gsum_kernel, dim3(1), dim3(256), 0, h->stream, (const double*)h->adjGpart, 3 * nb,
                        h->adjGbar);
     HIPCHK(hipGetLastError());
     return SIFV_OK;
