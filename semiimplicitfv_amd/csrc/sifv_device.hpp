@@ -287,6 +287,25 @@ __device__ __forceinline__ T rhoE_state(const FaceState<DIM, NPH, T>& w) {
 
 enum { RS_LF = 0, RS_RUSANOV = 1, RS_HLLC = 2 };
 
+// Element-wise choice between two face states.  Selecting values (not a
+// reference to one of two stack objects) keeps both states in registers.
+template <int DIM, int NPH, typename T>
+__device__ __forceinline__ FaceState<DIM, NPH, T> pick(bool first, const FaceState<DIM, NPH, T>& A,
+                                                       const FaceState<DIM, NPH, T>& B) {
+    FaceState<DIM, NPH, T> W;
+    W.rho = first ? A.rho : B.rho;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) W.u[d] = first ? A.u[d] : B.u[d];
+    W.p = first ? A.p : B.p;
+    W.sig = first ? A.sig : B.sig;
+    W.g = first ? A.g : B.g;
+    W.pi = first ? A.pi : B.pi;
+    W.gm1i = first ? A.gm1i : B.gm1i;
+#pragma unroll
+    for (int ph = 0; ph < (NPH > 0 ? NPH : 1); ++ph) W.a[ph] = first ? A.a[ph] : B.a[ph];
+    return W;
+}
+
 template <int RS, int D, int DIM, int NPH, typename T>
 __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const FaceState<DIM, NPH, T>& R,
                                         bool incP, bool igr, T* F) {
@@ -308,7 +327,7 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
         }
         if (sL >= T(0) || sR <= T(0)) {
             const bool lft = sL >= T(0);
-            const FaceState<DIM, NPH, T>& W = lft ? L : R;
+            const FaceState<DIM, NPH, T> W = pick(lft, L, R);
             T un = lft ? uL : uR;
             T E = lft ? EL : ER;
             F[0] = W.rho * un;
@@ -325,7 +344,7 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
             for (int ph = 0; ph < NPH; ++ph) F[DIM + 4 + ph] = W.a[ph] * un;
         } else {
             const bool left = sS >= T(0);
-            const FaceState<DIM, NPH, T>& W = left ? L : R;
+            const FaceState<DIM, NPH, T> W = pick(left, L, R);
             T un = left ? uL : uR;
             T s = left ? sL : sR;
             T E = left ? EL : ER;
