@@ -114,7 +114,10 @@ def bench_ensemble(ntubes: int = 4096):
     cfg, eos, rs, tubes = ensemble.build_tubes(p)
     build_s = time.perf_counter() - t0
     out = {"workload": f"{ntubes} 1D shock tubes, {int(p['ncells'].sum())} cells, seed {cases.ENSEMBLE_SEED}, "
-                       "WENO5+HLLC+SSP-RK3, CFL 0.6, t_end 0.1", "host_setup_s": build_s}
+                       "WENO5+HLLC+SSP-RK3, CFL 0.6, t_end 0.1", "host_setup_s": build_s,
+           "note": "tubes_non_finite: near-vacuum Riemann problems of the table on which the scheme itself fails "
+                   "(the compiled reference reaches negative density on the same tubes); FP32 errors are over "
+                   "tubes finite in both runs"}
     rho64 = None
     for prec in ("fp64", "mixed"):
         ens = RiemannEnsemble(tubes, cfg, rs, eos, precision=prec)
@@ -126,14 +129,15 @@ def bench_ensemble(ntubes: int = 4096):
         r = {"value": cu / (ms / 1e3), "unit": "cell-updates/s", "kernel_ms": ms, "cell_updates": cu,
              "steps_min": int(steps.min()), "steps_max": int(steps.max()), "all_reached_t_end": bool(
                  np.all(np.abs(times - 0.1) <= 1e-12)), "tile": {"block": block, "wg_per_cu": wg}}
+        finite = np.array([bool(np.isfinite(a).all()) for a in rho])
+        r["tubes_non_finite"] = int((~finite).sum())
         if prec == "fp64":
-            rho64 = rho
+            rho64, fin64 = rho, finite
         else:
-            errs = np.array([float(np.abs(a - b).sum() / np.abs(b).sum()) for a, b in zip(rho, rho64)])
-            fin = np.isfinite(errs)
-            r["tubes_non_finite"] = int((~fin).sum())
-            r["rho_rel_L1_vs_fp64_max_tube"] = float(errs[fin].max()) if fin.any() else None
-            r["rho_rel_L1_vs_fp64_median"] = float(np.median(errs[fin])) if fin.any() else None
+            both = finite & fin64
+            errs = np.array([float(np.abs(a - b).sum() / np.abs(b).sum()) for a, b, ok in zip(rho, rho64, both) if ok])
+            r["rho_rel_L1_vs_fp64_max_tube"] = float(errs.max()) if errs.size else None
+            r["rho_rel_L1_vs_fp64_median"] = float(np.median(errs)) if errs.size else None
         out[prec] = r
     out["ratio_fp32_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
     return out

@@ -271,9 +271,15 @@ template <int DIM, int NPH, typename T>
 __device__ __forceinline__ void sound_pair(const FaceState<DIM, NPH, T>& L, const FaceState<DIM, NPH, T>& R,
                                            T& cL, T& cR) {
     const T rL = tmax(L.rho, T(1e-14)), rR = tmax(R.rho, T(1e-14));
-    const T inv = t_rcp(rL * rR);
-    cL = t_sqrt(L.g * tmax(L.p + L.pi, T(1e-14)) * (rR * inv));
-    cR = t_sqrt(R.g * tmax(R.p + R.pi, T(1e-14)) * (rL * inv));
+    if constexpr (sizeof(T) == 8) {
+        const T inv = t_rcp(rL * rR);
+        cL = t_sqrt(L.g * tmax(L.p + L.pi, T(1e-14)) * (rR * inv));
+        cR = t_sqrt(R.g * tmax(R.p + R.pi, T(1e-14)) * (rL * inv));
+    } else {
+        // the product of two FP32 densities can leave the normal range
+        cL = t_sqrt(t_div(L.g * tmax(L.p + L.pi, T(1e-14)), rL));
+        cR = t_sqrt(t_div(R.g * tmax(R.p + R.pi, T(1e-14)), rR));
+    }
 }
 
 template <int DIM, int NPH, typename T>
@@ -350,7 +356,7 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
             T E = left ? EL : ER;
             const T A = s - un, B = s - sS;
             T rS, e, pOverRhoA = T(0);
-            if (incP) {
+            if (incP && sizeof(T) == 8) {
                 // s - un and s - sS are nonzero here (sound speed > 0, sS strictly
                 // between the outer waves): one division for all three quotients
                 const T rA = W.rho * A;
@@ -361,6 +367,7 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
             } else {
                 rS = t_div(W.rho * A, B);
                 e = t_div(E, W.rho);
+                if (incP) pOverRhoA = t_div(W.p, W.rho * A);
             }
             F[0] = W.rho * un + s * (rS - W.rho);
 #pragma unroll

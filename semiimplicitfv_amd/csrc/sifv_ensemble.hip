@@ -72,6 +72,10 @@ struct sifv_ensemble {
     std::vector<double> cost;   // relative cost estimate (cells x wave speed / dx)
     hipStream_t stream = nullptr;
     int *dNcell = nullptr, *dOrder = nullptr, *dQueue = nullptr, *dSteps = nullptr, *dStatus = nullptr;
+    int *dNparts = nullptr, *dBar = nullptr;
+    double* dDtSlot = nullptr;
+    int nitems = 0;
+    int lastGrid = 0;
     long long* dOff = nullptr;
     double *dDx = nullptr, *dTime = nullptr, *dVel = nullptr, *dPres = nullptr;
     void* U[3] = {nullptr, nullptr, nullptr};
@@ -92,6 +96,11 @@ int ens_launch(sifv_ensemble* e, double endTime, int maxSteps, float* ms) {
     A.eos = e->eos;
     A.ntubes = e->ntubes;
     A.order = e->dOrder;
+    A.nitems = e->nitems;
+    A.nparts = e->dNparts;
+    A.bar = e->dBar;
+    A.dtSlot = e->dDtSlot;
+    A.err = e->dStatus + 1;
     A.ncell = e->dNcell;
     A.off = e->dOff;
     A.dxw = e->dDx;
@@ -112,8 +121,9 @@ int ens_launch(sifv_ensemble* e, double endTime, int maxSteps, float* ms) {
     A.status = e->dStatus;
     A.cur = 0;
     EHIP(hipMemsetAsync(e->dQueue, 0, sizeof(int), e->stream));
-    EHIP(hipMemsetAsync(e->dStatus, 0, sizeof(int), e->stream));
-    const int grid = std::max(1, std::min(e->ntubes, e->cus * e->wgPerCu));
+    EHIP(hipMemsetAsync(e->dStatus, 0, 2 * sizeof(int), e->stream));
+    EHIP(hipMemsetAsync(e->dBar, 0, sizeof(int) * e->ntubes, e->stream));
+    const int grid = e->lastGrid;
     hipEvent_t e0, e1;
     EHIP(hipEventCreate(&e0));
     EHIP(hipEventCreate(&e1));
@@ -127,13 +137,62 @@ int ens_launch(sifv_ensemble* e, double endTime, int maxSteps, float* ms) {
     EHIP(hipEventDestroy(e0));
     EHIP(hipEventDestroy(e1));
     if (ms) *ms = t;
-    int st = 0;
-    EHIP(hipMemcpy(&st, e->dStatus, sizeof(int), hipMemcpyDeviceToHost));
-    if (st) return set_error(SIFV_ERR_NONFINITE, "non-finite time step in an ensemble member");
+    int st[2] = {0, 0};
+    EHIP(hipMemcpy(st, e->dStatus, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    if (st[1]) return set_error(SIFV_ERR_HIP, "ensemble part barrier exceeded its wait budget");
+    if (st[0]) return set_error(SIFV_ERR_NONFINITE, "non-finite time step in an ensemble member");
+    return SIFV_OK;
+}
+
+// Work items for the current shape: the persistent grid is capped at the
+// number of co-resident workgroups; a tube whose cost exceeds the fair share
+// of one workgroup is cut into up to 32 parts (each at least one tile, never
+// more parts than workgroups); items are queued costliest part first with
+// the parts of a tube adjacent (longest-processing-time order).
+template <typename T>
+int ens_plan(sifv_ensemble* e) {
+    EnsFn<T> fn = ens_kernel<T>(e->block, e->riemann == SIFV_HLLC ? RS_HLLC : e->riemann == SIFV_RUSANOV ? RS_RUSANOV : RS_LF);
+    if (!fn) return set_error(SIFV_ERR_INVALID, "unsupported ensemble block size");
+    int perCu = 0;
+    EHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void*>(fn), e->block, 0));
+    perCu = std::max(1, std::min(perCu, e->wgPerCu));
+    const int G = std::max(1, e->cus * perCu);
+    double total = 0.0;
+    for (int t = 0; t < e->ntubes; ++t) total += e->cost[t];
+    const double share = total / G;
+    std::vector<int> parts(e->ntubes, 1);
+    std::vector<std::pair<double, int>> items;
+    for (int t = 0; t < e->ntubes; ++t) {
+        const int tiles = (e->ncell[t] + e->block - 1) / e->block;
+        int p = share > 0.0 ? (int)std::ceil(e->cost[t] / share) : 1;
+        p = std::max(1, std::min({p, tiles, 32, G}));
+        parts[t] = p;
+    }
+    std::vector<int> tubes(e->ntubes);
+    std::iota(tubes.begin(), tubes.end(), 0);
+    std::stable_sort(tubes.begin(), tubes.end(),
+                     [&](int a, int b) { return e->cost[a] / parts[a] > e->cost[b] / parts[b]; });
+    std::vector<int> order;
+    for (int t : tubes)
+        for (int p = 0; p < parts[t]; ++p) order.push_back(t * 32 + p);
+    e->nitems = (int)order.size();
+    if (!e->dNparts) {
+        EHIP(hipMalloc(&e->dNparts, sizeof(int) * e->ntubes));
+        EHIP(hipMalloc(&e->dBar, sizeof(int) * e->ntubes));
+        EHIP(hipMalloc(&e->dDtSlot, sizeof(double) * 32 * e->ntubes));
+    }
+    hipFree(e->dOrder);
+    e->dOrder = nullptr;
+    EHIP(hipMalloc(&e->dOrder, sizeof(int) * order.size()));
+    EHIP(hipMemcpy(e->dOrder, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+    EHIP(hipMemcpy(e->dNparts, parts.data(), sizeof(int) * e->ntubes, hipMemcpyHostToDevice));
+    e->lastGrid = std::min(G, e->nitems);
     return SIFV_OK;
 }
 
 int ens_run(sifv_ensemble* e, double endTime, int maxSteps, float* ms) {
+    int rc = e->precision == SIFV_MIXED ? ens_plan<float>(e) : ens_plan<double>(e);
+    if (rc) return rc;
     return e->precision == SIFV_MIXED ? ens_launch<float>(e, endTime, maxSteps, ms)
                                       : ens_launch<double>(e, endTime, maxSteps, ms);
 }
@@ -205,7 +264,7 @@ int sifv_ensemble_create(int ntubes, const int* ncells, const double* nodes, con
     ECK(hipMalloc(&e->dSteps, sizeof(int) * ntubes));
     ECK(hipMalloc(&e->dTime, sizeof(double) * ntubes));
     ECK(hipMalloc(&e->dQueue, sizeof(int)));
-    ECK(hipMalloc(&e->dStatus, sizeof(int)));
+    ECK(hipMalloc(&e->dStatus, 2 * sizeof(int)));
     ECK(hipMalloc(&e->dDx, sizeof(double) * e->ntot));
     ECK(hipMalloc(&e->dVel, sizeof(double) * e->ntot));
     ECK(hipMalloc(&e->dPres, sizeof(double) * e->ntot));
@@ -233,6 +292,9 @@ int sifv_ensemble_destroy(sifv_ensemble* e) {
     hipFree(e->dTime);
     hipFree(e->dQueue);
     hipFree(e->dStatus);
+    hipFree(e->dNparts);
+    hipFree(e->dBar);
+    hipFree(e->dDtSlot);
     hipFree(e->dDx);
     hipFree(e->dVel);
     hipFree(e->dPres);
@@ -336,11 +398,6 @@ int sifv_ensemble_set_shape(sifv_ensemble* e, int block, int wg_per_cu) {
 int sifv_ensemble_run(sifv_ensemble* e, double endTime, int maxSteps, int* steps, double* times, double* kernel_ms) {
     if (!e) return set_error(SIFV_ERR_INVALID, "null argument");
     if (maxSteps < 1) return set_error(SIFV_ERR_INVALID, "maxSteps must be >= 1");
-    // queue order: costliest tubes first (longest-processing-time rule)
-    std::vector<int> order(e->ntubes);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return e->cost[a] > e->cost[b]; });
-    EHIP(hipMemcpy(e->dOrder, order.data(), sizeof(int) * e->ntubes, hipMemcpyHostToDevice));
     float ms = 0.0f;
     int rc = ens_run(e, endTime, maxSteps, &ms);
     if (kernel_ms) *kernel_ms = ms;
@@ -357,10 +414,6 @@ int sifv_ensemble_autotune(sifv_ensemble* e, double endTime, int maxSteps, int* 
     const size_t bytes = e->tsize() * 3 * e->ntot;
     if (!e->saved) EHIP(hipMalloc(&e->saved, bytes));
     EHIP(hipMemcpy(e->saved, e->U[0], bytes, hipMemcpyDeviceToDevice));
-    std::vector<int> order(e->ntubes);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return e->cost[a] > e->cost[b]; });
-    EHIP(hipMemcpy(e->dOrder, order.data(), sizeof(int) * e->ntubes, hipMemcpyHostToDevice));
     const int blocks[] = {64, 128, 256, 512, 1024};
     const int wgs[] = {1, 2, 4, 8};
     int bestB = e->block, bestW = e->wgPerCu;

@@ -15,8 +15,13 @@
 //   3. Riemann flux of faces -1/2..BLOCK-1/2;
 //   4. flux divergence + SSP-RK update of the owned cell (+ the acoustic dt
 //      of the new state in the last stage).
-// BLOCK is the tile shape the autotuner chooses.  No wave waits on another
-// workgroup, so every wave ends when the queue is empty.
+// BLOCK is the tile shape the autotuner chooses.  A tube whose estimated cost
+// exceeds a workgroup's fair share is cut into P contiguous parts (work items)
+// taken by P workgroups that meet at a per-tube barrier after every RK stage
+// and share the dt reduction through per-part slots.  Items are queued with
+// the parts of a tube adjacent and the launch never exceeds the number of
+// co-resident workgroups, so every started group completes; barrier waits are
+// bounded (an error flag ends all waits), so every wave ends.
 #pragma once
 
 #include "sifv_stage.hpp"
@@ -27,7 +32,12 @@ template <typename T>
 struct EnsArgs {
     Eos eos;
     int ntubes;
-    const int* order;        // tube ids, costliest first
+    const int* order;        // work items (tube * 32 + part), costliest first
+    int nitems;
+    const int* nparts;       // parts per tube
+    int* bar;                // per tube: barrier arrival counter
+    double* dtSlot;          // per tube x 32: per-part dt partials
+    int* err;                // set when a barrier wait exceeds its budget
     const int* ncell;        // cells per tube
     const long long* off;    // first cell of each tube in the packed arrays
     const double* dxw;       // packed cell widths
@@ -43,6 +53,27 @@ struct EnsArgs {
     int* status;             // set on a non-finite dt
     int cur;                 // buffer holding U^n on entry
 };
+
+// Barrier of the P workgroups of one tube (monotone counter; target = P x
+// number of barriers passed so far).  Data written before it by any member
+// is visible after it to every member.
+__device__ __forceinline__ void tube_barrier(int* cnt, int target, int* err) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(cnt, 1);
+        int spins = 0;
+        while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+            if (++spins > (1 << 25)) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
 
 __device__ __forceinline__ double block_min_d(double v, double* sh) {
     v = wave_min(v);
@@ -75,23 +106,43 @@ __global__ void __launch_bounds__(BLOCK) ensemble_kernel(EnsArgs<T> A) {
         if (tid == 0) tubeSlot = atomicAdd(A.queue, 1);
         __syncthreads();
         const int slot = tubeSlot;
-        if (slot >= A.ntubes) return;
-        const int tube = A.order[slot];
+        if (slot >= A.nitems) return;
+        const int item = A.order[slot];
+        const int tube = item >> 5, part = item & 31;
         const int N = A.ncell[tube];
+        const int P = A.nparts[tube];
         const long long o = A.off[tube];
         const double* dx = A.dxw + o;
-        const int ntile = (N + BLOCK - 1) / BLOCK;
+        // cells [c0, c1) of this part, in whole tiles
+        const int tilesAll = (N + BLOCK - 1) / BLOCK;
+        const int tpp = (tilesAll + P - 1) / P;
+        const int c0 = part * tpp * BLOCK;
+        const int c1 = (c0 + tpp * BLOCK < N) ? c0 + tpp * BLOCK : N;
+        const int ntile = (c1 - c0 + BLOCK - 1) / BLOCK;
+        int phase = 0;
+        int* bar = A.bar + tube;
+        double* dts = A.dtSlot + (size_t)tube * 32;
+        // dt shared by the parts: own partial -> slot, barrier, min over slots
+        auto group_min = [&](double v) {
+            if (P == 1) return v;
+            if (tid == 0) dts[part] = v;
+            ++phase;
+            tube_barrier(bar, P * phase, A.err);
+            double m = 1.0e300;
+            for (int q = 0; q < P; ++q) m = dmin_ref(m, __hip_atomic_load(&dts[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            return m;
+        };
         // acoustic dt of the initial state (computeAcousticTimeStep,
         // src/RKTimeStepping.cpp:43-76, on the state's primitive arrays)
         double dtc = 1.0e300;
-        for (int c = tid; c < N; c += BLOCK) {
+        for (int c = c0 + tid; c < c1; c += BLOCK) {
             const T* Ub = A.U[A.cur];
             const double rho = (double)Ub[o + c];
             const double p = A.presIC[o + c], u = A.velIC[o + c];
             const double cs = sqrt(g * dmax_ref(p + pinf, 1e-14) / dmax_ref(rho, 1e-14));
             dtc = dmin_ref(dtc, dx[c] / (fabs(u) + cs));
         }
-        dtc = block_min_d(dtc, red);
+        dtc = group_min(block_min_d(dtc, red));
         double t = 0.0;
         int step = 0;
         int a = A.cur, b = (A.cur + 1) % 3, cbuf = (A.cur + 2) % 3;
@@ -121,7 +172,7 @@ __global__ void __launch_bounds__(BLOCK) ensemble_kernel(EnsArgs<T> A) {
                 T* D = A.U[dst[s]] + o;
                 const long long nt = A.ntot;
                 for (int k = 0; k < ntile; ++k) {
-                    const int base = k * BLOCK;
+                    const int base = c0 + k * BLOCK;
                     // 1. primitives of cells base-3 .. base+BLOCK+2 (outflow ends)
                     for (int m = tid; m < W; m += BLOCK) {
                         int c = base - 3 + m;
@@ -166,7 +217,7 @@ __global__ void __launch_bounds__(BLOCK) ensemble_kernel(EnsArgs<T> A) {
                     __syncthreads();
                     // 4. update of the owned cell
                     const int c = base + tid;
-                    if (c < N) {
+                    if (c < c1) {
                         const double cx = (1.0 * 1.0) / ((dx[c] * 1.0) * 1.0);
                         const T cT = (T)cx;
                         double un[3];
@@ -192,12 +243,17 @@ __global__ void __launch_bounds__(BLOCK) ensemble_kernel(EnsArgs<T> A) {
                     }
                     __syncthreads();  // X is rewritten by the next tile
                 }
+                // the next stage reads the whole tube: wait for the other parts
+                if (P > 1 && !last) {
+                    ++phase;
+                    tube_barrier(bar, P * phase, A.err);
+                }
             }
-            dtc = block_min_d(dtNew, red);
+            dtc = group_min(block_min_d(dtNew, red));
             t += dt;
             ++step;
         }
-        if (tid == 0) {
+        if (tid == 0 && part == 0) {
             A.steps[tube] = step;
             A.time[tube] = t;
             if (bad) atomicOr(A.status, 1);

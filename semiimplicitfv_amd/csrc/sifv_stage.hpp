@@ -474,11 +474,18 @@ struct MarchTuning<float> {
     static constexpr bool kPreloadUpdate = SIFV_PRELOAD_UPDATE_F;
 };
 
+// The two-phase kernels hold nine fields per window row; asking for two
+// waves per SIMD costs them a few spilled registers but doubles the latency
+// hiding of a one-wave schedule.
+template <int DIM, int NPH>
+constexpr int stage_min_waves() { return NPH > 0 ? 2 : 1; }
+
 // ===========================================================================
 // x + y kernel (1D/2D full stage; 3D x-y part on one plane)
 // ===========================================================================
 template <int DIM, int R, int RS, int NPH, bool SEMI, int MODE, typename T>
-__global__ void __launch_bounds__(64) stage_xy_kernel(StageParams<T> P) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_waves<DIM, NPH>(), 8)))
+stage_xy_kernel(StageParams<T> P) {
     constexpr int NV = n_recon<DIM, NPH>();   // reconstructed variables
     constexpr int NF = n_fc<DIM, NPH>();      // face contribution length
     constexpr int NFL = DIM + 4 + NPH;        // Riemann output length
