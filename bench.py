@@ -220,12 +220,17 @@ def bench_adjoint(n: int = 512, nsteps: int = 400, every: int = 20, ndir: int = 
                     "directions can cross one; the adjoint itself matches exact tangents to 1e-9 in the tests"}
 
 
-def bench_bubble(n: int = 512, steps: int = 3):
+def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred: bool = False):
     """3D water/air bubble (unit cube, n^3 cells, air sphere r = 0.2, alpha
     tanh over 2 cells, alphaMin 1e-8, slip walls, p = 1e5 at rest), semi-
     implicit SSP-RK3 with the Jacobi pressure solve (tol 1e-6, <= 1000
     sweeps), WENO5 + HLLC, FP64.  Fields are built one at a time to bound host
-    memory.  One step = 3 advective stages + 3 pressure solves + corrections."""
+    memory.  One step = 3 advective stages + 3 pressure solves + corrections.
+    precision "mixed": the same FP64 state with the pressure solve run as FP32
+    Jacobi sweeps refined by FP64 residual passes.  stirred: the velocity
+    field of the 12^3 bubble fixture (u = 2 sin(pi x) cos(pi y), v = -2 cos(pi x)
+    sin(pi y), w = 0.5 sin(pi z)), which makes every pressure solve run to its
+    1000-sweep cap."""
     import numpy as np
 
     from semiimplicitfv_amd import (BoundaryCondition, HLLCSolver, JacobiPressureSolver, PhaseEOS,
@@ -247,7 +252,7 @@ def bench_bubble(n: int = 512, steps: int = 3):
         mesh.setBoundaryCondition(f, BoundaryCondition.SlipWall)
     eos = StiffenedGasEOS(4.4, 6.0e8, 1.0, cfg)
     t0 = time.perf_counter()
-    g = SemiImplicitSolver(mesh, HLLCSolver(eos), JacobiPressureSolver(), eos, None, cfg)
+    g = SemiImplicitSolver(mesh, HLLCSolver(eos), JacobiPressureSolver(), eos, None, cfg, precision=precision)
     xc, yc, zc = mesh.centroids()
     r = np.sqrt((xc[None, None, :] - 0.5) ** 2 + (yc[None, :, None] - 0.5) ** 2 + (zc[:, None, None] - 0.5) ** 2)
     amin = cfg.multiPhaseParams.alphaMin
@@ -263,6 +268,17 @@ def bench_bubble(n: int = 512, steps: int = 3):
         _abi.F_PRES: lambda: np.full(aw.shape, p0),
         _abi.F_RHOE: lambda: aw * (p0 + 4.4 * 6.0e8) / 3.4 + (1.0 - aw) * p0 / 0.4,
     }
+    if stirred:
+        X, Y, Z = xc[None, None, :], yc[None, :, None], zc[:, None, None]
+        u = (2.0 * np.sin(np.pi * X) * np.cos(np.pi * Y), -2.0 * np.cos(np.pi * X) * np.sin(np.pi * Y),
+             0.5 * np.sin(np.pi * Z))
+        rho_f = lambda: aw * 1000.0 + (1.0 - aw) * 1.0  # noqa: E731
+        ke = lambda: 0.5 * rho_f() * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2)  # noqa: E731
+        for d, (mf, vf) in enumerate(((_abi.F_RHOU, _abi.F_VELU), (_abi.F_RHOV, _abi.F_VELV), (_abi.F_RHOW, _abi.F_VELW))):
+            fields[mf] = (lambda d=d: rho_f() * u[d] * np.ones_like(aw))
+            fields[vf] = (lambda d=d: u[d] * np.ones_like(aw))
+        e0 = fields[_abi.F_RHOE]
+        fields[_abi.F_RHOE] = lambda: e0() + ke()
     buf = np.zeros(mesh.totalCells())
     for fid, make in fields.items():
         buf[...] = 0.0
@@ -281,12 +297,13 @@ def bench_bubble(n: int = 512, steps: int = 3):
         iters.append(g.lastPressureIterations())
     g.synchronize()
     el = time.perf_counter() - t0
-    rho = mesh.interior(g.download_field(_abi.F_RHO))
-    finite = bool(np.isfinite(rho).all())
+    pres = mesh.interior(g.download_field(_abi.F_PRES)).copy()
+    finite = bool(np.isfinite(pres).all())
     g.close()
-    return {"workload": f"3D water/air bubble {n}^3, semi-implicit SSP-RK3 + Jacobi (tol 1e-6), WENO5+HLLC, FP64",
+    return {"workload": f"3D water/air bubble {n}^3, semi-implicit SSP-RK3 + Jacobi (tol 1e-6), WENO5+HLLC, FP64 "
+                        + ("state, FP32 Jacobi sweeps + FP64 refinement" if precision == "mixed" else "throughout"),
             "value": n ** 3 * steps / el, "unit": "cell-updates/s", "ms_per_step": el / steps * 1e3,
-            "jacobi_iterations_last_stage": iters, "host_setup_s": setup_s, "finite": finite}
+            "jacobi_iterations_last_stage": iters, "host_setup_s": setup_s, "finite": finite, "_pres": pres}
 
 
 def measured_traffic(n, precision):
@@ -400,8 +417,20 @@ def main():
                                     "rho_rel_Linf_vs_fp64": linf, "roofline_achieved_GBs": mb / mm / 1e9,
                                     "mean_launch_ms": mm * 1e3,
                                     "note": "error against the FP64 GPU run, itself within 1e-12 of the FP64 oracle"}
-    for name, fn in (("ensemble", bench_ensemble), ("adjoint", bench_adjoint),
-                     ("bubble", lambda: bench_bubble(a.bubble_n))):
+    def bubble_pair():
+        import numpy as np
+
+        out = {}
+        for tag, n, steps, stirred in (("at_rest", a.bubble_n, 3, False), ("stirred_256", 256, 1, True)):
+            r64 = bench_bubble(n, steps, precision="fp64", stirred=stirred)
+            r32 = bench_bubble(n, steps, precision="mixed", stirred=stirred)
+            p64, p32 = r64.pop("_pres"), r32.pop("_pres")
+            r32["pres_rel_L1_vs_fp64"] = float(np.abs(p32 - p64).sum() / np.abs(p64).sum())
+            r32["ratio_vs_fp64"] = r32["value"] / r64["value"]
+            out[tag] = {"fp64": r64, "fp32_sweeps": r32}
+        return out
+
+    for name, fn in (("ensemble", bench_ensemble), ("adjoint", bench_adjoint), ("bubble", bubble_pair)):
         if name in want:
             t0 = time.perf_counter()
             try:

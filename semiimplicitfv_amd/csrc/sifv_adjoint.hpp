@@ -151,9 +151,8 @@ __global__ void __launch_bounds__(256) adj_prim_kernel(AdjArgs A) {
     e.gamma = A.gamma;
     e.pInf = 0.0;
     e.stiffened = 0;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        int i, j;
-        adj_ijk(g, t, i, j);
+    int i, j, k_;
+    for (InteriorIter it_(g.nx, g.ny, 1); it_.next(i, j, k_);) {
         const size_t id = cell_index(g, i, j, 0);
         double mom[2] = {A.U[A.total + id], A.U[2 * A.total + id]}, u[2], p;
         cons_to_prim<2>(e, A.U[id], mom, A.U[3 * A.total + id], u, p);
@@ -252,9 +251,8 @@ __global__ void __launch_bounds__(256) adj_face_kernel(AdjArgs A) {
     const size_t n = (size_t)g.nx * g.ny;
     const size_t T = A.total;
     double gsum = 0.0;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        int i, j;
-        adj_ijk(g, t, i, j);
+    int i, j, k_;
+    for (InteriorIter it_(g.nx, g.ny, 1); it_.next(i, j, k_);) {
         // face at the low side of cell R = (i, j) in direction DIR; L = R - e_DIR
         const int iL = DIR == 0 ? wrapi(i - 1, g.nx) : i;
         const int jL = DIR == 1 ? wrapi(j - 1, g.ny) : j;
@@ -301,9 +299,8 @@ __global__ void __launch_bounds__(256) adj_recon_kernel(AdjArgs A) {
     const Grid& g = A.g;
     const size_t n = (size_t)g.nx * g.ny;
     const size_t T = A.total;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        int i, j;
-        adj_ijk(g, t, i, j);
+    int i, j, k_;
+    for (InteriorIter it_(g.nx, g.ny, 1); it_.next(i, j, k_);) {
         const size_t id = cell_index(g, i, j, 0);
 #pragma unroll
         for (int dir = 0; dir < 2; ++dir)
@@ -334,9 +331,8 @@ __global__ void __launch_bounds__(256) adj_gather_kernel(AdjArgs A) {
     const size_t T = A.total;
     const double gam = A.gamma;
     double gsum = 0.0;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        int i, j;
-        adj_ijk(g, t, i, j);
+    int i, j, k_;
+    for (InteriorIter it_(g.nx, g.ny, 1); it_.next(i, j, k_);) {
         const size_t id = cell_index(g, i, j, 0);
         double Wb[4] = {0, 0, 0, 0};
 #pragma unroll

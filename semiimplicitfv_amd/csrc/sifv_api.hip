@@ -175,6 +175,9 @@ struct sifv_handle {
     int igrEpoch = 0;
     bool sigPrim = false;
     bool unifY = false;       // all interior y (and z) widths are the same double
+    bool jacMixed = false;    // semi-implicit pressure solve: FP32 sweeps + FP64 refinement
+    double* jacPn = nullptr;  // FP64 Jacobi update of the refined iterate
+    float* jac32 = nullptr;   // rho, co, r, d0, d1 (FP32)
     // discrete adjoint: checkpoints of the forward run and work arrays
     std::vector<double*> adjCk;   // states at steps 0, k, 2k, ...
     std::vector<double*> adjSeg;  // states of one segment during the backward sweep
@@ -648,6 +651,91 @@ int run_jacobi(sifv_handle* h, const double* rho, const double* dt, double tol, 
     return SIFV_OK;
 }
 
+// Pressure solve with FP32 Jacobi sweeps and FP64 residual refinement (see
+// jacobi_mix_pass_kernel): an FP64 pass measures the reference's update size
+// max |pn - p| and forms the residual, kInner FP32 sweeps solve for the
+// correction (2 sweeps after the first pass, doubling up to 16, so that a solve
+// the FP64 sweep finishes in a few iterations stays short), repeat.
+// *n counts FP64 passes and FP32 sweeps together, capped at
+// maxIter; *pfinal is the FP64 update of the last pass (the reference's answer
+// when the update size fell below tol).
+int run_jacobi_mixed(sifv_handle* h, const double* rho, const double* dt, double tol, int maxIter, int* n,
+                     const double** pfinal) {
+    int inner = 2;
+    const size_t T = h->total;
+    if (!h->jacPn) {
+        HIPCHK(hipMalloc(&h->jacPn, sizeof(double) * T));
+        HIPCHK(hipMemset(h->jacPn, 0, sizeof(double) * T));
+        HIPCHK(hipMalloc(&h->jac32, sizeof(float) * 5 * T));
+        HIPCHK(hipMemset(h->jac32, 0, sizeof(float) * 5 * T));
+    }
+    float* rho32 = h->jac32;
+    float* co32 = h->jac32 + T;
+    float* r32 = h->jac32 + 2 * T;
+    float* dbuf[2] = {h->jac32 + 3 * T, h->jac32 + 4 * T};
+    const int nb = elementwise_blocks(h);
+    hipLaunchKernelGGL(to_f32_kernel, dim3(std::min<size_t>((T + 255) / 256, 8192)), dim3(256), 0, h->stream, rho,
+                       rho32, T);
+    HIPCHK(hipGetLastError());
+    JacobiMixArgs M;
+    M.g = h->grid();
+    M.dim = h->dim;
+    M.rho = rho;
+    M.rhoc2 = h->jac[2];
+    M.rhs = h->jac[3];
+    M.pn = h->jacPn;
+    M.r32 = r32;
+    M.co32 = co32;
+    M.rho32 = rho32;
+    M.resid = h->resid;
+    M.dt = dt;
+    Jacobi32Args S;
+    S.g = h->grid();
+    S.dim = h->dim;
+    S.rho32 = rho32;
+    S.co32 = co32;
+    S.r32 = r32;
+    double* pin = h->jac[0];
+    double* pout = h->jac[1];
+    const float* din = nullptr;
+    int total = 0, slot = 0;
+    *n = maxIter;
+    while (true) {
+        M.pin = pin;
+        M.din = din;
+        M.pout = pout;
+        M.slot = slot;
+        void* args[] = {&M};
+        int rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_mix_pass_kernel), dim3(nb), dim3(256), args);
+        if (rc) return rc;
+        ++total;
+        HIPCHK(hipMemcpyAsync(h->hostResid + slot, h->resid + slot, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (h->hostResid[slot] < tol) {
+            *n = total;
+            break;
+        }
+        if (total >= maxIter) break;
+        const int k = std::min(inner, maxIter - total);
+        inner = std::min(inner * 2, 16);
+        const float* d = nullptr;
+        for (int q = 0; q < k; ++q) {
+            S.din = d;
+            S.dout = dbuf[q % 2];
+            void* sargs[] = {&S};
+            rc = timed_launch(h, reinterpret_cast<const void*>(jacobi32_kernel), dim3(nb), dim3(256), sargs);
+            if (rc) return rc;
+            d = dbuf[q % 2];
+        }
+        total += k;
+        din = d;
+        std::swap(pin, pout);
+        ++slot;
+    }
+    *pfinal = h->jacPn;
+    return SIFV_OK;
+}
+
 // One semi-implicit step (SemiImplicitSolver::step, src/SemiImplicitSolver.cpp:106-236):
 // advective dt, then per RK stage the fused advective stage kernel (star state,
 // advected pressure), PRIM ghosts of the star density / velocity, the pressure
@@ -712,7 +800,13 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
         hipLaunchKernelGGL(semi_setup_kernel, dim3(nb), dim3(256), 0, h->stream, S);
         HIPCHK(hipGetLastError());
         int n = 0;
-        rc = run_jacobi(h, rho, h->scal, sp.pressureTol, sp.maxPressureIters, &n);
+        const double* pfinal = nullptr;
+        if (h->jacMixed) {
+            rc = run_jacobi_mixed(h, rho, h->scal, sp.pressureTol, sp.maxPressureIters, &n, &pfinal);
+        } else {
+            rc = run_jacobi(h, rho, h->scal, sp.pressureTol, sp.maxPressureIters, &n);
+            pfinal = h->jac[n % 2];
+        }
         if (rc) return rc;
         h->lastIters = n;
         CorrectArgs K;
@@ -723,7 +817,7 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
         K.nph = h->nph;
         K.total = h->total;
         K.cons = h->consField<double>(dst[s], 0);
-        K.p = h->jac[n % 2];
+        K.p = pfinal;
         K.sigma = h->sigma;
         for (int d = 0; d < 3; ++d) K.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
         K.pres = h->pres;
@@ -956,8 +1050,8 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
         if (mesh->bc[f] < 0 || mesh->bc[f] > 4) return fail(SIFV_ERR_INVALID, "unknown boundary condition");
     if (cfg->multiPhaseParams.nPhases != 0 && cfg->multiPhaseParams.nPhases != 2)
         return fail(SIFV_ERR_UNSUPPORTED, "the GPU path carries single-phase or two-phase states");
-    if (precision == SIFV_MIXED && (cfg->semiImplicit || cfg->multiPhaseParams.nPhases > 0))
-        return fail(SIFV_ERR_UNSUPPORTED, "mixed precision is available for the single-phase explicit path only");
+    if (precision == SIFV_MIXED && !cfg->semiImplicit && cfg->multiPhaseParams.nPhases > 0)
+        return fail(SIFV_ERR_UNSUPPORTED, "the FP32-bulk explicit form is available for single-phase states only");
 
     auto* h = new sifv_handle();
     h->dim = mesh->dim;
@@ -975,7 +1069,9 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
     h->cfg = *cfg;
     h->riemann = riemann;
     h->eos = *eos;
-    h->precision = precision;
+    // semi-implicit + SIFV_MIXED: FP64 state, Jacobi sweeps in FP32 with FP64 refinement
+    h->jacMixed = cfg->semiImplicit && precision == SIFV_MIXED;
+    h->precision = h->jacMixed ? SIFV_FP64 : precision;
     h->nph = cfg->multiPhaseParams.nPhases;
     h->nc = h->dim + 2 + 2 * h->nph;
     const int nn[3] = {h->nx, h->ny, h->nz};
@@ -1082,6 +1178,8 @@ int sifv_destroy(sifv_handle* h) {
     hipFree(h->scal);
     hipFree(h->flags);
     hipFree(h->part);
+    hipFree(h->jacPn);
+    hipFree(h->jac32);
     for (double* c : h->adjCk) hipFree(c);
     for (double* c : h->adjSeg) hipFree(c);
     hipFree(h->adjWork);

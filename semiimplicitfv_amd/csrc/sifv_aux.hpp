@@ -54,11 +54,8 @@ __global__ void __launch_bounds__(kRedThreads) dt_cells_kernel(DtArgs A) {
     const Grid& g = A.g;
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     double m = 1.0e300;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (size_t)gridDim.x * blockDim.x) {
-        int i = (int)(t % g.nx);
-        int j = (int)((t / g.nx) % g.ny);
-        int k = (int)(t / ((size_t)g.nx * g.ny));
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         size_t id = cell_index(g, i, j, k);
         const double h[3] = {g.dxw[i + g.ngx], g.dyw[j + g.ngy], g.dzw[k + g.ngz]};
         double dc;
@@ -161,11 +158,8 @@ __global__ void __launch_bounds__(256) cons_to_prim_kernel(C2PArgs<T> A) {
     const Grid& g = A.g;
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const int dim = g.dim;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (size_t)gridDim.x * blockDim.x) {
-        int i = (int)(t % g.nx);
-        int j = (int)((t / g.nx) % g.ny);
-        int k = (int)(t / ((size_t)g.nx * g.ny));
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         size_t id = cell_index(g, i, j, k);
         if (A.mix.nph > 0) {
             double r = 0.0;

@@ -37,6 +37,33 @@ __device__ __forceinline__ size_t cell_index(const Grid& g, int i, int j, int k)
 // RectilinearMesh::fillGhostX/Y/Z (src/RectilinearMesh.cpp:137-418) and the
 // periodic self exchange (src/HaloExchange.cpp:414-465).  flipN/flipT report
 // the normal / tangential velocity sign flips of wall rules.
+// Grid-stride walk over the interior cells of a block's share: work items are
+// (row, chunk of up to kIterChunk columns) so that consecutive threads take
+// consecutive x, and the 32-bit index split happens once per item, not per cell.
+constexpr int kIterChunk = 2048;
+struct InteriorIter {
+    int nx, ny, C, items, item, i, iend, j, k;
+    __device__ __forceinline__ explicit InteriorIter(int nx_, int ny_, int nz_)
+        : nx(nx_), ny(ny_), C((nx_ + kIterChunk - 1) / kIterChunk), items(ny_ * nz_ * ((nx_ + kIterChunk - 1) / kIterChunk)),
+          item((int)blockIdx.x - (int)gridDim.x), i(0), iend(0), j(0), k(0) {}
+    __device__ __forceinline__ bool next(int& oi, int& oj, int& ok) {
+        i += blockDim.x;
+        while (i >= iend) {
+            item += gridDim.x;
+            if (item >= items) return false;
+            const int row = item / C, xc = item - row * C;
+            j = row % ny;
+            k = row / ny;
+            i = xc * kIterChunk + (int)threadIdx.x;
+            iend = (xc + 1) * kIterChunk < nx ? (xc + 1) * kIterChunk : nx;
+        }
+        oi = i;
+        oj = j;
+        ok = k;
+        return true;
+    }
+};
+
 __device__ __forceinline__ int map_axis(int c, int n, int lo, int hi, bool& flipN, bool& flipT) {
     if (c < 0) {
         switch (lo) {

@@ -43,10 +43,8 @@ __global__ void __launch_bounds__(256) igr_pre_kernel(IgrPreArgs<T> A) {
     const Grid& g = P.g;
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const double alpha = A.alpha;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(t % g.nx);
-        const int j = (int)((t / g.nx) % g.ny);
-        const int k = (int)(t / ((size_t)g.nx * g.ny));
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
         Cell<DIM, NPH> c, m, p;
         load_cell<DIM, NPH>(P, P.src, i, j, k, c);

@@ -51,9 +51,8 @@ __global__ void __launch_bounds__(256) semi_setup_kernel(SemiArgs A) {
     const double dt = *A.dt;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)A.maxIter; t += stride)
         A.resid[t] = 0.0;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
-        int i, j, k;
-        interior_ijk(g, t, i, j, k);
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
         const double rho = A.rho[id];
         const double p = A.pres[id];
@@ -117,9 +116,8 @@ __global__ void __launch_bounds__(256) jacobi_kernel(JacobiArgs A) {
     const double dt = *A.dt;
     const double dt2 = dt * dt;
     double mr = 0.0;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        int i, j, k;
-        interior_ijk(g, t, i, j, k);
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
         const double co = A.rhoc2[id] * dt2;
         const double pc = A.pin[id];
@@ -177,9 +175,8 @@ __global__ void __launch_bounds__(256) jacobi_kernel(JacobiArgs A) {
 // Writes interior cells of `src` into `dst` (ghosts of dst untouched).
 __global__ void __launch_bounds__(256) copy_interior_kernel(Grid g, const double* src, double* dst) {
     const size_t n = (size_t)g.nx * g.ny * g.nz;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        int i, j, k;
-        interior_ijk(g, t, i, j, k);
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
         dst[id] = src[id];
     }
@@ -204,9 +201,8 @@ __global__ void __launch_bounds__(256) semi_correct_kernel(CorrectArgs A) {
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const double dt = *A.dt;
     const int dim = A.dim;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        int i, j, k;
-        interior_ijk(g, t, i, j, k);
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
         const double pc = A.p[id] + A.sigma[id];
         const double rho = A.cons[id];
@@ -242,6 +238,144 @@ __global__ void __launch_bounds__(256) semi_correct_kernel(CorrectArgs A) {
         }
         A.cons[(size_t)(dim + 1) * A.total + id] = E;
         A.pres[id] = p;
+    }
+}
+
+}  // namespace sifv
+
+namespace sifv {
+
+// ---- Jacobi with FP32 sweeps and FP64 residual refinement ----------------------
+// The system (I + dt^2 rho c^2 L) p = rhs is written as den_i p_i - co_i sum_j c_ij p_j = rhs_i
+// (den = 1 + co sum_j c_ij, co = dt^2 rho c^2).  An FP64 pass from the current
+// iterate p computes the reference's Jacobi update pn = (rhs + co off(p)) / den,
+// its size max |pn - p| (the reference's convergence measure) and the residual
+// r = den (pn - p) = rhs - A p; FP32 Jacobi sweeps then solve A d = r from d = 0,
+// and the next FP64 pass starts from p + d.  When the FP64 update size drops
+// below tol, pn is the answer, as in the reference.
+struct JacobiMixArgs {
+    Grid g;
+    int dim;
+    const double* rho;      // ghosts as given (PRIM-filled)
+    const double* rhoc2;
+    const double* rhs;
+    const double* pin;      // iterate before the correction
+    const float* din;       // correction from the FP32 sweeps (null: none)
+    double* pout;           // pin + din (interior)
+    double* pn;             // FP64 Jacobi update of pout
+    float* r32;             // residual rhs - A pout
+    float* co32;            // dt^2 rho c^2 (FP32, written by every pass)
+    float* rho32;           // rho (FP32, padded, written by every pass incl. ghosts it reads)
+    double* resid;          // resid[slot] accumulates max |pn - pout|
+    int slot;
+    const double* dt;
+};
+
+__device__ __forceinline__ double pd_at(const Grid& g, const double* p, const float* d, int i, int j, int k) {
+    int si = map_axis_scalar(i, g.nx, g.sbc[0], g.sbc[1]);
+    int sj = g.dim >= 2 ? map_axis_scalar(j, g.ny, g.sbc[2], g.sbc[3]) : j;
+    int sk = g.dim >= 3 ? map_axis_scalar(k, g.nz, g.sbc[4], g.sbc[5]) : k;
+    const size_t q = cell_index(g, si, sj, sk);
+    return d ? p[q] + (double)d[q] : p[q];
+}
+
+__global__ void __launch_bounds__(256) jacobi_mix_pass_kernel(JacobiMixArgs A) {
+    __shared__ double sh[256 / kWave];
+    const Grid& g = A.g;
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    const double dt = *A.dt;
+    const double dt2 = dt * dt;
+    double mr = 0.0;
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
+        const size_t id = cell_index(g, i, j, k);
+        const double co = A.rhoc2[id] * dt2;
+        const double pc = pd_at(g, A.pin, A.din, i, j, k);
+        const double rc = A.rho[id];
+        double off = 0.0, diag = 0.0;
+        const size_t st[3] = {1, (size_t)g.nxT, (size_t)g.nxT * g.nyT};
+        const double* hw[3] = {g.dxw, g.dyw, g.dzw};
+        const int ng[3] = {g.ngx, g.ngy, g.ngz};
+        const int ii[3] = {i, j, k};
+        for (int d = 0; d < A.dim; ++d) {
+            const double hl = hw[d][ii[d] - 1 + ng[d]], hc = hw[d][ii[d] + ng[d]], hr = hw[d][ii[d] + 1 + ng[d]];
+            const double rL = 0.5 * (rc + A.rho[id - st[d]]), rR = 0.5 * (rc + A.rho[id + st[d]]);
+            const double dL = 0.5 * (hl + hc), dR = 0.5 * (hc + hr);
+            const double cL = 1.0 / (dmax_ref(rL, 1e-14) * dL * hc);
+            const double cR = 1.0 / (dmax_ref(rR, 1e-14) * dR * hc);
+            const double pm = pd_at(g, A.pin, A.din, i - (d == 0), j - (d == 1), k - (d == 2));
+            const double pp = pd_at(g, A.pin, A.din, i + (d == 0), j + (d == 1), k + (d == 2));
+            off += cL * pm + cR * pp;
+            diag += cL + cR;
+        }
+        const double den = 1.0 + co * diag;
+        const double pn = (A.rhs[id] + co * off) / den;
+        A.pout[id] = pc;
+        A.pn[id] = pn;
+        A.r32[id] = (float)(den * (pn - pc));
+        A.co32[id] = (float)co;
+        mr = dmax_ref(mr, fabs(pn - pc));
+    }
+    double w = wave_max(mr);
+    const int wv = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    if (l == 0) sh[wv] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int q = 0; q < (int)(blockDim.x / kWave); ++q) m = dmax_ref(m, sh[q]);
+        atomicMax(reinterpret_cast<unsigned long long*>(&A.resid[A.slot]),
+                  (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// rho as FP32 over the padded array (ghosts included, as the sweeps read them).
+__global__ void __launch_bounds__(256) to_f32_kernel(const double* a, float* b, size_t n) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        b[t] = (float)a[t];
+}
+
+struct Jacobi32Args {
+    Grid g;
+    int dim;
+    const float* rho32;
+    const float* co32;
+    const float* r32;
+    const float* din;       // null on the first sweep (d = 0)
+    float* dout;
+};
+
+__device__ __forceinline__ float d_at(const Grid& g, const float* d, int i, int j, int k) {
+    if (!d) return 0.0f;
+    int si = map_axis_scalar(i, g.nx, g.sbc[0], g.sbc[1]);
+    int sj = g.dim >= 2 ? map_axis_scalar(j, g.ny, g.sbc[2], g.sbc[3]) : j;
+    int sk = g.dim >= 3 ? map_axis_scalar(k, g.nz, g.sbc[4], g.sbc[5]) : k;
+    return d[cell_index(g, si, sj, sk)];
+}
+
+__global__ void __launch_bounds__(256) jacobi32_kernel(Jacobi32Args A) {
+    const Grid& g = A.g;
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
+        const size_t id = cell_index(g, i, j, k);
+        const float co = A.co32[id];
+        const float rc = A.rho32[id];
+        float off = 0.0f, diag = 0.0f;
+        const size_t st[3] = {1, (size_t)g.nxT, (size_t)g.nxT * g.nyT};
+        const double* hw[3] = {g.dxw, g.dyw, g.dzw};
+        const int ng[3] = {g.ngx, g.ngy, g.ngz};
+        const int ii[3] = {i, j, k};
+        for (int d = 0; d < A.dim; ++d) {
+            const float hl = (float)hw[d][ii[d] - 1 + ng[d]], hc = (float)hw[d][ii[d] + ng[d]];
+            const float hr = (float)hw[d][ii[d] + 1 + ng[d]];
+            const float rL = 0.5f * (rc + A.rho32[id - st[d]]), rR = 0.5f * (rc + A.rho32[id + st[d]]);
+            const float cL = __builtin_amdgcn_rcpf(fmaxf(rL, 1e-14f) * (0.5f * (hl + hc)) * hc);
+            const float cR = __builtin_amdgcn_rcpf(fmaxf(rR, 1e-14f) * (0.5f * (hc + hr)) * hc);
+            off += cL * d_at(g, A.din, i - (d == 0), j - (d == 1), k - (d == 2)) +
+                   cR * d_at(g, A.din, i + (d == 0), j + (d == 1), k + (d == 2));
+            diag += cL + cR;
+        }
+        A.dout[id] = (A.r32[id] + co * off) * __builtin_amdgcn_rcpf(1.0f + co * diag);
     }
 }
 

@@ -148,3 +148,34 @@ def test_jacobi_standalone(gpu, oracle_c, tol, maxit):
         l1, li = rel_errs(mesh.interior(pg), mesh.interior(dgold["p_tol1e-06"]))
         assert l1 <= 1e-14 and li <= 1e-14
     g.close()
+
+
+@pytest.mark.parametrize("case", ["s2", "s3", "single2d"])
+def test_mixed_precision_jacobi(gpu, case):
+    """Semi-implicit step with FP32 Jacobi sweeps and FP64 residual
+    refinement (precision="mixed") against the FP64 solve: both stop when the
+    FP64 Jacobi update falls below pressureTol, so they differ by the solver
+    tolerance, not by FP32 rounding: relative L1 of every field <= 1e-8."""
+    if case == "single2d":
+        cfg, mesh, st, eos = make_semi_case(2, (40, 33), seed=4, periods=(1, 0, 0), tol=1e-8)
+    else:
+        build, rs, nsteps, _ = problems.two_phase_cases()[case]
+        cfg, mesh, st, eos = build()
+    out = []
+    for prec in ("fp64", "mixed"):
+        g = SemiImplicitSolver(mesh, HLLCSolver(eos), JacobiPressureSolver(), eos, None, cfg, precision=prec)
+        s = st.copy()
+        g.upload(s)
+        its = []
+        for k in range(3):
+            cfg.step = k
+            g.step_resident(cfg)
+            its.append(g.lastPressureIterations())
+        g.download(s)
+        g.close()
+        out.append((s, its))
+    (a, ia), (b, ib) = out
+    assert all(0 < x <= cfg.semiImplicitParams.maxPressureIters for x in ib)
+    for fid in a.field_ids():
+        l1, li = rel_errs(mesh.interior(b.field(fid)), mesh.interior(a.field(fid)))
+        assert l1 <= 1e-8, (fid, l1, li, ia, ib)
