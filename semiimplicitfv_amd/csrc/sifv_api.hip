@@ -614,7 +614,25 @@ int launch_copy_interior(sifv_handle* h, const double* src, double* dst) {
 // predecessor converged returns at once, so a chunk may overshoot harmlessly.
 // *n = sweeps done (the reference's iteration count); iterate k lives in
 // jac[k % 2] and h->hostResid[0..n) holds the residual history.
+// Grid of the tiled sweeps (jacobi_tile_kernel): 64-column x kTileRows-row tiles,
+// kTilePlanes planes per block in 3D.
+dim3 jacobi_tile_grid(const sifv_handle* h) {
+    return dim3((h->nx + 63) / 64, (h->ny + kTileRows - 1) / kTileRows,
+                h->dim >= 3 ? (h->nz + kTilePlanes - 1) / kTilePlanes : 1);
+}
+
 int run_jacobi(sifv_handle* h, const double* rho, const double* dt, double tol, int maxIter, int* n) {
+    // 2D/3D: tiled z-march sweeps; 1D: the elementwise sweep
+    const bool tiled = h->dim >= 2;
+    JacobiTileArgs<double> Jt;
+    Jt.g = h->grid();
+    Jt.dim = h->dim;
+    Jt.rho = rho;
+    Jt.co = h->jac[2];
+    Jt.rhs = h->jac[3];
+    Jt.resid = h->resid;
+    Jt.tol = tol;
+    Jt.dt = dt;
     JacobiArgs J;
     J.g = h->grid();
     J.dim = h->dim;
@@ -630,11 +648,21 @@ int run_jacobi(sifv_handle* h, const double* rho, const double* dt, double tol, 
     while (it < maxIter) {
         const int end = std::min(maxIter, it + chunk);
         for (; it < end; ++it) {
-            J.iter = it;
-            J.pin = h->jac[it % 2];
-            J.pout = h->jac[(it + 1) % 2];
-            void* args[] = {&J};
-            int rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_kernel), dim3(nb), dim3(256), args);
+            int rc;
+            if (tiled) {
+                Jt.iter = it;
+                Jt.pin = h->jac[it % 2];
+                Jt.pout = h->jac[(it + 1) % 2];
+                void* args[] = {&Jt};
+                rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_tile_kernel<double>), jacobi_tile_grid(h),
+                                  dim3(64, kTileRows), args);
+            } else {
+                J.iter = it;
+                J.pin = h->jac[it % 2];
+                J.pout = h->jac[(it + 1) % 2];
+                void* args[] = {&J};
+                rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_kernel), dim3(nb), dim3(256), args);
+            }
             if (rc) return rc;
         }
         HIPCHK(hipMemcpyAsync(h->hostResid + seen, h->resid + seen, sizeof(double) * (end - seen),
@@ -695,6 +723,17 @@ int run_jacobi_mixed(sifv_handle* h, const double* rho, const double* dt, double
     S.rho32 = rho32;
     S.co32 = co32;
     S.r32 = r32;
+    const bool tiled = h->dim >= 2;
+    JacobiTileArgs<float> St;
+    St.g = h->grid();
+    St.dim = h->dim;
+    St.rho = rho32;
+    St.co = co32;
+    St.rhs = r32;
+    St.resid = nullptr;
+    St.iter = 0;
+    St.tol = 0.0;
+    St.dt = nullptr;
     double* pin = h->jac[0];
     double* pout = h->jac[1];
     const float* din = nullptr;
@@ -720,10 +759,18 @@ int run_jacobi_mixed(sifv_handle* h, const double* rho, const double* dt, double
         inner = std::min(inner * 2, 16);
         const float* d = nullptr;
         for (int q = 0; q < k; ++q) {
-            S.din = d;
-            S.dout = dbuf[q % 2];
-            void* sargs[] = {&S};
-            rc = timed_launch(h, reinterpret_cast<const void*>(jacobi32_kernel), dim3(nb), dim3(256), sargs);
+            if (tiled) {
+                St.pin = d;
+                St.pout = dbuf[q % 2];
+                void* sargs[] = {&St};
+                rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_tile_kernel<float>), jacobi_tile_grid(h),
+                                  dim3(64, kTileRows), sargs);
+            } else {
+                S.din = d;
+                S.dout = dbuf[q % 2];
+                void* sargs[] = {&S};
+                rc = timed_launch(h, reinterpret_cast<const void*>(jacobi32_kernel), dim3(nb), dim3(256), sargs);
+            }
             if (rc) return rc;
             d = dbuf[q % 2];
         }

@@ -379,4 +379,212 @@ __global__ void __launch_bounds__(256) jacobi32_kernel(Jacobi32Args A) {
     }
 }
 
+// ---- Tiled Jacobi sweeps (2D/3D) ----------------------------------------------------
+// The Jacobi sweeps above (the FP64 update with its max-norm, and the FP32 correction
+// sweep) organised for HBM: a block of 64 x kTileRows threads owns a tile of 64
+// columns x kTileRows rows and marches it through up to kTilePlanes planes of z.
+// The z-neighbours of p and rho stay in registers; each plane's p and rho tile with
+// a one-cell x/y halo is staged in LDS, double-buffered by plane parity (one barrier
+// per plane).  Lanes on the halo column i == nx / row j == ny load it; the tile's
+// outer halo comes from extra loads by lanes 0 / 63 and rows 0 / kTileRows-1.  p
+// goes through the scalar ghost rule, rho through its filled ghosts.  Per cell the
+// arithmetic is the elementwise kernels' expression for expression, so both forms
+// give identical bits.
+constexpr int kTileRows = 4;
+constexpr int kTilePlanes = 16;
+
+template <typename T>
+struct JacobiTileArgs {
+    Grid g;
+    int dim;
+    const T* rho;        // padded, ghosts filled (rho / rho32)
+    const T* co;         // FP64: rho c^2 (times dt^2 here); FP32: dt^2 rho c^2
+    const T* rhs;        // FP64: rhs; FP32: residual r32
+    const T* pin;        // iterate; FP32: null on the first sweep (zero)
+    T* pout;
+    double* resid;       // FP64: resid[iter] accumulates max |pNew - p|
+    int iter;
+    double tol;
+    const double* dt;    // FP64 only
+};
+
+template <typename T>
+__device__ __forceinline__ T tile_coef(T r, T dL, T hc) {
+    if constexpr (sizeof(T) == 8) return 1.0 / (dmax_ref(r, 1e-14) * dL * hc);
+    else return __builtin_amdgcn_rcpf(fmaxf(r, 1e-14f) * dL * hc);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64 * kTileRows) jacobi_tile_kernel(JacobiTileArgs<T> A) {
+    constexpr bool F64 = sizeof(T) == 8;
+    constexpr int TY = kTileRows;
+    if constexpr (F64) {
+        // converged at an earlier iteration: nothing left to do
+        if (A.iter > 0 && A.resid[A.iter - 1] < A.tol) return;
+    }
+    __shared__ T sp[2][TY + 2][66];
+    __shared__ T sr[2][TY + 2][66];
+    __shared__ double sh[TY];
+    const Grid& g = A.g;
+    const bool z = A.dim >= 3;
+    // launch order deals blocks round-robin to the 8 XCDs: hand each XCD a
+    // contiguous run of tiles so that neighbouring tiles share its L2
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    {
+        const int gx = gridDim.x, gy = gridDim.y, nb = gx * gy * gridDim.z;
+        if (nb % 8 == 0) {
+            const int L = bx + gx * (by + gy * bz);
+            const int t = (L % 8) * (nb / 8) + L / 8;
+            bx = t % gx;
+            by = (t / gx) % gy;
+            bz = t / (gx * gy);
+        }
+    }
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = bx * 64, j0 = by * TY;
+    const int i = i0 + tx, j = j0 + ty;
+    const int k0 = z ? bz * kTilePlanes : 0, k1 = z ? min(g.nz, k0 + kTilePlanes) : 1;
+    const long sy = g.nxT, sz = (long)g.nxT * g.nyT;
+    auto raw = [&](int a, int b, int c) {
+        return (long)(a + g.ngx) + sy * (long)(b + g.ngy) + sz * (long)(c + g.ngz);
+    };
+    const bool active = i < g.nx && j < g.ny;
+    const bool loads = i <= g.nx && j <= g.ny;
+    const int ic = min(i, g.nx), jc = min(j, g.ny);
+    const long pofs = (long)(map_axis_scalar(ic, g.nx, g.sbc[0], g.sbc[1]) - ic) +
+                      (long)(map_axis_scalar(jc, g.ny, g.sbc[2], g.sbc[3]) - jc) * sy;
+    const bool xh = (tx == 0 || (tx == 63 && i0 + 64 <= g.nx)) && j < g.ny;
+    const bool yh = (ty == 0 || (ty == TY - 1 && j0 + TY <= g.ny)) && i < g.nx;
+    const int hi = tx == 0 ? i0 - 1 : i0 + 64, hj = ty == 0 ? j0 - 1 : j0 + TY;
+    const int xs = tx == 0 ? 0 : 65, ys = ty == 0 ? 0 : TY + 1;
+    const long hx = raw(hi, jc, k0), hy = raw(ic, hj, k0);
+    const long hxP = (long)(map_axis_scalar(hi, g.nx, g.sbc[0], g.sbc[1]) - hi);
+    const long hyP = (long)(map_axis_scalar(hj, g.ny, g.sbc[2], g.sbc[3]) - hj) * sy;
+    const bool hasP = A.pin != nullptr;
+    auto zofs = [&](int k) { return (long)(map_axis_scalar(k, g.nz, g.sbc[4], g.sbc[5]) - k) * sz; };
+    T hcx = 0, dLx = 0, dRx = 0, hcy = 0, dLy = 0, dRy = 0;
+    if (active) {
+        const T hlx = (T)g.dxw[i - 1 + g.ngx], hrx = (T)g.dxw[i + 1 + g.ngx];
+        hcx = (T)g.dxw[i + g.ngx];
+        dLx = T(0.5) * (hlx + hcx);
+        dRx = T(0.5) * (hcx + hrx);
+        const T hly = (T)g.dyw[j - 1 + g.ngy], hry = (T)g.dyw[j + 1 + g.ngy];
+        hcy = (T)g.dyw[j + g.ngy];
+        dLy = T(0.5) * (hly + hcy);
+        dRy = T(0.5) * (hcy + hry);
+    }
+    double dt2 = 0.0;
+    if constexpr (F64) {
+        const double dt = *A.dt;
+        dt2 = dt * dt;
+    }
+    long id = raw(ic, jc, k0);
+    T pzm = 0, pc = 0, pzp = 0, rzm = 0, rc = 0, rzp = 0;
+    T hpx = 0, hrx = 0, hpy = 0, hry = 0;
+    if (loads) {
+        rc = A.rho[id];
+        if (hasP) pc = A.pin[id + pofs];
+        if (z) {
+            rzm = A.rho[id - sz];
+            if (hasP) pzm = A.pin[id + pofs - sz + zofs(k0 - 1)];
+        }
+    }
+    if (xh) {
+        hrx = A.rho[hx];
+        if (hasP) hpx = A.pin[hx + hxP];
+    }
+    if (yh) {
+        hry = A.rho[hy];
+        if (hasP) hpy = A.pin[hy + hyP];
+    }
+    double mr = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        const int b = k & 1;
+        sp[b][ty + 1][tx + 1] = pc;
+        sr[b][ty + 1][tx + 1] = rc;
+        if (xh) {
+            sp[b][ty + 1][xs] = hpx;
+            sr[b][ty + 1][xs] = hrx;
+        }
+        if (yh) {
+            sp[b][ys][tx + 1] = hpy;
+            sr[b][ys][tx + 1] = hry;
+        }
+        // operands of this plane not staged in LDS, and the next plane's halo
+        if (loads && z) {
+            rzp = A.rho[id + sz];
+            if (hasP) pzp = A.pin[id + pofs + sz + zofs(k + 1)];
+        }
+        T q2 = 0, qs = 0;
+        if (active) {
+            q2 = A.co[id];
+            qs = A.rhs[id];
+        }
+        const long dk = (long)(k + 1 - k0) * sz;
+        T nhpx = 0, nhrx = 0, nhpy = 0, nhry = 0;
+        if (k + 1 < k1) {
+            if (xh) {
+                nhrx = A.rho[hx + dk];
+                if (hasP) nhpx = A.pin[hx + dk + hxP];
+            }
+            if (yh) {
+                nhry = A.rho[hy + dk];
+                if (hasP) nhpy = A.pin[hy + dk + hyP];
+            }
+        }
+        __syncthreads();
+        if (active) {
+            T off = 0, diag = 0;
+            {
+                const T rL = T(0.5) * (rc + sr[b][ty + 1][tx]), rR = T(0.5) * (rc + sr[b][ty + 1][tx + 2]);
+                const T cL = tile_coef(rL, dLx, hcx), cR = tile_coef(rR, dRx, hcx);
+                off += cL * sp[b][ty + 1][tx] + cR * sp[b][ty + 1][tx + 2];
+                diag += cL + cR;
+            }
+            {
+                const T rL = T(0.5) * (rc + sr[b][ty][tx + 1]), rR = T(0.5) * (rc + sr[b][ty + 2][tx + 1]);
+                const T cL = tile_coef(rL, dLy, hcy), cR = tile_coef(rR, dRy, hcy);
+                off += cL * sp[b][ty][tx + 1] + cR * sp[b][ty + 2][tx + 1];
+                diag += cL + cR;
+            }
+            if (z) {
+                const T hl = (T)g.dzw[k - 1 + g.ngz], hc = (T)g.dzw[k + g.ngz], hr = (T)g.dzw[k + 1 + g.ngz];
+                const T rL = T(0.5) * (rc + rzm), rR = T(0.5) * (rc + rzp);
+                const T cL = tile_coef(rL, T(0.5) * (hl + hc), hc), cR = tile_coef(rR, T(0.5) * (hc + hr), hc);
+                off += cL * pzm + cR * pzp;
+                diag += cL + cR;
+            }
+            if constexpr (F64) {
+                const double co = q2 * dt2;
+                const double den = 1.0 + co * diag;
+                const double pn = (qs + co * off) / den;
+                A.pout[id] = pn;
+                mr = dmax_ref(mr, fabs(pn - pc));
+            } else {
+                A.pout[id] = (qs + q2 * off) * __builtin_amdgcn_rcpf(1.0f + q2 * diag);
+            }
+        }
+        pzm = pc;
+        pc = pzp;
+        rzm = rc;
+        rc = rzp;
+        hpx = nhpx;
+        hrx = nhrx;
+        hpy = nhpy;
+        hry = nhry;
+        id += sz;
+    }
+    if constexpr (F64) {
+        const double w = wave_max(mr);
+        if (tx == 0) sh[ty] = w;
+        __syncthreads();
+        if (tx == 0 && ty == 0) {
+            double m = 0.0;
+            for (int q = 0; q < TY; ++q) m = dmax_ref(m, sh[q]);
+            atomicMax(reinterpret_cast<unsigned long long*>(&A.resid[A.iter]),
+                      (unsigned long long)__double_as_longlong(m));
+        }
+    }
+}
+
 }  // namespace sifv

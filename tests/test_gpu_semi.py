@@ -150,6 +150,65 @@ def test_jacobi_standalone(gpu, oracle_c, tol, maxit):
     g.close()
 
 
+def _tile_problem(shape, periods, seed):
+    """Variable-coefficient pressure system on a stretched mesh whose extents
+    cross the sweep tiles' edges: x past one and two 64-column tiles, y not a
+    multiple of the tile rows, z past one 16-plane chunk."""
+    from semiimplicitfv_amd import RectilinearMesh, SimulationConfig
+    rng = np.random.default_rng(seed)
+    dim = len(shape)
+    cfg = SimulationConfig(dim=dim, nGhost=3, RKOrder=3, semiImplicit=True)
+    nodes = []
+    for n in shape:
+        h = 1.0 + 0.4 * rng.random(n)
+        nodes.append(np.concatenate([[0.0], np.cumsum(h / h.sum())]))
+    while len(nodes) < 3:
+        nodes.append(np.array([0.0, 1.0]))
+    mesh = RectilinearMesh(cfg, nodes[0], nodes[1], nodes[2], periods=periods)
+    for d in range(dim):
+        if not periods[d]:
+            mesh.setBoundaryCondition(2 * d, BoundaryCondition.SlipWall)
+            mesh.setBoundaryCondition(2 * d + 1, BoundaryCondition.SlipWall)
+    N = mesh.totalCells()
+    sh = tuple(reversed(shape))
+    rho = np.ones(N)
+    mesh.interior(rho)[...] = np.where(rng.random(sh) < 0.3, 1000.0, 1.0) * (1.0 + 0.1 * rng.random(sh))
+    rho = problems._fill_scalar_np(mesh, rho)
+    rhoc2 = np.zeros(N)
+    mesh.interior(rhoc2)[...] = 1.4e5 + 2.25e9 * rng.random(sh)
+    rhs = np.zeros(N)
+    mesh.interior(rhs)[...] = 1e5 + 1e3 * rng.standard_normal(sh)
+    p0 = np.zeros(N)
+    mesh.interior(p0)[...] = 1e5 + 10.0 * rng.standard_normal(sh)
+    return cfg, mesh, rho, rhoc2, rhs, p0, 2e-5
+
+
+@pytest.mark.parametrize("shape,periods", [((130, 9, 37), (1, 0, 0)), ((70, 6, 17), (0, 1, 1)),
+                                           ((129, 67), (0, 1, 0)), ((65, 5), (1, 1, 0))])
+def test_jacobi_tiles(gpu, oracle_c, shape, periods):
+    """Tiled Jacobi sweeps across tile, halo and plane-chunk edges: iteration
+    count, residual history and every cell against the oracle's sweep (the
+    same arithmetic, so to rounding), for a 9-sweep solve (bar 1e-14) and a
+    converged one of several hundred sweeps, where FMA-contraction differences
+    accumulate (bar 1e-12)."""
+    cfg, mesh, rho, rhoc2, rhs, p0, dt = _tile_problem(shape, periods, seed=sum(shape))
+    eos = IdealGasEOS(1.4, 1.0, cfg)
+    g = SemiImplicitSolver(mesh, HLLCSolver(eos), JacobiPressureSolver(), eos, None, cfg)
+    js = JacobiPressureSolver(g)
+    cpu = oracle_c.CpuSolver(mesh, cfg, 2, eos, "c")
+    for tol, maxit in ((1e-300, 9), (5.0, 2000)):
+        pg, pc = p0.copy(), p0.copy()
+        hg, hc = np.zeros(maxit), np.zeros(maxit)
+        ig = js.solve(mesh, rho, rhoc2, rhs, pg, dt, tol, maxit, history=hg)
+        ic = cpu.jacobi(rho, rhoc2, rhs, pc, dt, tol, maxit, hc)
+        assert ig == ic, (tol, ig, ic)
+        assert np.abs(hg[:ig] - hc[:ic]).max() <= 1e-9 * hc[0]
+        l1, li = rel_errs(mesh.interior(pg), mesh.interior(pc))
+        bar = 1e-14 if maxit < 10 else TOL
+        assert l1 <= bar and li <= bar, (tol, l1, li)
+    g.close()
+
+
 @pytest.mark.parametrize("case", ["s2", "s3", "single2d"])
 def test_mixed_precision_jacobi(gpu, case):
     """Semi-implicit step with FP32 Jacobi sweeps and FP64 residual
