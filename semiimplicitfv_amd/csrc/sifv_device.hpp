@@ -221,48 +221,70 @@ struct Recon<2, T> {
     }
 };
 
+// 1/x for a normal x in [1e-290, 1e290]: the hardware reciprocal refined by two
+// Newton-Raphson steps r <- r + r (1 - x r) (error well below one ulp of r).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// WENO5 of cell c = v[2].  With the first differences a..d of the stencil, the
+// quadratic of sub-stencil k (cells k..k+2) has second difference t_k and slope
+// term S_k (S_0 = 3b - a, S_1 = b + c, S_2 = 3c - d); its values at the cell's
+// high / low face are v[2] + (t_k +- 3 S_k) / 12, and its Jiang-Shu smoothness
+// indicator is beta_k = 13/12 t_k^2 + 1/4 S_k^2.  These are the candidate
+// polynomials and indicators of Reconstructor::weno5* (src/Reconstruction.cpp:
+// 119-200) rewritten around the cell value, so that both faces share the
+// differences and each face is v[2] plus a weighted correction.  The high face
+// weights sub-stencils 0,1,2 with d = 1/10, 6/10, 3/10; the low face, the mirror
+// image, with 3/10, 6/10, 1/10.
 template <typename T>
 struct Recon<3, T> {
     __device__ __forceinline__ static void run(const T* v, T eps, bool weno, T& minus, T& plus) {
-        const T c13 = T(1.0) / T(3.0), c76 = T(7.0) / T(6.0), c116 = T(11.0) / T(6.0);
-        const T c16 = T(1.0) / T(6.0), c56 = T(5.0) / T(6.0);
-        // left-biased candidates (face c+1/2)
-        T pl0 = c13 * v[0] - c76 * v[1] + c116 * v[2];
-        T pl1 = -c16 * v[1] + c56 * v[2] + c13 * v[3];
-        T pl2 = c13 * v[2] + c56 * v[3] - c16 * v[4];
-        // right-biased candidates (face c-1/2), mirrored stencil
-        T pr0 = c13 * v[4] - c76 * v[3] + c116 * v[2];
-        T pr1 = -c16 * v[3] + c56 * v[2] + c13 * v[1];
-        T pr2 = c13 * v[2] + c56 * v[1] - c16 * v[0];
+        const T a = v[1] - v[0], b = v[2] - v[1], c = v[3] - v[2], d = v[4] - v[3];
+        const T t0 = b - a, t1 = c - b, t2 = d - c;
+        const T S0 = T(3.0) * b - a, S1 = b + c, S2 = T(3.0) * c - d;
+        const T Gp0 = t0 + T(3.0) * S0, Gp1 = t1 + T(3.0) * S1, Gp2 = t2 + T(3.0) * S2;
+        const T Gm0 = t0 - T(3.0) * S0, Gm1 = t1 - T(3.0) * S1, Gm2 = t2 - T(3.0) * S2;
         const T d0 = T(1.0) / T(10.0), d1 = T(6.0) / T(10.0), d2 = T(3.0) / T(10.0);
+        const T c112 = T(1.0) / T(12.0);
         if (!weno) {
-            plus = d0 * pl0 + d1 * pl1 + d2 * pl2;
-            minus = d0 * pr0 + d1 * pr1 + d2 * pr2;
+            plus = v[2] + (d0 * Gp0 + d1 * Gp1 + d2 * Gp2) * c112;
+            minus = v[2] + (d2 * Gm0 + d1 * Gm1 + d0 * Gm2) * c112;
             return;
         }
         const T c1312 = T(13.0) / T(12.0), c14 = T(1.0) / T(4.0);
-        T t0 = v[0] - T(2.0) * v[1] + v[2], s0 = v[0] - T(4.0) * v[1] + T(3.0) * v[2];
-        T t1 = v[1] - T(2.0) * v[2] + v[3], s1 = v[1] - v[3];
-        T t2 = v[2] - T(2.0) * v[3] + v[4], s2 = T(3.0) * v[2] - T(4.0) * v[3] + v[4];
-        T b0 = c1312 * t0 * t0 + c14 * s0 * s0;
-        T b1 = c1312 * t1 * t1 + c14 * s1 * s1;
-        T b2 = c1312 * t2 * t2 + c14 * s2 * s2;
-        T e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
-        T Q0 = e0 * e0, Q1 = e1 * e1, Q2 = e2 * e2;
+        const T b0 = c1312 * t0 * t0 + c14 * S0 * S0;
+        const T b1 = c1312 * t1 * t1 + c14 * S1 * S1;
+        const T b2 = c1312 * t2 * t2 + c14 * S2 * S2;
+        const T e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+        const T Q0 = e0 * e0, Q1 = e1 * e1, Q2 = e2 * e2;
         if constexpr (sizeof(T) == 8) {
-            // plus: a_k = d_k / Q_k; minus: d0/Q2, d1/Q1, d2/Q0
+            // weights d_k / Q_k cleared of their denominators: high face
+            // (d0 Q1Q2, d1 Q0Q2, d2 Q0Q1), low face (d2 Q1Q2, d1 Q0Q2, d0 Q0Q1)
             const T P12 = Q1 * Q2, P02 = Q0 * Q2, P01 = Q0 * Q1;
-            const T wP0 = d0 * P12, wP1 = d1 * P02, wP2 = d2 * P01;
-            const T wM0 = d0 * P01, wM1 = d1 * P02, wM2 = d2 * P12;
-            const T nP = wP0 * pl0 + wP1 * pl1 + wP2 * pl2, dP = wP0 + wP1 + wP2;
-            const T nM = wM0 * pr0 + wM1 * pr1 + wM2 * pr2, dM = wM0 + wM1 + wM2;
-            weno_finish(nP, dP, nM, dM, plus, minus);
+            const T wP0 = d0 * P12, w1 = d1 * P02, wP2 = d2 * P01;
+            const T wM0 = d2 * P12, wM2 = d0 * P01;
+            const T nP = wP0 * Gp0 + w1 * Gp1 + wP2 * Gp2, dP = wP0 + w1 + wP2;
+            const T nM = wM0 * Gm0 + w1 * Gm1 + wM2 * Gm2, dM = wM0 + w1 + wM2;
+            const T pr = dP * dM;
+            if (pr <= 1.0e290 && pr >= 1.0e-290) {
+                const T inv = rcp_nr(pr) * c112;
+                plus = v[2] + (nP * dM) * inv;
+                minus = v[2] + (nM * dP) * inv;
+            } else {
+                plus = v[2] + nP / (dP * T(12.0));
+                minus = v[2] + nM / (dM * T(12.0));
+            }
         } else {
-            T q0 = frcp(Q0), q1 = frcp(Q1), q2 = frcp(Q2);
-            T aL0 = d0 * q0, aL1 = d1 * q1, aL2 = d2 * q2;
-            T aR0 = d0 * q2, aR1 = d1 * q1, aR2 = d2 * q0;
-            plus = (aL0 * pl0 + aL1 * pl1 + aL2 * pl2) * frcp(aL0 + aL1 + aL2);
-            minus = (aR0 * pr0 + aR1 * pr1 + aR2 * pr2) * frcp(aR0 + aR1 + aR2);
+            const T q0 = frcp(Q0), q1 = frcp(Q1), q2 = frcp(Q2);
+            const T aP0 = d0 * q0, a1 = d1 * q1, aP2 = d2 * q2;
+            const T aM0 = d2 * q0, aM2 = d0 * q2;
+            plus = v[2] + (aP0 * Gp0 + a1 * Gp1 + aP2 * Gp2) * (frcp(aP0 + a1 + aP2) * c112);
+            minus = v[2] + (aM0 * Gm0 + a1 * Gm1 + aM2 * Gm2) * (frcp(aM0 + a1 + aM2) * c112);
         }
     }
 };

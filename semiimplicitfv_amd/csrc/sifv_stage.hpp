@@ -477,8 +477,11 @@ struct MarchTuning<float> {
 // The two-phase kernels hold nine fields per window row; asking for two
 // waves per SIMD costs them a few spilled registers but doubles the latency
 // hiding of a one-wave schedule.
+#ifndef SIFV_STAGE_WAVES_2D
+#define SIFV_STAGE_WAVES_2D 3
+#endif
 template <int DIM, int NPH>
-constexpr int stage_min_waves() { return NPH > 0 ? 2 : 1; }
+constexpr int stage_min_waves() { return NPH > 0 ? 2 : (DIM == 2 ? SIFV_STAGE_WAVES_2D : 1); }
 
 // ===========================================================================
 // x + y kernel (1D/2D full stage; 3D x-y part on one plane)
