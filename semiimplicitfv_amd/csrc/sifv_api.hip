@@ -654,7 +654,7 @@ int run_jacobi(sifv_handle* h, const double* rho, const double* dt, double tol, 
                 Jt.pin = h->jac[it % 2];
                 Jt.pout = h->jac[(it + 1) % 2];
                 void* args[] = {&Jt};
-                rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_tile_kernel<double>), jacobi_tile_grid(h),
+                rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_tile_kernel<double, TILE_SWEEP64>), jacobi_tile_grid(h),
                                   dim3(64, kTileRows), args);
             } else {
                 J.iter = it;
@@ -717,6 +717,18 @@ int run_jacobi_mixed(sifv_handle* h, const double* rho, const double* dt, double
     M.rho32 = rho32;
     M.resid = h->resid;
     M.dt = dt;
+    JacobiTileArgs<double> Mt;
+    Mt.g = h->grid();
+    Mt.dim = h->dim;
+    Mt.rho = rho;
+    Mt.co = h->jac[2];
+    Mt.rhs = h->jac[3];
+    Mt.resid = h->resid;
+    Mt.tol = 0.0;
+    Mt.dt = dt;
+    Mt.pn = h->jacPn;
+    Mt.r32 = r32;
+    Mt.co32 = co32;
     Jacobi32Args S;
     S.g = h->grid();
     S.dim = h->dim;
@@ -740,12 +752,23 @@ int run_jacobi_mixed(sifv_handle* h, const double* rho, const double* dt, double
     int total = 0, slot = 0;
     *n = maxIter;
     while (true) {
-        M.pin = pin;
-        M.din = din;
-        M.pout = pout;
-        M.slot = slot;
-        void* args[] = {&M};
-        int rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_mix_pass_kernel), dim3(nb), dim3(256), args);
+        int rc;
+        if (tiled) {
+            Mt.pin = pin;
+            Mt.din = din;
+            Mt.pout = pout;
+            Mt.iter = slot;
+            void* args[] = {&Mt};
+            rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_tile_kernel<double, TILE_PASS>),
+                              jacobi_tile_grid(h), dim3(64, kTileRows), args);
+        } else {
+            M.pin = pin;
+            M.din = din;
+            M.pout = pout;
+            M.slot = slot;
+            void* args[] = {&M};
+            rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_mix_pass_kernel), dim3(nb), dim3(256), args);
+        }
         if (rc) return rc;
         ++total;
         HIPCHK(hipMemcpyAsync(h->hostResid + slot, h->resid + slot, sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -763,7 +786,7 @@ int run_jacobi_mixed(sifv_handle* h, const double* rho, const double* dt, double
                 St.pin = d;
                 St.pout = dbuf[q % 2];
                 void* sargs[] = {&St};
-                rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_tile_kernel<float>), jacobi_tile_grid(h),
+                rc = timed_launch(h, reinterpret_cast<const void*>(jacobi_tile_kernel<float, TILE_SWEEP32>), jacobi_tile_grid(h),
                                   dim3(64, kTileRows), sargs);
             } else {
                 S.din = d;

@@ -380,18 +380,20 @@ __global__ void __launch_bounds__(256) jacobi32_kernel(Jacobi32Args A) {
 }
 
 // ---- Tiled Jacobi sweeps (2D/3D) ----------------------------------------------------
-// The Jacobi sweeps above (the FP64 update with its max-norm, and the FP32 correction
-// sweep) organised for HBM: a block of 64 x kTileRows threads owns a tile of 64
-// columns x kTileRows rows and marches it through up to kTilePlanes planes of z.
-// The z-neighbours of p and rho stay in registers; each plane's p and rho tile with
-// a one-cell x/y halo is staged in LDS, double-buffered by plane parity (one barrier
-// per plane).  Lanes on the halo column i == nx / row j == ny load it; the tile's
-// outer halo comes from extra loads by lanes 0 / 63 and rows 0 / kTileRows-1.  p
-// goes through the scalar ghost rule, rho through its filled ghosts.  Per cell the
-// arithmetic is the elementwise kernels' expression for expression, so both forms
-// give identical bits.
+// The Jacobi kernels above (the FP64 update with its max-norm, the FP32 correction
+// sweep and the FP64 pass of the mixed solve) organised for HBM: a block of
+// 64 x kTileRows threads owns a tile of 64 columns x kTileRows rows and marches it
+// through up to kTilePlanes planes of z.  The z-neighbours of p and rho stay in
+// registers; each plane's p and rho tile with a one-cell x/y halo is staged in LDS,
+// double-buffered by plane parity (one barrier per plane).  Lanes on the halo
+// column i == nx / row j == ny load it; the tile's outer halo comes from extra
+// loads by lanes 0 / 63 and rows 0 / kTileRows-1.  p goes through the scalar ghost
+// rule, rho through its filled ghosts.  Per cell the arithmetic is the elementwise
+// kernels' expression for expression, so both forms give identical bits.
 constexpr int kTileRows = 4;
 constexpr int kTilePlanes = 16;
+
+enum TileMode : int { TILE_SWEEP64 = 0, TILE_SWEEP32 = 1, TILE_PASS = 2 };
 
 template <typename T>
 struct JacobiTileArgs {
@@ -401,11 +403,16 @@ struct JacobiTileArgs {
     const T* co;         // FP64: rho c^2 (times dt^2 here); FP32: dt^2 rho c^2
     const T* rhs;        // FP64: rhs; FP32: residual r32
     const T* pin;        // iterate; FP32: null on the first sweep (zero)
-    T* pout;
+    T* pout;             // sweep: the new iterate; pass: pin + din
     double* resid;       // FP64: resid[iter] accumulates max |pNew - p|
     int iter;
     double tol;
     const double* dt;    // FP64 only
+    // pass of the mixed solve (jacobi_mix_pass_kernel's outputs)
+    const float* din;    // correction from the FP32 sweeps (null: none)
+    double* pn;          // FP64 Jacobi update of pin + din
+    float* r32;          // residual rhs - A (pin + din)
+    float* co32;         // dt^2 rho c^2 in FP32
 };
 
 template <typename T>
@@ -414,11 +421,13 @@ __device__ __forceinline__ T tile_coef(T r, T dL, T hc) {
     else return __builtin_amdgcn_rcpf(fmaxf(r, 1e-14f) * dL * hc);
 }
 
-template <typename T>
+template <typename T, int MODE>
 __global__ void __launch_bounds__(64 * kTileRows) jacobi_tile_kernel(JacobiTileArgs<T> A) {
     constexpr bool F64 = sizeof(T) == 8;
+    constexpr bool PASS = MODE == TILE_PASS;
+    static_assert(F64 == (MODE != TILE_SWEEP32), "FP64 sweeps and passes, FP32 sweeps");
     constexpr int TY = kTileRows;
-    if constexpr (F64) {
+    if constexpr (MODE == TILE_SWEEP64) {
         // converged at an earlier iteration: nothing left to do
         if (A.iter > 0 && A.resid[A.iter - 1] < A.tol) return;
     }
@@ -461,6 +470,12 @@ __global__ void __launch_bounds__(64 * kTileRows) jacobi_tile_kernel(JacobiTileA
     const long hxP = (long)(map_axis_scalar(hi, g.nx, g.sbc[0], g.sbc[1]) - hi);
     const long hyP = (long)(map_axis_scalar(hj, g.ny, g.sbc[2], g.sbc[3]) - hj) * sy;
     const bool hasP = A.pin != nullptr;
+    const bool hasD = PASS && A.din != nullptr;
+    // p of the sweep: the iterate; of the pass: iterate + FP32 correction
+    auto ldp = [&](long q) -> T {
+        if constexpr (PASS) return hasD ? A.pin[q] + (double)A.din[q] : A.pin[q];
+        else return A.pin[q];
+    };
     auto zofs = [&](int k) { return (long)(map_axis_scalar(k, g.nz, g.sbc[4], g.sbc[5]) - k) * sz; };
     T hcx = 0, dLx = 0, dRx = 0, hcy = 0, dLy = 0, dRy = 0;
     if (active) {
@@ -483,19 +498,19 @@ __global__ void __launch_bounds__(64 * kTileRows) jacobi_tile_kernel(JacobiTileA
     T hpx = 0, hrx = 0, hpy = 0, hry = 0;
     if (loads) {
         rc = A.rho[id];
-        if (hasP) pc = A.pin[id + pofs];
+        if (hasP) pc = ldp(id + pofs);
         if (z) {
             rzm = A.rho[id - sz];
-            if (hasP) pzm = A.pin[id + pofs - sz + zofs(k0 - 1)];
+            if (hasP) pzm = ldp(id + pofs - sz + zofs(k0 - 1));
         }
     }
     if (xh) {
         hrx = A.rho[hx];
-        if (hasP) hpx = A.pin[hx + hxP];
+        if (hasP) hpx = ldp(hx + hxP);
     }
     if (yh) {
         hry = A.rho[hy];
-        if (hasP) hpy = A.pin[hy + hyP];
+        if (hasP) hpy = ldp(hy + hyP);
     }
     double mr = 0.0;
     for (int k = k0; k < k1; ++k) {
@@ -513,7 +528,7 @@ __global__ void __launch_bounds__(64 * kTileRows) jacobi_tile_kernel(JacobiTileA
         // operands of this plane not staged in LDS, and the next plane's halo
         if (loads && z) {
             rzp = A.rho[id + sz];
-            if (hasP) pzp = A.pin[id + pofs + sz + zofs(k + 1)];
+            if (hasP) pzp = ldp(id + pofs + sz + zofs(k + 1));
         }
         T q2 = 0, qs = 0;
         if (active) {
@@ -525,11 +540,11 @@ __global__ void __launch_bounds__(64 * kTileRows) jacobi_tile_kernel(JacobiTileA
         if (k + 1 < k1) {
             if (xh) {
                 nhrx = A.rho[hx + dk];
-                if (hasP) nhpx = A.pin[hx + dk + hxP];
+                if (hasP) nhpx = ldp(hx + dk + hxP);
             }
             if (yh) {
                 nhry = A.rho[hy + dk];
-                if (hasP) nhpy = A.pin[hy + dk + hyP];
+                if (hasP) nhpy = ldp(hy + dk + hyP);
             }
         }
         __syncthreads();
@@ -554,7 +569,16 @@ __global__ void __launch_bounds__(64 * kTileRows) jacobi_tile_kernel(JacobiTileA
                 off += cL * pzm + cR * pzp;
                 diag += cL + cR;
             }
-            if constexpr (F64) {
+            if constexpr (PASS) {
+                const double co = q2 * dt2;
+                const double den = 1.0 + co * diag;
+                const double pn = (qs + co * off) / den;
+                A.pout[id] = pc;
+                A.pn[id] = pn;
+                A.r32[id] = (float)(den * (pn - pc));
+                A.co32[id] = (float)co;
+                mr = dmax_ref(mr, fabs(pn - pc));
+            } else if constexpr (F64) {
                 const double co = q2 * dt2;
                 const double den = 1.0 + co * diag;
                 const double pn = (qs + co * off) / den;
