@@ -173,8 +173,8 @@ class SemiImplicitSolver : public GpuSolverBase {
 public:
     SemiImplicitSolver(const RectilinearMesh&, std::shared_ptr<RiemannSolver> riemannSolver,
                        std::shared_ptr<PressureSolver>, std::shared_ptr<EquationOfState> eos,
-                       std::shared_ptr<IGRSolver>, const SimulationConfig& config)
-        : GpuSolverBase(std::move(riemannSolver), std::move(eos), config, SIFV_FP64) {}
+                       std::shared_ptr<IGRSolver>, const SimulationConfig& config, int precision = SIFV_FP64)
+        : GpuSolverBase(std::move(riemannSolver), std::move(eos), config, precision) {}
 
     double step(const SimulationConfig& config, const RectilinearMesh& mesh, SolutionState& state,
                 double targetDt = -1.0) {
