@@ -480,14 +480,19 @@ struct MarchTuning<float> {
 #ifndef SIFV_STAGE_WAVES_2D
 #define SIFV_STAGE_WAVES_2D 3
 #endif
-template <int DIM, int NPH>
-constexpr int stage_min_waves() { return NPH > 0 ? 2 : (DIM == 2 ? SIFV_STAGE_WAVES_2D : 1); }
+#ifndef SIFV_STAGE_WAVES_2D_F
+#define SIFV_STAGE_WAVES_2D_F 4
+#endif
+template <int DIM, int NPH, typename T>
+constexpr int stage_min_waves() {
+    return NPH > 0 ? 2 : (DIM == 2 ? (sizeof(T) == 8 ? SIFV_STAGE_WAVES_2D : SIFV_STAGE_WAVES_2D_F) : 1);
+}
 
 // ===========================================================================
 // x + y kernel (1D/2D full stage; 3D x-y part on one plane)
 // ===========================================================================
 template <int DIM, int R, int RS, int NPH, bool SEMI, int MODE, typename T>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_waves<DIM, NPH>(), 8)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_waves<DIM, NPH, T>(), 8)))
 stage_xy_kernel(StageParams<T> P) {
     constexpr int NV = n_recon<DIM, NPH>();   // reconstructed variables
     constexpr int NF = n_fc<DIM, NPH>();      // face contribution length
