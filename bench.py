@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
-    ap.add_argument("--variants", default="mixed,ensemble,adjoint,bubble",
+    ap.add_argument("--variants", default="mixed,ensemble,adjoint,bubble,graphs",
                     help="comma list of the side workloads to run with an FP64 main run")
     ap.add_argument("--bubble-n", type=int, default=512, help="cells per side of the 3D bubble workload")
     ap.add_argument("--rows", type=int, default=0, help="rows marched per wave (0 = built-in choice)")
@@ -97,6 +97,33 @@ def run_gpu(n, steps, warmup, precision, rows=0):
     g.close()
     return {"seconds": el, "kernel_ms": kms, "launches": launches, "rho": rho, "finite": finite,
             "cells": n * n, "dt_sum": t_sim}
+
+
+def bench_graphs(sizes=(128, 256, 512), steps: int = 2000):
+    """Step replay from a captured HIP graph vs one enqueue per launch
+    (the library default), on small 2D vortex grids where per-step
+    submission cost would matter most."""
+    from semiimplicitfv_amd import ExplicitSolver, cases
+
+    out = {}
+    for n in sizes:
+        cfg, mesh, st, eos, rs = cases.isentropic_vortex_2d(n)
+        row = {}
+        for graphs in (True, False):
+            g = ExplicitSolver(mesh, rs, eos, None, cfg)
+            g.set_graphs(graphs)
+            g.upload(st)
+            g.advance(cfg, 20, 1e30, 0.0)
+            g.synchronize()
+            t0 = time.perf_counter()
+            g.advance(cfg, steps, 1e30, 0.0)
+            g.synchronize()
+            el = time.perf_counter() - t0
+            row["graph_us_per_step" if graphs else "launch_us_per_step"] = el / steps * 1e6
+            g.close()
+        row["speedup"] = row["launch_us_per_step"] / row["graph_us_per_step"]
+        out[f"vortex{n}"] = row
+    return out
 
 
 def bench_ensemble(ntubes: int = 4096):
@@ -433,7 +460,8 @@ def main():
             out[tag] = {"fp64": r64, "fp32_sweeps": r32}
         return out
 
-    for name, fn in (("ensemble", bench_ensemble), ("adjoint", bench_adjoint), ("bubble", bubble_pair)):
+    for name, fn in (("ensemble", bench_ensemble), ("adjoint", bench_adjoint), ("bubble", bubble_pair),
+                     ("graphs", bench_graphs)):
         if name in want:
             t0 = time.perf_counter()
             try:

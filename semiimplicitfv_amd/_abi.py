@@ -127,6 +127,8 @@ def _declare(lib):
         "sifv_timing_read": (i, [vp, P(d), P(C.c_long), i]),
         "sifv_synchronize": (i, [vp]),
         "sifv_set_tile": (i, [vp, i]),
+        "sifv_set_graphs": (i, [vp, i]),
+        "sifv_graph_replays": (i, [vp, P(C.c_long)]),
         "sifv_adjoint_forward": (i, [vp, P(Config_C), i, i]),
         "sifv_adjoint_backward": (i, [vp, vp, vp, P(d)]),
         "sifv_adjoint_backward_host": (i, [vp, P(d), P(d), P(d)]),

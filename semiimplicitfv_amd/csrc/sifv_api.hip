@@ -157,6 +157,16 @@ struct sifv_handle {
     bool primFresh = true;
     int rowsOverride = 0;
     Timing timing;
+    // sifv_explicit_advance: one RK step (dt finalize + stages) captured as a HIP
+    // graph and replayed per step (off by default: measured slower than plain
+    // launches, which the device-resident time loop already keeps back to back);
+    // rebuilt when a launch argument changes
+    bool useGraphs = false;
+    hipGraph_t stepGraph = nullptr;
+    hipGraphExec_t stepExec = nullptr;
+    double graphEnd = 0.0;
+    int graphCur = -1, graphRows = -1, graphNpart = -1;
+    long graphReplays = 0;
     // 3D: x+y part of the right-hand side (n_fc fields)
     void* part = nullptr;
     int nfc = 0;
@@ -1081,6 +1091,54 @@ int adj_stages(sifv_handle* h, const double* un, double* next) {
     return rc;
 }
 
+void drop_step_graph(sifv_handle* h) {
+    if (h->stepExec) hipGraphExecDestroy(h->stepExec);
+    if (h->stepGraph) hipGraphDestroy(h->stepGraph);
+    h->stepExec = nullptr;
+    h->stepGraph = nullptr;
+}
+
+// Whether a step of sifv_explicit_advance can be replayed from a graph: the
+// launch arguments of a step are then the same every step (RK2/RK3 keep the
+// state in buffer A; RK1 rotates it), no dt_cells pass is due, no per-launch
+// events are recorded, and IGR (whose warm start reads the step counter and
+// whose sweeps carry epochs) is off.
+bool step_graph_usable(const sifv_handle* h) {
+    return h->useGraphs && !h->timing.on && h->cfg.RKOrder >= 2 && !h->cfg.useIGR &&
+           (h->cfg.explicitParams.constDt > 0.0 || h->dtNextValid);
+}
+
+// Enqueues one explicit RK step of the device-resident time loop through the
+// step graph, capturing it first when none matches the current arguments.
+// The captured work is exactly enqueue_explicit_step's launches (no memset
+// or copy nodes).
+int launch_step_graph(sifv_handle* h, double endTime) {
+    if (!h->stepExec || h->graphEnd != endTime || h->graphCur != h->cur || h->graphRows != h->rowsOverride ||
+        h->graphNpart != h->npartialValid) {
+        drop_step_graph(h);
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+        int rc = enqueue_explicit_step(h, -1.0, true, endTime);
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(h->stream, &gr);
+        if (rc) {
+            if (gr) hipGraphDestroy(gr);
+            return rc;
+        }
+        HIPCHK(e);
+        h->stepGraph = gr;
+        HIPCHK(hipGraphInstantiate(&h->stepExec, gr, nullptr, nullptr, 0));
+        h->graphEnd = endTime;
+        h->graphCur = h->cur;
+        h->graphRows = h->rowsOverride;
+        h->graphNpart = h->npartialValid;
+    }
+    HIPCHK(hipGraphLaunch(h->stepExec, h->stream));
+    ++h->graphReplays;
+    h->dtNextValid = h->cfg.explicitParams.constDt <= 0.0;
+    h->primFresh = false;
+    return SIFV_OK;
+}
+
 int check_step_config(sifv_handle* h, const sifv_config* cfg) {
     if (cfg) {
         // per-call fields of SimulationConfig (step/time, and parameters the
@@ -1232,6 +1290,7 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
 int sifv_destroy(sifv_handle* h) {
     if (!h) return SIFV_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
+    drop_step_graph(h);
     for (auto& pr : h->timing.pending) {
         hipEventDestroy(pr.first);
         hipEventDestroy(pr.second);
@@ -1383,7 +1442,7 @@ int sifv_explicit_advance(sifv_handle* h, const sifv_config* cfg, int nsteps, do
     const int step0 = h->cfg.step;
     for (int s = 0; s < nsteps; ++s) {
         h->cfg.step = step0 + s;
-        int rc = enqueue_explicit_step(h, -1.0, true, endTime);
+        int rc = step_graph_usable(h) ? launch_step_graph(h, endTime) : enqueue_explicit_step(h, -1.0, true, endTime);
         if (rc) return rc;
     }
     h->cfg.step = step0;
@@ -1505,6 +1564,22 @@ int sifv_set_tile(sifv_handle* h, int rows_per_wave) {
     if (!h) return fail(SIFV_ERR_INVALID, "null handle");
     if (rows_per_wave < 0 || rows_per_wave > 4096) return fail(SIFV_ERR_INVALID, "rows_per_wave out of range");
     h->rowsOverride = rows_per_wave;
+    return SIFV_OK;
+}
+
+int sifv_set_graphs(sifv_handle* h, int on) {
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    h->useGraphs = on != 0;
+    if (!h->useGraphs) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        drop_step_graph(h);
+    }
+    return SIFV_OK;
+}
+
+int sifv_graph_replays(const sifv_handle* h, long* replays) {
+    if (!h || !replays) return fail(SIFV_ERR_INVALID, "null argument");
+    *replays = h->graphReplays;
     return SIFV_OK;
 }
 

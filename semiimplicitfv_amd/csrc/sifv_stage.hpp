@@ -32,6 +32,12 @@
 #ifndef SIFV_PREFETCH_ROW_D
 #define SIFV_PREFETCH_ROW_D true
 #endif
+#ifndef SIFV_PREFETCH_Z
+#define SIFV_PREFETCH_Z true
+#endif
+#ifndef SIFV_PRELOAD_PART_Z
+#define SIFV_PRELOAD_PART_Z false
+#endif
 #ifndef SIFV_PRELOAD_UPDATE_D
 #define SIFV_PRELOAD_UPDATE_D false
 #endif
@@ -773,14 +779,31 @@ __global__ void __launch_bounds__(64) stage_z_kernel(StageParams<T> P) {
         load_cell<DIM, NPH>(P, P.src, cc, j, kk, win[w]);
         sigwin[w] = useSig ? (T)load_sigma<DIM>(P, cc, j, kk) : T(0);
     }
+    // raw values of the plane entering the window next iteration, and the
+    // x+y part of the current cell, are loaded one step ahead of their use
+    constexpr bool PF = SIFV_PREFETCH_Z;
+    constexpr bool PP = SIFV_PRELOAD_PART_Z;
+    RawCell<DIM, NPH, T> nextRow;
+    if (PF) load_raw<DIM, NPH, T>(P, P.src, cc, j, rStart + R, nextRow);
     for (int r = rStart; r < Z1; ++r) {
 #pragma unroll
         for (int w = 0; w < WS - 1; ++w) {
             win[w] = win[w + 1];
             sigwin[w] = sigwin[w + 1];
         }
-        load_cell<DIM, NPH>(P, P.src, cc, j, r + R, win[WS - 1]);
+        if constexpr (PF) {
+            raw_to_cell<DIM, NPH, T>(P, nextRow, win[WS - 1]);
+            if (r + 1 < Z1) load_raw<DIM, NPH, T>(P, P.src, cc, j, r + 1 + R, nextRow);
+        } else {
+            load_cell<DIM, NPH>(P, P.src, cc, j, r + R, win[WS - 1]);
+        }
         sigwin[WS - 1] = useSig ? (T)load_sigma<DIM>(P, cc, j, r + R) : T(0);
+        T accPre[NF];
+        if (PP && r >= Z0 && owned) {
+            const size_t pid = cell_index(g, col, j, r);
+#pragma unroll
+            for (int q = 0; q < NF; ++q) accPre[q] = P.part[(size_t)q * P.total + pid];
+        }
         FaceState<DIM, NPH, T> mZ, pZ;
         recon_window<DIM, NPH, R, T>(P, win, sigwin, WST, useSig, eps, weno, mZ, pZ);
         T Hhi[NF];
@@ -798,7 +821,7 @@ __global__ void __launch_bounds__(64) stage_z_kernel(StageParams<T> P) {
             const size_t id = cell_index(g, col, j, r);
             T acc[NF];
 #pragma unroll
-            for (int q = 0; q < NF; ++q) acc[q] = P.part[(size_t)q * P.total + id];
+            for (int q = 0; q < NF; ++q) acc[q] = PP ? accPre[q] : P.part[(size_t)q * P.total + id];
             accumulate<DIM, NPH, T>(acc, Hlo, Hhi, (T)cz, false);
             const double h[3] = {dxc, dyj, dzr};
             double dc = finish_cell<DIM, NPH, SEMI, T>(P, id, acc, win[WROW], h, dt, P.dtPartial != nullptr);

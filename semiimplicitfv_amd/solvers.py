@@ -103,6 +103,16 @@ class _GpuSolverBase:
     def set_tile(self, rows: int):
         _abi.check(self._lib.sifv_set_tile(self._h, rows))
 
+    def set_graphs(self, on: bool = True):
+        """advance() replays each RK step from a captured HIP graph (on=True)
+        or enqueues every launch (on=False, the default)."""
+        _abi.check(self._lib.sifv_set_graphs(self._h, int(on)))
+
+    def graph_replays(self) -> int:
+        n = C.c_long()
+        _abi.check(self._lib.sifv_graph_replays(self._h, C.byref(n)))
+        return n.value
+
     def timing(self, on: bool = True):
         _abi.check(self._lib.sifv_timing_enable(self._h, int(on)))
 
