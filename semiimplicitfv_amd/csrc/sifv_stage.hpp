@@ -33,7 +33,7 @@
 #define SIFV_PREFETCH_ROW_D true
 #endif
 #ifndef SIFV_PREFETCH_Z
-#define SIFV_PREFETCH_Z true
+#define SIFV_PREFETCH_Z false
 #endif
 #ifndef SIFV_PRELOAD_PART_Z
 #define SIFV_PRELOAD_PART_Z false
