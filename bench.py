@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
-    ap.add_argument("--variants", default="mixed,ensemble,adjoint,bubble,graphs",
+    ap.add_argument("--variants", default="mixed,ensemble,adjoint,bubble,graphs,igr",
                     help="comma list of the side workloads to run with an FP64 main run")
     ap.add_argument("--bubble-n", type=int, default=512, help="cells per side of the 3D bubble workload")
     ap.add_argument("--rows", type=int, default=0, help="rows marched per wave (0 = built-in choice)")
@@ -122,6 +122,42 @@ def bench_graphs(sizes=(128, 256, 512), steps: int = 2000):
             row["graph_us_per_step" if graphs else "launch_us_per_step"] = el / steps * 1e6
             g.close()
         row["speedup"] = row["launch_us_per_step"] / row["graph_us_per_step"]
+        out[f"vortex{n}"] = row
+    return out
+
+
+def bench_igr(sizes=(1024, 2048), steps: int = 20):
+    """2D vortex with IGR (LF + WENO5 + SSP-RK3, IGRIters 5 exact
+    Gauss-Seidel sweeps per stage, alphaCoeff 1) against the same run
+    without IGR: wall time per step after a warm-up step (whose 50 warm-start
+    sweeps are excluded)."""
+    from semiimplicitfv_amd import ExplicitSolver, cases
+
+    out = {}
+    for n in sizes:
+        row = {}
+        for igr in (False, True):
+            cfg, mesh, st, eos, rs = cases.isentropic_vortex_2d(n, riemann="lf")
+            if igr:
+                cfg.useIGR = True
+                cfg.igrParams.alphaCoeff = 1.0
+                cfg.igrParams.IGRIters = 5
+                cfg.igrParams.IGRWarmStartIters = 50
+                cfg.validate()
+            g = ExplicitSolver(mesh, rs, eos, None, cfg)
+            g.upload(st)
+            t, _ = g.advance(cfg, 1, 1e30, 0.0)
+            g.synchronize()
+            cfg.step = 1
+            t0 = time.perf_counter()
+            g.advance(cfg, steps, 1e30, t)
+            g.synchronize()
+            el = time.perf_counter() - t0
+            g.close()
+            row["igr_ms_per_step" if igr else "plain_ms_per_step"] = el / steps * 1e3
+        row["igr_sweeps_per_step"] = 15
+        row["ms_per_igr_sweep"] = (row["igr_ms_per_step"] - row["plain_ms_per_step"]) / 15
+        row["cell_sweeps_per_s"] = n * n / (row["ms_per_igr_sweep"] * 1e-3)
         out[f"vortex{n}"] = row
     return out
 
@@ -461,7 +497,7 @@ def main():
         return out
 
     for name, fn in (("ensemble", bench_ensemble), ("adjoint", bench_adjoint), ("bubble", bubble_pair),
-                     ("graphs", bench_graphs)):
+                     ("graphs", bench_graphs), ("igr", bench_igr)):
         if name in want:
             t0 = time.perf_counter()
             try:

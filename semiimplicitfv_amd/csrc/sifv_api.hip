@@ -183,6 +183,7 @@ struct sifv_handle {
     int* igrFlags = nullptr;   // [ntiles] tile epochs, [ntiles] error flag
     int igrNtx = 0, igrNty = 0, igrTj = 1, igrTiles = 0;
     int igrEpoch = 0;
+    int igrBlocks = 1;          // sweep grid: at most the co-resident block count
     bool sigPrim = false;
     bool unifY = false;       // all interior y (and z) widths are the same double
     bool jacMixed = false;    // semi-implicit pressure solve: FP32 sweeps + FP64 refinement
@@ -917,7 +918,7 @@ int setup_igr(sifv_handle* h) {
     const int ntx = (h->nx + kWave - 1) / kWave;
     long best = -1;
     int bestTj = 1;
-    for (int tj = 1; tj <= 256; tj *= 2) {
+    for (int tj = 1; tj <= kIgrTj; tj *= 2) {
         if (h->dim == 1 && tj > 1) break;
         const int nty = (h->ny + tj - 1) / tj;
         long cost = (long)(ntx + nty + h->nz - 2) * (tj + kWave - 1);
@@ -944,6 +945,14 @@ int setup_igr(sifv_handle* h) {
     HIPCHK(hipMalloc(&h->igrFlags, sizeof(int) * (h->igrTiles + 1)));
     HIPCHK(hipMemset(h->igrFlags, 0, sizeof(int) * (h->igrTiles + 1)));
     h->igrEpoch = 0;
+    // every sweep block must be co-resident (blocks wait on tiles of other blocks)
+    int perCu = 0, dev = 0, cus = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &perCu, reinterpret_cast<const void*>(igr_sweep_kernel_for(h->dim)), kWave, 0));
+    if (perCu < 1 || cus < 1) return fail(SIFV_ERR_HIP, "IGR sweep kernel cannot be resident");
+    h->igrBlocks = std::min(h->igrTiles, perCu * cus);
     return SIFV_OK;
 }
 
@@ -1000,7 +1009,7 @@ int run_igr(sifv_handle* h, int src, const int* active) {
     S.err = h->igrFlags + h->igrTiles;
     S.active = active;
     IgrSweepFn sweep = igr_sweep_kernel_for(h->dim);
-    const int nblk = std::min(h->igrTiles, 512);
+    const int nblk = h->igrBlocks;
     for (int it = 0; it < iters; ++it) {
         S.epoch = ++h->igrEpoch;
         void* sargs[] = {&S};

@@ -13,15 +13,23 @@
 //                  Cell (i,j,k) needs the new values of (i-1,j,k), (i,j-1,k),
 //                  (i,j,k-1) and the old values of the +1 neighbours, so the
 //                  sweep is a wavefront.  The domain is cut into tiles of 64
-//                  columns x tj rows of one plane; a wave owns a tile and
-//                  walks its anti-diagonals (lane l <-> column, row = step - l:
-//                  the left neighbour's new value comes from lane l-1's previous
-//                  step, the lower neighbour's from the lane's own previous
-//                  step).  Tiles start when their three upstream tiles have
-//                  published the current epoch in a flag array; every block
-//                  handles tiles in topological order, so the lowest unfinished
-//                  tile can always proceed.  Waits are bounded: a wait that
-//                  exceeds its budget raises an error flag and every wave exits.
+//                  columns x tj (<= kIgrTj) rows of one plane; a wave owns a
+//                  tile and walks its anti-diagonals (lane l <-> column, row =
+//                  step - l: the left neighbour's new value comes from lane
+//                  l-1's previous step, the lower neighbour's from the lane's
+//                  own previous step).  Before the walk the wave stages in LDS
+//                  every operand that does not depend on this sweep's new
+//                  values (face densities, the old +x/+y quotients, rhs,
+//                  diagonal) -- before it waits for its upstream tiles -- and,
+//                  after the wait, the new values of the left column, the row
+//                  below and the plane below, so the walk's dependent chain
+//                  touches LDS only.  Tiles start when their three upstream
+//                  tiles have published the current epoch in a flag array;
+//                  every block handles tiles in topological order and the grid
+//                  never exceeds the co-resident block count, so the lowest
+//                  unfinished tile can always proceed.  Waits are bounded: a
+//                  wait that exceeds its budget raises an error flag and every
+//                  wave exits.
 #pragma once
 
 #include "sifv_stage.hpp"
@@ -115,9 +123,28 @@ __device__ __forceinline__ bool igr_wait(const IgrSweepArgs& A, int t) {
     return true;
 }
 
+constexpr int kIgrTj = 16;   // most rows per sweep tile (LDS staging)
+// The walk multiplies by the staged reciprocals of the face densities and of
+// the diagonal instead of dividing (one rounding per operand more, within the
+// parity bars; the divisions sat on the dependent chain).
+#ifndef SIFV_IGR_RCP
+#define SIFV_IGR_RCP 1
+#endif
+
 template <int DIM>
 __global__ void __launch_bounds__(64) igr_sweep_kernel(IgrSweepArgs A) {
     if (A.active && *A.active == 0) return;
+    // per-cell operands of the tile, [row][lane]
+    __shared__ double sFl[kIgrTj][kWave];    // x face density to the left
+    __shared__ double sTx[kIgrTj][kWave];    // old sigma(+x) / right face density
+    __shared__ double sFly[DIM >= 2 ? kIgrTj : 1][kWave];
+    __shared__ double sTy[DIM >= 2 ? kIgrTj : 1][kWave];
+    __shared__ double sQz[DIM >= 3 ? kIgrTj : 1][kWave];   // both z quotients
+    __shared__ double sRhs[kIgrTj][kWave];
+    __shared__ double sDiag[kIgrTj][kWave];
+    __shared__ double sLeft[kIgrTj];         // new sigma of the column left of the tile
+    __shared__ double sBot[kWave];           // new sigma of the row below the tile
+    __shared__ double sD2y[kIgrTj];
     const Grid& g = A.g;
     const int lane = threadIdx.x;
     const size_t sy = (size_t)g.nxT, sz = (size_t)g.nxT * g.nyT;
@@ -128,11 +155,8 @@ __global__ void __launch_bounds__(64) igr_sweep_kernel(IgrSweepArgs A) {
         const int rest = t / A.ntx;
         const int J = rest % A.nty;
         const int K = rest / A.nty;
-        bool ok = igr_wait(A, I > 0 ? t - 1 : -1);
-        ok = ok && igr_wait(A, J > 0 ? t - A.ntx : -1);
-        ok = ok && igr_wait(A, K > 0 ? t - A.ntx * A.nty : -1);
-        if (!ok) return;
-        const int i = I * kWave + lane;
+        const int i0 = I * kWave;
+        const int i = i0 + lane;
         const int j0 = J * A.tj;
         const int nrows = (j0 + A.tj <= g.ny) ? A.tj : g.ny - j0;
         const bool colOk = i < g.nx;
@@ -140,37 +164,87 @@ __global__ void __launch_bounds__(64) igr_sweep_kernel(IgrSweepArgs A) {
         const double dx = g.dxw[ci + g.ngx];
         const double d2x = 1.0 / (dx * dx);
         const double d2z = DIM >= 3 ? 1.0 / (g.dzw[K + g.ngz] * g.dzw[K + g.ngz]) : 0.0;
+        // operands that this sweep's new values do not change: the +x / +y
+        // neighbours are in this tile or in tiles that wait for it
+#pragma unroll 4
+        for (int jj = 0; jj < kIgrTj; ++jj) {
+            if (jj < nrows && colOk) {
+                const size_t id = cell_index(g, i, j0 + jj, K);
+                const double ri = A.rho[id];
+#if SIFV_IGR_RCP
+                sFl[jj][lane] = 1.0 / (0.5 * (ri + A.rho[id - 1]));
+#else
+                sFl[jj][lane] = 0.5 * (ri + A.rho[id - 1]);
+#endif
+                sTx[jj][lane] = A.sg[id + 1] / (0.5 * (ri + A.rho[id + 1]));
+                if (DIM >= 2) {
+#if SIFV_IGR_RCP
+                    sFly[jj][lane] = 1.0 / (0.5 * (ri + A.rho[id - sy]));
+#else
+                    sFly[jj][lane] = 0.5 * (ri + A.rho[id - sy]);
+#endif
+                    sTy[jj][lane] = A.sg[id + sy] / (0.5 * (ri + A.rho[id + sy]));
+                }
+                sRhs[jj][lane] = A.rhs[id];
+#if SIFV_IGR_RCP
+                sDiag[jj][lane] = 1.0 / A.diag[id];
+#else
+                sDiag[jj][lane] = A.diag[id];
+#endif
+            }
+        }
+        if (DIM >= 2 && lane < nrows) {
+            const double hy = g.dyw[j0 + lane + g.ngy];
+            sD2y[lane] = 1.0 / (hy * hy);
+        }
+        bool ok = igr_wait(A, I > 0 ? t - 1 : -1);
+        ok = ok && igr_wait(A, J > 0 ? t - A.ntx : -1);
+        ok = ok && igr_wait(A, K > 0 ? t - A.ntx * A.nty : -1);
+        if (!ok) return;
+        // new values published by the upstream tiles (ghosts at domain edges)
+        if (lane < nrows) sLeft[lane] = A.sg[cell_index(g, i0 - 1, j0 + lane, K)];
+        if (DIM >= 2 && colOk) sBot[lane] = A.sg[cell_index(g, i, j0 - 1, K)];
+        if (DIM >= 3) {
+#pragma unroll 4
+            for (int jj = 0; jj < kIgrTj; ++jj) {
+                if (jj < nrows && colOk) {
+                    const size_t id = cell_index(g, i, j0 + jj, K);
+                    const double ri = A.rho[id];
+                    const double fl = 0.5 * (ri + A.rho[id - sz]), fr = 0.5 * (ri + A.rho[id + sz]);
+                    sQz[jj][lane] = A.sg[id - sz] / fl + A.sg[id + sz] / fr;
+                }
+            }
+        }
+        __syncthreads();
         double last = 0.0;
         for (int s = 0; s < nrows + kWave - 1; ++s) {
             const int jj = s - lane;
             const double fromLeft = __shfl_up(last, 1);
             if (colOk && jj >= 0 && jj < nrows) {
-                const int j = j0 + jj;
-                const size_t id = cell_index(g, i, j, K);
-                const double ri = A.rho[id];
                 double off = 0.0;
                 {
-                    const double sl = lane == 0 ? A.sg[id - 1] : fromLeft;
-                    const double sr = A.sg[id + 1];
-                    const double fl = 0.5 * (ri + A.rho[id - 1]), fr = 0.5 * (ri + A.rho[id + 1]);
-                    off += alpha * d2x * (sl / fl + sr / fr);
+                    const double sl = lane == 0 ? sLeft[jj] : fromLeft;
+#if SIFV_IGR_RCP
+                    off += alpha * d2x * (sl * sFl[jj][lane] + sTx[jj][lane]);
+#else
+                    off += alpha * d2x * (sl / sFl[jj][lane] + sTx[jj][lane]);
+#endif
                 }
                 if (DIM >= 2) {
-                    const double sl = jj == 0 ? A.sg[id - sy] : last;
-                    const double sr = A.sg[id + sy];
-                    const double fl = 0.5 * (ri + A.rho[id - sy]), fr = 0.5 * (ri + A.rho[id + sy]);
-                    const double hy = g.dyw[j + g.ngy];
-                    const double d2 = 1.0 / (hy * hy);
-                    off += alpha * d2 * (sl / fl + sr / fr);
+                    const double sl = jj == 0 ? sBot[lane] : last;
+#if SIFV_IGR_RCP
+                    off += alpha * sD2y[jj] * (sl * sFly[jj][lane] + sTy[jj][lane]);
+#else
+                    off += alpha * sD2y[jj] * (sl / sFly[jj][lane] + sTy[jj][lane]);
+#endif
                 }
-                if (DIM >= 3) {
-                    const double sl = A.sg[id - sz];
-                    const double sr = A.sg[id + sz];
-                    const double fl = 0.5 * (ri + A.rho[id - sz]), fr = 0.5 * (ri + A.rho[id + sz]);
-                    off += alpha * d2z * (sl / fl + sr / fr);
-                }
-                last = (A.rhs[id] + off) / A.diag[id];
-                A.sg[id] = last;
+                if (DIM >= 3) off += alpha * d2z * sQz[jj][lane];
+#if SIFV_IGR_RCP
+                last = (sRhs[jj][lane] + off) * sDiag[jj][lane];
+#else
+                last = (sRhs[jj][lane] + off) / sDiag[jj][lane];
+#endif
+                A.sg[cell_index(g, i, j0 + jj, K)] = last;
             }
         }
         // publish the tile: data before flag

@@ -486,12 +486,18 @@ struct MarchTuning<float> {
 #ifndef SIFV_STAGE_WAVES_2D
 #define SIFV_STAGE_WAVES_2D 3
 #endif
+#ifndef SIFV_STAGE_WAVES_NPH
+#define SIFV_STAGE_WAVES_NPH 2
+#endif
+#ifndef SIFV_STAGE_WAVES_Z
+#define SIFV_STAGE_WAVES_Z 1
+#endif
 #ifndef SIFV_STAGE_WAVES_2D_F
 #define SIFV_STAGE_WAVES_2D_F 4
 #endif
 template <int DIM, int NPH, typename T>
 constexpr int stage_min_waves() {
-    return NPH > 0 ? 2 : (DIM == 2 ? (sizeof(T) == 8 ? SIFV_STAGE_WAVES_2D : SIFV_STAGE_WAVES_2D_F) : 1);
+    return NPH > 0 ? SIFV_STAGE_WAVES_NPH : (DIM == 2 ? (sizeof(T) == 8 ? SIFV_STAGE_WAVES_2D : SIFV_STAGE_WAVES_2D_F) : 1);
 }
 
 // ===========================================================================
@@ -731,7 +737,8 @@ stage_xy_kernel(StageParams<T> P) {
 // 3D z pass: 64 columns per wave, marching in z with a register window
 // ===========================================================================
 template <int R, int RS, int NPH, bool SEMI, typename T>
-__global__ void __launch_bounds__(64) stage_z_kernel(StageParams<T> P) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SIFV_STAGE_WAVES_Z, 8)))
+stage_z_kernel(StageParams<T> P) {
     constexpr int DIM = 3;
     constexpr int NF = n_fc<DIM, NPH>();
     constexpr int NFL = DIM + 4 + NPH;
