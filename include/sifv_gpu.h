@@ -196,6 +196,21 @@ int sifv_set_tile(sifv_handle* h, int rows_per_wave);
  * counterpart in the reference (a launch-mechanism switch).
  * sifv_graph_replays: graph launches since the handle was created. */
 int sifv_set_graphs(sifv_handle* h, int on);
+
+/* ---- output of the resident state (VTK XML, ASCII) ------------------------- */
+/* VTKWriter::writeVTR (src/VTKWriter.cpp:21-192) of the handle's current state:
+ * downloads the fields it writes (Pressure, Sigma with IGR, TotalEnergy,
+ * Velocity, Momentum, Density or AlphaRho_k / Alpha_k) and writes them with
+ * the reference's layout and 15-digit formatting; rank >= 0 adds the Rank
+ * array as VTKSession::write does on that rank (src/VTKSession.cpp:23-31),
+ * rank < 0 is the serial form.  Parent directories are created. */
+int sifv_write_vtr(sifv_handle* h, const char* filename, int rank);
+/* VTKWriter::writePVTR (src/VTKWriter.cpp:194-243) for the one-piece
+ * decomposition whose piece file is piece_file. */
+int sifv_write_pvtr(const sifv_handle* h, const char* filename, const char* piece_file);
+/* VTKWriter::writePVD (src/VTKWriter.cpp:245-291): mode "w" (new
+ * collection), "a" (append a dataset at `time`), "close" (no-op). */
+int sifv_write_pvd(const char* filename, const char* mode, double time, const char* data_file);
 int sifv_graph_replays(const sifv_handle* h, long* replays);
 
 /* ---- discrete adjoint ----------------------------------------------------- */

@@ -83,6 +83,12 @@ def _lib(impl: str):
         lib.ref_jacobi.argtypes = [vp, pd, pd, pd, pd, d, d, i]
         lib.ref_recon_line.restype = i
         lib.ref_recon_line.argtypes = [i, d, i, i, pd, pd, pd]
+        lib.ref_write_vtr.restype = i
+        lib.ref_write_vtr.argtypes = [vp, ppd, C.c_char_p, i]
+        lib.ref_write_pvtr.restype = i
+        lib.ref_write_pvtr.argtypes = [vp, C.c_char_p, C.c_char_p]
+        lib.ref_write_pvd.restype = i
+        lib.ref_write_pvd.argtypes = [C.c_char_p, C.c_char_p, d, C.c_char_p]
     _libs[impl] = (lib, pre)
     return _libs[impl]
 
@@ -164,6 +170,23 @@ class CpuSolver:
         f, keep = _fieldptrs(state)
         cc = config.to_c()
         self._fn("igr")(self.h, f, C.byref(cc))
+
+    # ---- the compiled reference's VTK writer (VTKWriter, src/VTKWriter.cpp) ----
+    def write_vtr(self, state, filename: str, rank: int = 0):
+        assert self.impl == "ref"
+        f, keep = _fieldptrs(state)
+        if self.lib.ref_write_vtr(self.h, f, filename.encode(), int(rank)) != 0:
+            raise RuntimeError(self.lib.ref_last_error().decode())
+
+    def write_pvtr(self, filename: str, piece_file: str):
+        assert self.impl == "ref"
+        if self.lib.ref_write_pvtr(self.h, filename.encode(), piece_file.encode()) != 0:
+            raise RuntimeError(self.lib.ref_last_error().decode())
+
+    def write_pvd(self, filename: str, mode: str, time: float = 0.0, data_file: str = ""):
+        assert self.impl == "ref"
+        if self.lib.ref_write_pvd(filename.encode(), mode.encode(), float(time), data_file.encode()) != 0:
+            raise RuntimeError(self.lib.ref_last_error().decode())
 
     def jacobi(self, rho, rhoc2, rhs, p, dt, tol, maxIter, hist=None):
         if self.impl == "c":

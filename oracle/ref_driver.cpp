@@ -29,6 +29,7 @@
 #include "SimulationConfig.hpp"
 #include "SolutionState.hpp"
 #include "StiffenedGasEOS.hpp"
+#include "VTKWriter.hpp"
 
 #include <mpi.h>
 
@@ -364,6 +365,48 @@ void ref_riemann_flux(int type, const double* Lp, const double* Rp, int dir, int
     for (int q = 0; q < 8; ++q) out[5 + q] = F.alphaFlux[q];
     out[13] = F.pressureFlux;
     out[14] = F.faceVelocity;
+}
+
+// VTKWriter::writeVTR of the given fields with this rank's extent and rank
+// number, as VTKSession::write calls it (src/VTKSession.cpp:23-31); rank < 0:
+// the serial form (no extent, no Rank array).
+int ref_write_vtr(void* p, double* const* f, const char* filename, int rank) {
+    try {
+        auto* s = static_cast<RefSolver*>(p);
+        load(s, f);
+        if (rank >= 0)
+            VTKWriter::writeVTR(filename, *s->mesh, s->state, s->config, s->mpi->localExtent(), rank);
+        else
+            VTKWriter::writeVTR(filename, *s->mesh, s->state, s->config);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return 1;
+    }
+}
+
+// VTKWriter::writePVTR for the one-rank decomposition (src/VTKSession.cpp:33-49).
+int ref_write_pvtr(void* p, const char* filename, const char* pieceFile) {
+    try {
+        auto* s = static_cast<RefSolver*>(p);
+        const auto& e = s->mpi->localExtent();
+        VTKWriter::writePVTR(filename, e[1] - e[0], e[3] - e[2], e[5] - e[4], {e}, {std::string(pieceFile)},
+                             s->config);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return 1;
+    }
+}
+
+int ref_write_pvd(const char* filename, const char* mode, double time, const char* dataFile) {
+    try {
+        VTKWriter::writePVD(filename, mode, time, dataFile);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return 1;
+    }
 }
 
 }  // extern "C"

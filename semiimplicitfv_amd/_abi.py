@@ -129,6 +129,9 @@ def _declare(lib):
         "sifv_set_tile": (i, [vp, i]),
         "sifv_set_graphs": (i, [vp, i]),
         "sifv_graph_replays": (i, [vp, P(C.c_long)]),
+        "sifv_write_vtr": (i, [vp, C.c_char_p, i]),
+        "sifv_write_pvtr": (i, [vp, C.c_char_p, C.c_char_p]),
+        "sifv_write_pvd": (i, [C.c_char_p, C.c_char_p, d, C.c_char_p]),
         "sifv_adjoint_forward": (i, [vp, P(Config_C), i, i]),
         "sifv_adjoint_backward": (i, [vp, vp, vp, P(d)]),
         "sifv_adjoint_backward_host": (i, [vp, P(d), P(d), P(d)]),

@@ -12,6 +12,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
 #include <string>
 #include <vector>
 
@@ -156,6 +159,7 @@ struct sifv_handle {
     bool dtNextValid = false;
     bool primFresh = true;
     int rowsOverride = 0;
+    std::vector<double> nodes[3];   // physical node coordinates (VTK output)
     Timing timing;
     // sifv_explicit_advance: one RK step (dt finalize + stages) captured as a HIP
     // graph and replayed per step (off by default: measured slower than plain
@@ -1231,6 +1235,9 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
         }
     }
     static const double unit[2] = {0.0, 1.0};
+    h->nodes[0].assign(mesh->xNodes, mesh->xNodes + h->nx + 1);
+    h->nodes[1] = h->dim >= 2 ? std::vector<double>(mesh->yNodes, mesh->yNodes + h->ny + 1) : std::vector<double>(unit, unit + 2);
+    h->nodes[2] = h->dim >= 3 ? std::vector<double>(mesh->zNodes, mesh->zNodes + h->nz + 1) : std::vector<double>(unit, unit + 2);
     std::vector<double> xe = extended_nodes(mesh->xNodes, h->nx, h->ngx);
     std::vector<double> ye = extended_nodes(h->dim >= 2 ? mesh->yNodes : unit, h->ny, h->ngy);
     std::vector<double> ze = extended_nodes(h->dim >= 3 ? mesh->zNodes : unit, h->nz, h->ngz);
@@ -1582,6 +1589,204 @@ int sifv_set_graphs(sifv_handle* h, int on) {
     if (!h->useGraphs) {
         HIPCHK(hipStreamSynchronize(h->stream));
         drop_step_graph(h);
+    }
+    return SIFV_OK;
+}
+
+// ---- VTK output of the resident state -------------------------------------
+// The reference's ASCII XML writer (VTKWriter::writeVTR / writePVTR / writePVD,
+// src/VTKWriter.cpp:21-293) as VTKSession drives it on one rank
+// (src/VTKSession.cpp:23-56): the same arrays in the same order, 15
+// significant digits through the same stream formatting.
+
+namespace {
+
+void vtk_parent_dir(const std::string& path) {
+    auto parent = std::filesystem::path(path).parent_path();
+    if (!parent.empty()) std::filesystem::create_directories(parent);
+}
+
+}  // namespace
+
+int sifv_write_vtr(sifv_handle* h, const char* filename, int rank) {
+    if (!h || !filename) return fail(SIFV_ERR_INVALID, "null argument");
+    const size_t n = h->total;
+    const int nph = h->nph, dim = h->dim;
+    const bool igr = h->cfg.useIGR != 0;
+    auto get = [&](int fid, std::vector<double>& v) {
+        v.resize(n);
+        return sifv_download(h, fid, v.data(), n);
+    };
+    std::vector<double> pres, sigma, rhoE, rho, vel[3], mom[3];
+    std::vector<std::vector<double>> ar(nph), al(nph);
+    int rc = get(SIFV_F_PRES, pres);
+    if (!rc && igr) rc = get(SIFV_F_SIGMA, sigma);
+    if (!rc) rc = get(SIFV_F_RHOE, rhoE);
+    for (int d = 0; d < dim && !rc; ++d) {
+        rc = get(SIFV_F_VELU + d, vel[d]);
+        if (!rc) rc = get(SIFV_F_RHOU + d, mom[d]);
+    }
+    if (!rc && nph == 0) rc = get(SIFV_F_RHO, rho);
+    for (int ph = 0; ph < nph && !rc; ++ph) {
+        rc = get(SIFV_F_ALPHARHO + ph, ar[ph]);
+        if (!rc) rc = get(SIFV_F_ALPHA + ph, al[ph]);
+    }
+    if (rc) return rc;
+    try {
+        vtk_parent_dir(filename);
+        std::ofstream file(filename);
+        if (!file.is_open()) return fail(SIFV_ERR_INVALID, std::string("cannot open ") + filename);
+        const int nx = h->nx, ny = h->ny, nz = h->nz;
+        auto at = [&](int i, int j, int k) {
+            return (size_t)(i + h->ngx) + (size_t)h->nxT * ((size_t)(j + h->ngy) + (size_t)h->nyT * (size_t)(k + h->ngz));
+        };
+        const std::vector<double>& X = h->nodes[0];
+        const std::vector<double>& Y = h->nodes[1];
+        const std::vector<double>& Z = h->nodes[2];
+        file << std::setprecision(15);
+        file << "<?xml version=\"1.0\"?>\n";
+        file << "<VTKFile type=\"RectilinearGrid\" version=\"1.0\" byte_order=\"LittleEndian\">\n";
+        file << "  <RectilinearGrid WholeExtent=\"0 " << nx << " 0 " << ny << " 0 " << nz << "\">\n";
+        file << "    <Piece Extent=\"0 " << nx << " 0 " << ny << " 0 " << nz << "\">\n";
+        file << "      <Coordinates>\n";
+        file << "        <DataArray type=\"Float64\" Name=\"X\" format=\"ascii\">\n         ";
+        for (int i = 0; i <= nx; ++i) file << " " << X[i];
+        file << "\n        </DataArray>\n";
+        file << "        <DataArray type=\"Float64\" Name=\"Y\" format=\"ascii\">\n         ";
+        const double dx0 = X[1] - X[0], dy0 = Y[1] - Y[0];
+        if (ny == 1) file << " 0.0 " << dx0;
+        else
+            for (int j = 0; j <= ny; ++j) file << " " << Y[j];
+        file << "\n        </DataArray>\n";
+        file << "        <DataArray type=\"Float64\" Name=\"Z\" format=\"ascii\">\n         ";
+        if (nz == 1) file << " 0.0 " << std::min(dx0, std::min(dy0, 1.0));
+        else
+            for (int k = 0; k <= nz; ++k) file << " " << Z[k];
+        file << "\n        </DataArray>\n";
+        file << "      </Coordinates>\n";
+        file << "      <CellData>\n";
+        auto scalar = [&](const std::string& name, const std::vector<double>& f) {
+            file << "        <DataArray type=\"Float64\" Name=\"" << name << "\" format=\"ascii\">\n";
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j) {
+                    file << "         ";
+                    for (int i = 0; i < nx; ++i) file << " " << f[at(i, j, k)];
+                    file << "\n";
+                }
+            file << "        </DataArray>\n";
+        };
+        auto vector = [&](const std::string& name, const std::vector<double>* f) {
+            file << "        <DataArray type=\"Float64\" Name=\"" << name
+                 << "\" NumberOfComponents=\"3\" format=\"ascii\">\n";
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j) {
+                    file << "         ";
+                    for (int i = 0; i < nx; ++i) {
+                        const size_t q = at(i, j, k);
+                        file << " " << f[0][q] << " " << (dim >= 2 ? f[1][q] : 0.0) << " "
+                             << (dim >= 3 ? f[2][q] : 0.0);
+                    }
+                    file << "\n";
+                }
+            file << "        </DataArray>\n";
+        };
+        scalar("Pressure", pres);
+        if (igr) scalar("Sigma", sigma);
+        scalar("TotalEnergy", rhoE);
+        vector("Velocity", vel);
+        vector("Momentum", mom);
+        if (nph == 0) scalar("Density", rho);
+        for (int ph = 0; ph < nph; ++ph) scalar("AlphaRho_" + std::to_string(ph), ar[ph]);
+        for (int ph = 0; ph < nph; ++ph) scalar("Alpha_" + std::to_string(ph), al[ph]);
+        if (rank >= 0) {
+            file << "        <DataArray type=\"Int32\" Name=\"Rank\" format=\"ascii\">\n";
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j) {
+                    file << "         ";
+                    for (int i = 0; i < nx; ++i) file << " " << rank;
+                    file << "\n";
+                }
+            file << "        </DataArray>\n";
+        }
+        file << "      </CellData>\n";
+        file << "    </Piece>\n";
+        file << "  </RectilinearGrid>\n";
+        file << "</VTKFile>\n";
+        if (!file) return fail(SIFV_ERR_INVALID, std::string("write failed: ") + filename);
+    } catch (const std::exception& e) {
+        return fail(SIFV_ERR_INVALID, e.what());
+    }
+    return SIFV_OK;
+}
+
+int sifv_write_pvtr(const sifv_handle* h, const char* filename, const char* piece_file) {
+    if (!h || !filename || !piece_file) return fail(SIFV_ERR_INVALID, "null argument");
+    try {
+        vtk_parent_dir(filename);
+        std::ofstream file(filename);
+        if (!file.is_open()) return fail(SIFV_ERR_INVALID, std::string("cannot open ") + filename);
+        file << "<?xml version=\"1.0\"?>\n";
+        file << "<VTKFile type=\"PRectilinearGrid\" version=\"1.0\" byte_order=\"LittleEndian\">\n";
+        file << "  <PRectilinearGrid WholeExtent=\"0 " << h->nx << " 0 " << h->ny << " 0 " << h->nz
+             << "\" GhostLevel=\"0\">\n";
+        file << "    <PCoordinates>\n";
+        for (const char* c : {"X", "Y", "Z"}) file << "      <PDataArray type=\"Float64\" Name=\"" << c << "\"/>\n";
+        file << "    </PCoordinates>\n";
+        file << "    <PCellData>\n";
+        auto decl = [&](const std::string& name, bool vec) {
+            file << "      <PDataArray type=\"Float64\" Name=\"" << name << "\"" << (vec ? " NumberOfComponents=\"3\"" : "")
+                 << "/>\n";
+        };
+        decl("Pressure", false);
+        if (h->cfg.useIGR) decl("Sigma", false);
+        decl("TotalEnergy", false);
+        decl("Velocity", true);
+        decl("Momentum", true);
+        const int nph = h->cfg.multiPhaseParams.nPhases;
+        if (nph <= 0) decl("Density", false);
+        for (int ph = 0; ph < nph; ++ph) decl("AlphaRho_" + std::to_string(ph), false);
+        for (int ph = 0; ph < nph; ++ph) decl("Alpha_" + std::to_string(ph), false);
+        file << "      <PDataArray type=\"Int32\" Name=\"Rank\"/>\n";
+        file << "    </PCellData>\n";
+        file << "    <Piece Extent=\"0 " << h->nx << " 0 " << h->ny << " 0 " << h->nz << "\" Source=\"" << piece_file
+             << "\"/>\n";
+        file << "  </PRectilinearGrid>\n";
+        file << "</VTKFile>\n";
+        if (!file) return fail(SIFV_ERR_INVALID, std::string("write failed: ") + filename);
+    } catch (const std::exception& e) {
+        return fail(SIFV_ERR_INVALID, e.what());
+    }
+    return SIFV_OK;
+}
+
+int sifv_write_pvd(const char* filename, const char* mode, double time, const char* data_file) {
+    if (!filename || !mode) return fail(SIFV_ERR_INVALID, "null argument");
+    static const std::string footer = "  </Collection>\n</VTKFile>\n";
+    try {
+        vtk_parent_dir(filename);
+        const std::string m(mode);
+        if (m == "w") {
+            std::ofstream file(filename);
+            if (!file.is_open()) return fail(SIFV_ERR_INVALID, std::string("cannot open ") + filename);
+            file << "<?xml version=\"1.0\"?>\n";
+            file << "<VTKFile type=\"Collection\" version=\"1.0\" byte_order=\"LittleEndian\">\n";
+            file << "  <Collection>\n";
+            file << footer;
+        } else if (m == "a") {
+            if (!data_file) return fail(SIFV_ERR_INVALID, "null data file");
+            const auto size = std::filesystem::file_size(filename);
+            if (size < footer.size()) return fail(SIFV_ERR_INVALID, std::string("not a collection file: ") + filename);
+            std::filesystem::resize_file(filename, size - footer.size());
+            std::ofstream file(filename, std::ios::app);
+            if (!file.is_open()) return fail(SIFV_ERR_INVALID, std::string("cannot open ") + filename);
+            file << std::setprecision(15);
+            file << "    <DataSet timestep=\"" << time << "\" file=\"" << data_file << "\"/>\n";
+            file << footer;
+        } else if (m != "close") {
+            return fail(SIFV_ERR_INVALID, "mode must be w, a or close");
+        }
+    } catch (const std::exception& e) {
+        return fail(SIFV_ERR_INVALID, e.what());
     }
     return SIFV_OK;
 }
