@@ -74,8 +74,10 @@ enum {
 typedef struct sifv_phase_eos { double gamma, pInf; } sifv_phase_eos;
 
 /* Mirror of SimulationConfig and its *Params structs
- * (include/SimulationConfig.hpp:22-91).  Body force, viscosity and surface
- * tension are outside the GPU path and must be zero. */
+ * (include/SimulationConfig.hpp:22-91).  The body force is applied in the
+ * stage update of the explicit and semi-implicit solvers at the step's config
+ * time; viscosity and surface tension are outside the GPU path and must be
+ * zero (the ensemble and the adjoint also reject a body force). */
 typedef struct sifv_config {
     int dim, nGhost, RKOrder, useIGR, semiImplicit, reconOrder;
     double wenoEps;

@@ -86,6 +86,12 @@ SimulationConfig to_config(const sifv_config* c) {
         e.pInf = c->multiPhaseParams.phases[p].pInf;
         s.multiPhaseParams.phases.push_back(e);
     }
+    for (int q = 0; q < 3; ++q) {
+        s.bodyForceParams.a[q] = c->bodyForceParams.a[q];
+        s.bodyForceParams.b[q] = c->bodyForceParams.b[q];
+        s.bodyForceParams.c[q] = c->bodyForceParams.c[q];
+        s.bodyForceParams.d[q] = c->bodyForceParams.d[q];
+    }
     return s;
 }
 

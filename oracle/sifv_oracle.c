@@ -890,6 +890,32 @@ static void compute_rhs(or_solver* s, double* const* f, const sifv_config* c, in
                     size_t id = IDX(m, i, j, k);
                     s->rhsP[id] += f[SIFV_F_PRES][id] * s->divU[id];
                 }
+    /* body force: a + b cos(c t + d) per axis at the config time, momentum
+     * rho*a and energy rho*(u.a) (src/ExplicitSolver.cpp:498-521,
+     * src/SemiImplicitSolver.cpp:484-506; hasBodyForce,
+     * include/SimulationConfig.hpp:101-106) */
+    int bf = 0;
+    for (int q = 0; q < 3; ++q)
+        if (c->bodyForceParams.a[q] != 0.0 || c->bodyForceParams.b[q] != 0.0) bf = 1;
+    if (bf) {
+        double acc[3];
+        for (int q = 0; q < 3; ++q)
+            acc[q] = c->bodyForceParams.a[q] + c->bodyForceParams.b[q] * cos(c->bodyForceParams.c[q] * c->time +
+                                                                              c->bodyForceParams.d[q]);
+        for (int k = 0; k < m->nz; ++k)
+            for (int j = 0; j < m->ny; ++j)
+                for (int i = 0; i < m->nx; ++i) {
+                    size_t id = IDX(m, i, j, k);
+                    double rho = f[SIFV_F_RHO][id];
+                    s->rhs[SIFV_F_RHOU][id] += rho * acc[0];
+                    if (dim >= 2) s->rhs[SIFV_F_RHOV][id] += rho * acc[1];
+                    if (dim >= 3) s->rhs[SIFV_F_RHOW][id] += rho * acc[2];
+                    double work = f[SIFV_F_VELU][id] * acc[0];
+                    if (dim >= 2) work += f[SIFV_F_VELV][id] * acc[1];
+                    if (dim >= 3) work += f[SIFV_F_VELW][id] * acc[2];
+                    s->rhs[SIFV_F_RHOE][id] += rho * work;
+                }
+    }
 }
 
 /* ------------------------------------------------------------------ */

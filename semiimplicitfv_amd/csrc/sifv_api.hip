@@ -111,9 +111,7 @@ int validate_config(const sifv_config* c) {
     // terms outside the GPU path
     if (c->viscousMu > 0.0) return fail(SIFV_ERR_UNSUPPORTED, "viscous stress is not on the GPU path");
     if (c->surfaceTensionSigma > 0.0) return fail(SIFV_ERR_UNSUPPORTED, "surface tension is not on the GPU path");
-    for (int d = 0; d < 3; ++d)
-        if (c->bodyForceParams.a[d] != 0.0 || c->bodyForceParams.b[d] != 0.0)
-            return fail(SIFV_ERR_UNSUPPORTED, "body force is not on the GPU path");
+
     return SIFV_OK;
 }
 
@@ -303,6 +301,13 @@ StageKernels<float> stage_kernels<float>(const sifv_handle* h, bool semi) {
     return stage_kernels_fp32(h->dim, recon_radius(h->cfg.reconOrder), h->riemann);
 }
 
+// SimulationConfig::hasBodyForce (include/SimulationConfig.hpp:101-106)
+bool has_body_force(const sifv_config& c) {
+    for (int d = 0; d < 3; ++d)
+        if (c.bodyForceParams.a[d] != 0.0 || c.bodyForceParams.b[d] != 0.0) return true;
+    return false;
+}
+
 int timed_launch(sifv_handle* h, const void* fn, dim3 grid, dim3 block, void** args) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (h->timing.on) {
@@ -347,6 +352,8 @@ int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const d
     P.pres = h->pres;
     for (int d = 0; d < 3; ++d) P.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
     P.pAdv = h->jac[4];
+    P.bodyForce = has_body_force(h->cfg) ? 1 : 0;
+    P.accel = h->scal + 4;
     StageKernels<T> k = stage_kernels<T>(h, semi);
     if (!k.xy || (h->dim == 3 && !k.z)) return fail(SIFV_ERR_UNSUPPORTED, "no stage kernel for this configuration");
     const bool wantDt = last && !semi && h->cfg.explicitParams.constDt <= 0.0;
@@ -454,6 +461,14 @@ int launch_finalize(sifv_handle* h, double cfl, double maxDt, double minDt, doub
     F.clampMin = clampMin ? 1 : 0;
     F.loop = loop ? 1 : 0;
     F.endTime = endTime;
+    F.time0 = h->cfg.time;
+    F.bodyForce = has_body_force(h->cfg) ? 1 : 0;
+    for (int d = 0; d < 3; ++d) {
+        F.bfA[d] = h->cfg.bodyForceParams.a[d];
+        F.bfB[d] = h->cfg.bodyForceParams.b[d];
+        F.bfC[d] = h->cfg.bodyForceParams.c[d];
+        F.bfD[d] = h->cfg.bodyForceParams.d[d];
+    }
     F.scal = h->scal;
     F.flags = h->flags;
     hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(kRedThreads), 0, h->stream, F);
@@ -1050,6 +1065,9 @@ int adj_check(const sifv_handle* h) {
         return fail(SIFV_ERR_UNSUPPORTED, "the adjoint covers the single-phase ideal-gas FP64 explicit path");
     if (!(c.explicitParams.constDt > 0.0))
         return fail(SIFV_ERR_UNSUPPORTED, "the adjoint needs a fixed time step (explicitParams.constDt > 0)");
+    for (int d = 0; d < 3; ++d)
+        if (c.bodyForceParams.a[d] != 0.0 || c.bodyForceParams.b[d] != 0.0)
+            return fail(SIFV_ERR_UNSUPPORTED, "the adjoint covers the flux terms only (no body force)");
     return SIFV_OK;
 }
 
@@ -1280,8 +1298,8 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
     CK(hipMemset(h->pres, 0, sizeof(double) * h->total));
     CK(hipMalloc(&h->sigma, sizeof(double) * h->total));
     CK(hipMemset(h->sigma, 0, sizeof(double) * h->total));
-    CK(hipMalloc(&h->scal, sizeof(double) * 4));
-    CK(hipMemset(h->scal, 0, sizeof(double) * 4));
+    CK(hipMalloc(&h->scal, sizeof(double) * 8));
+    CK(hipMemset(h->scal, 0, sizeof(double) * 8));
     CK(hipMalloc(&h->flags, sizeof(int) * 4));
     CK(hipMemset(h->flags, 0, sizeof(int) * 4));
     CK(hipHostMalloc(&h->hostPinned, sizeof(double) * 4, hipHostMallocDefault));

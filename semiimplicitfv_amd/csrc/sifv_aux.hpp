@@ -98,7 +98,10 @@ struct DtFinalize {
     int clampMin;      // semi-implicit clamp
     int loop;          // time-loop mode
     double endTime;
-    double* scal;      // [0] dt, [1] time
+    double time0;      // config time of the step (not time-loop mode)
+    int bodyForce;     // hasBodyForce: accel = bfA + bfB cos(bfC t + bfD) per axis
+    double bfA[3], bfB[3], bfC[3], bfD[3];
+    double* scal;      // [0] dt, [1] time, [2] time at the start of the step, [4..6] accel
     int* flags;        // [0] active, [1] steps taken
 };
 
@@ -111,6 +114,12 @@ __global__ void __launch_bounds__(kRedThreads) dt_finalize_kernel(DtFinalize F) 
     m = block_min(m, sh);
     if (threadIdx.x != 0) return;
     double target = F.target;
+    // the config time the step's stages see (runTimeLoop sets config.time
+    // before calling the step, src/RKTimeStepping.cpp:265)
+    F.scal[2] = F.loop ? F.scal[1] : F.time0;
+    // body-force acceleration at that time (src/ExplicitSolver.cpp:500-503)
+    if (F.bodyForce)
+        for (int d = 0; d < 3; ++d) F.scal[4 + d] = F.bfA[d] + F.bfB[d] * cos(F.bfC[d] * F.scal[2] + F.bfD[d]);
     if (F.loop) {
         double time = F.scal[1];
         if (!(time < F.endTime)) {

@@ -102,6 +102,8 @@ struct StageParams {
     double* pres;         // semi-implicit: stage-start pressure (written)
     double* vel[3];       // semi-implicit: star velocity (written)
     double* pAdv;         // semi-implicit: advected pressure (written)
+    int bodyForce;        // hasBodyForce
+    const double* accel;  // device-resident body-force acceleration of the step
 };
 
 // Logical block id with all blocks of one XCD contiguous (blocks are
@@ -360,6 +362,19 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
 #pragma unroll
     for (int ph = 0; ph < NPH; ++ph) acc[DIM + 2 + NPH + ph] = acc[DIM + 2 + NPH + ph] + (T)cell.a[ph] * acc[QD];
     if constexpr (SEMI) acc[QP] = acc[QP] + (T)cell.p * acc[QD];
+    if (P.bodyForce) {
+        // body force on the stage state (src/ExplicitSolver.cpp:498-521,
+        // src/SemiImplicitSolver.cpp:484-506)
+        const double a[3] = {P.accel[0], P.accel[1], P.accel[2]};
+        const double rho = cell.rho;
+        double work = cell.u[0] * a[0];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            acc[1 + d] = (T)((double)acc[1 + d] + rho * a[d]);
+            if (d > 0) work += cell.u[d] * a[d];
+        }
+        acc[DIM + 1] = (T)((double)acc[DIM + 1] + rho * work);
+    }
     double un[NCV];
     double U0v[NCV];
     const bool use0 = P.c2 != 0.0;

@@ -22,8 +22,9 @@ from semiimplicitfv_amd import (BoundaryCondition, IdealGasEOS, ReconstructionOr
                                 SolutionState, StiffenedGasEOS, cases, uniform_mesh)
 from semiimplicitfv_amd.cases import _fill  # noqa: E402
 from semiimplicitfv_amd.timeloop import run_time_loop  # noqa: E402
-from tests.golden.problems import (bubble_3d, jacobi_problem, liquid_gas_tube, random_ghost_case,  # noqa: E402
-                                   ensemble_sample, igr_cases, recon_lines, riemann_pairs, two_phase_cases)
+from tests.golden.problems import (body_force_cases, bubble_3d, jacobi_problem, liquid_gas_tube,  # noqa: E402
+                                   random_ghost_case, ensemble_sample, igr_cases, recon_lines, riemann_pairs,
+                                   two_phase_cases)
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 
@@ -172,6 +173,30 @@ def g_igr():
     _run_cases(igr_cases(), "igr_ref.npz")
 
 
+def g_body_force():
+    """Body-force cases, config.time advanced per step (runTimeLoop,
+    src/RKTimeStepping.cpp:258-266)."""
+    out = {}
+    for key, (build, rs, nsteps, semi) in body_force_cases().items():
+        cfg, mesh, st, eos = build()
+        s = po.CpuSolver(mesh, cfg, rs, eos, "ref")
+        dts, its, t = [], [], 0.0
+        for k in range(nsteps):
+            cfg.step, cfg.time = k, t
+            if semi:
+                dt, it = s.semi_step(st, cfg)
+                its.append(it)
+            else:
+                dt = s.explicit_step(st, cfg)
+            dts.append(dt)
+            t += dt
+        out[f"{key}_dts"] = np.array(dts)
+        out[f"{key}_iters"] = np.array(its, dtype=np.int64)
+        for fid in st.field_ids():
+            out[f"{key}_f{fid}"] = np.ascontiguousarray(mesh.interior(st.field(fid)))
+    save("body_force_ref.npz", **out)
+
+
 def g_ensemble():
     """Shock tubes of the batched ensemble sample run to t = 0.1 one by one
     (runTimeLoop dt rule): step counts, final times and interior fields."""
@@ -196,7 +221,7 @@ def g_ensemble():
 
 GENERATORS = {"recon": g_recon, "riemann": g_riemann, "sod": g_sod, "vortex": g_vortex, "ghost": g_ghost,
               "jacobi": g_jacobi, "semi": g_semi, "liquid_gas": g_liquid_gas, "two_phase": g_two_phase,
-              "igr": g_igr, "ensemble": g_ensemble}
+              "igr": g_igr, "ensemble": g_ensemble, "body_force": g_body_force}
 
 if __name__ == "__main__":
     assert po.available("ref"), "build oracle/_ref first (make -C oracle)"

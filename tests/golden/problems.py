@@ -325,6 +325,37 @@ def two_phase_cases():
     return out
 
 
+def body_force_cases():
+    """Seeded cases with a body force accel(t) = a + b cos(c t + d) per axis,
+    run with config.time advanced by each step's dt as runTimeLoop does;
+    reference outputs in body_force_ref.npz:
+    key -> (builder, Riemann kind (0 LF, 1 Rusanov, 2 HLLC), steps, semi)."""
+    def force(cfg, a, b, c, d):
+        bf = cfg.bodyForceParams
+        bf.a, bf.b, bf.c, bf.d = list(a), list(b), list(c), list(d)
+        return cfg
+
+    walls = [BoundaryCondition.SlipWall] * 6
+
+    def e1():
+        cfg, mesh, st, eos = make_case(1, (90,), bcs=walls, seed=21)
+        return force(cfg, (2.0, 0, 0), (1.0, 0, 0), (7.0, 0, 0), (0.3, 0, 0)), mesh, st, eos
+
+    def e2():
+        cfg, mesh, st, eos = make_case(2, (30, 22), bcs=walls, seed=22, nonuniform=True)
+        return force(cfg, (0.3, -1.2, 0), (0.2, 0.4, 0), (5.0, 3.0, 0), (0.1, 0.7, 0)), mesh, st, eos
+
+    def e3():
+        cfg, mesh, st, eos = make_case(3, (11, 9, 8), bcs=walls, periods=(1, 0, 0), seed=23)
+        return force(cfg, (0.0, 0.5, -1.0), (0.5, 0.0, 0.3), (4.0, 0.0, 6.0), (0.0, 0.0, 1.1)), mesh, st, eos
+
+    def s2():
+        cfg, mesh, st, eos = make_two_phase_case(2, (24, 28), bcs=walls, seed=24, semi=True, p_jump=False)
+        return force(cfg, (0, -9.81, 0), (0, 0, 0), (0, 0, 0), (0, 0, 0)), mesh, st, eos
+
+    return {"e1": (e1, 2, 40, False), "e2": (e2, 2, 25, False), "e3": (e3, 1, 12, False), "s2": (s2, 2, 3, True)}
+
+
 def make_case(dim, n, bcs=None, periods=(0, 0, 0), recon=2, rk=3, seed=0, nonuniform=False, eos_kind="ideal",
               ng=None, cfl=0.5, shock=True):
     """A random but well-posed state: smooth background + optional jump."""

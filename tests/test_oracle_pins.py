@@ -345,6 +345,45 @@ def test_two_phase_golden(oracle_c, key):
         assert l1 <= 1e-11 and li <= 1e-10, (fid, l1, li)
 
 
+def _run_body_force(po, key, drop_force=False):
+    build, rs, nsteps, semi = problems.body_force_cases()[key]
+    cfg, mesh, st, eos = build()
+    if drop_force:
+        bf = cfg.bodyForceParams
+        bf.a, bf.b = [0.0] * 3, [0.0] * 3
+    s = po.CpuSolver(mesh, cfg, rs, eos, "c")
+    dts, its, t = [], [], 0.0
+    for k in range(nsteps):
+        cfg.step, cfg.time = k, t
+        if semi:
+            dt, it = s.semi_step(st, cfg)
+            its.append(it)
+        else:
+            dt = s.explicit_step(st, cfg)
+        dts.append(dt)
+        t += dt
+    return mesh, st, dts, its
+
+
+@pytest.mark.parametrize("key", list(problems.body_force_cases()))
+def test_body_force_golden(oracle_c, key):
+    """Body force a + b cos(c t + d) (explicit 1D-3D, semi-implicit two-phase
+    2D under gravity): oracle against the reference's own output, with the
+    config time advanced per step; the same run without the force is far
+    from the fixture, so the term is exercised."""
+    d = load("body_force_ref.npz")
+    mesh, st, dts, its = _run_body_force(oracle_c, key)
+    assert np.allclose(dts, d[f"{key}_dts"], rtol=1e-13, atol=0.0)
+    assert list(its) == list(d[f"{key}_iters"])
+    floor = 1e-10 if key.startswith("s") else 1e-12
+    for fid in st.field_ids():
+        l1, li = rel_errs(mesh.interior(st.field(fid)), d[f"{key}_f{fid}"])
+        assert l1 <= floor and li <= 100 * floor, (fid, l1, li)
+    mesh0, st0, _, _ = _run_body_force(oracle_c, key, drop_force=True)
+    far = max(rel_errs(mesh0.interior(st0.field(fid)), d[f"{key}_f{fid}"])[1] for fid in (1, 4))
+    assert far > 1e-6, far
+
+
 @pytest.mark.parametrize("key", list(problems.igr_cases()))
 def test_igr_golden(oracle_c, key):
     """IGR (lexicographic Gauss-Seidel entropic pressure, warm start at step
