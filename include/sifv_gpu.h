@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SIFV_ABI_VERSION 1
+#define SIFV_ABI_VERSION 2
 #define SIFV_MAX_PHASES 8
 
 /* ---- status codes ---------------------------------------------------- */
@@ -76,8 +76,9 @@ typedef struct sifv_phase_eos { double gamma, pInf; } sifv_phase_eos;
 /* Mirror of SimulationConfig and its *Params structs
  * (include/SimulationConfig.hpp:22-91).  The body force is applied in the
  * stage update of the explicit and semi-implicit solvers at the step's config
- * time; viscosity and surface tension are outside the GPU path and must be
- * zero (the ensemble and the adjoint also reject a body force). */
+ * time; the viscous stress in the explicit solver's stage update; surface
+ * tension is outside the GPU path and must be zero (the ensemble and the
+ * adjoint carry the flux terms only). */
 typedef struct sifv_config {
     int dim, nGhost, RKOrder, useIGR, semiImplicit, reconOrder;
     double wenoEps;
@@ -90,6 +91,9 @@ typedef struct sifv_config {
     struct { double a[3], b[3], c[3], d[3]; } bodyForceParams;
     double viscousMu;
     double surfaceTensionSigma;
+    /* ViscousParams::phaseMu (empty: viscousNPhaseMu = 0, the scalar mu) */
+    int viscousNPhaseMu;
+    double viscousPhaseMu[SIFV_MAX_PHASES];
 } sifv_config;
 
 /* Mirror of the RectilinearMesh constructor arguments plus the boundary

@@ -67,7 +67,10 @@ inline sifv_config to_c(const SimulationConfig& c) {
         o.bodyForceParams.c[d] = c.bodyForceParams.c[d];
         o.bodyForceParams.d[d] = c.bodyForceParams.d[d];
     }
-    o.viscousMu = c.hasViscosity() ? 1.0 : 0.0;
+    o.viscousMu = c.viscousParams.mu;
+    o.viscousNPhaseMu = (int)c.viscousParams.phaseMu.size();
+    for (int p = 0; p < o.viscousNPhaseMu && p < SIFV_MAX_PHASES; ++p)
+        o.viscousPhaseMu[p] = c.viscousParams.phaseMu[p];
     o.surfaceTensionSigma = c.surfaceTensionParams.sigma;
     return o;
 }

@@ -86,6 +86,9 @@ SimulationConfig to_config(const sifv_config* c) {
         e.pInf = c->multiPhaseParams.phases[p].pInf;
         s.multiPhaseParams.phases.push_back(e);
     }
+    s.viscousParams.mu = c->viscousMu;
+    s.viscousParams.phaseMu.assign(c->viscousPhaseMu, c->viscousPhaseMu + c->viscousNPhaseMu);
+    s.surfaceTensionParams.sigma = c->surfaceTensionSigma;
     for (int q = 0; q < 3; ++q) {
         s.bodyForceParams.a[q] = c->bodyForceParams.a[q];
         s.bodyForceParams.b[q] = c->bodyForceParams.b[q];

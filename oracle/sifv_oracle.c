@@ -794,6 +794,105 @@ void or_riemann_flux(int type, const double* Lp, const double* Rp, int dir, int 
 /* RHS: ExplicitSolver::computeRHS (src/ExplicitSolver.cpp:270-496) and   */
 /* SemiImplicitSolver::computeRHS (src/SemiImplicitSolver.cpp:238-482)    */
 /* ------------------------------------------------------------------ */
+/* viscous stress: addViscousFluxes (src/ViscousFlux.cpp:25-303).  For the
+ * face between cells L and R along `dir`: normal derivatives from L and R,
+ * transverse derivatives as the mean of the central differences in L and R,
+ * tau = mu (grad u + grad u^T) - 2/3 mu div(u) I on the face row, work
+ * tau . u_face; L gains +area/vol(L) * (tau, work), R gains -area/vol(R).
+ * Faces are visited in the reference's loop order (x, y, z; k, j, i).  Face
+ * viscosity: the scalar mu, or the mean of sum_k alpha_k mu_k of L and R. */
+static double visc_face_mu(double* const* f, const sifv_config* c, size_t L, size_t R) {
+    if (c->viscousNPhaseMu <= 0) return c->viscousMu;
+    double mL = 0.0, mR = 0.0;
+    for (int ph = 0; ph < c->viscousNPhaseMu && ph < c->multiPhaseParams.nPhases; ++ph) {
+        mL += f[SIFV_F_ALPHA + ph][L] * c->viscousPhaseMu[ph];
+        mR += f[SIFV_F_ALPHA + ph][R] * c->viscousPhaseMu[ph];
+    }
+    return 0.5 * (mL + mR);
+}
+
+static void add_viscous(or_solver* s, double* const* f, const sifv_config* c) {
+    or_mesh* m = &s->m;
+    const int dim = m->dim;
+    const int velId[3] = {SIFV_F_VELU, SIFV_F_VELV, SIFV_F_VELW};
+    const int momId[3] = {SIFV_F_RHOU, SIFV_F_RHOV, SIFV_F_RHOW};
+    for (int dir = 0; dir < dim; ++dir) {
+        const int ei = dir == 0, ej = dir == 1, ek = dir == 2;
+        for (int k = 0; k < m->nz + ek; ++k)
+            for (int j = 0; j < m->ny + ej; ++j)
+                for (int i = 0; i < m->nx + ei; ++i) {
+                    const int li = i - ei, lj = j - ej, lk = k - ek;
+                    const size_t L = IDX(m, li, lj, lk), R = IDX(m, i, j, k);
+                    /* grad[a][b] = d u_a / d x_b on the face */
+                    double grad[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                    double hn = dir == 0 ? 0.5 * (DX(m, li) + DX(m, i))
+                                         : dir == 1 ? 0.5 * (DY(m, lj) + DY(m, j)) : 0.5 * (DZ(m, lk) + DZ(m, k));
+                    for (int a = 0; a < dim; ++a) grad[a][dir] = (f[velId[a]][R] - f[velId[a]][L]) / hn;
+                    for (int b = 0; b < dim; ++b) {
+                        if (b == dir) continue;
+                        const int bi = b == 0, bj = b == 1, bk = b == 2;
+                        /* transverse spacing of the face's row: the R cell's width along b */
+                        double hb = b == 0 ? DX(m, i) : b == 1 ? DY(m, j) : DZ(m, k);
+                        const size_t Lm = IDX(m, li - bi, lj - bj, lk - bk), Lp = IDX(m, li + bi, lj + bj, lk + bk);
+                        const size_t Rm = IDX(m, i - bi, j - bj, k - bk), Rp = IDX(m, i + bi, j + bj, k + bk);
+                        for (int a = 0; a < dim; ++a) {
+                            /* the reference forms only the components its stress needs */
+                            if (!(a == b || a == dir)) continue;
+                            double dL = (f[velId[a]][Lp] - f[velId[a]][Lm]) / (2.0 * hb);
+                            double dR = (f[velId[a]][Rp] - f[velId[a]][Rm]) / (2.0 * hb);
+                            grad[a][b] = 0.5 * (dL + dR);
+                        }
+                    }
+                    double divU = grad[0][0];
+                    if (dim >= 2) divU += grad[1][1];
+                    if (dim >= 3) divU += grad[2][2];
+                    const double mu = visc_face_mu(f, c, L, R);
+                    double tau[3] = {0, 0, 0};
+                    for (int a = 0; a < 3; ++a) {
+                        if (a == dir) tau[a] = mu * (2.0 * grad[dir][dir] - (2.0 / 3.0) * divU);
+                        else tau[a] = mu * (grad[a][dir] + grad[dir][a]);
+                    }
+                    double uf[3] = {0, 0, 0};
+                    for (int a = 0; a < dim; ++a) uf[a] = 0.5 * (f[velId[a]][L] + f[velId[a]][R]);
+                    double work = tau[0] * uf[0] + tau[1] * uf[1] + tau[2] * uf[2];
+                    double area = dir == 0 ? DY(m, j) * DZ(m, k) : dir == 1 ? DX(m, i) * DZ(m, k) : DX(m, i) * DY(m, j);
+                    const int inL = dir == 0 ? i >= 1 : dir == 1 ? j >= 1 : k >= 1;
+                    const int inR = dir == 0 ? i < m->nx : dir == 1 ? j < m->ny : k < m->nz;
+                    if (inL) {
+                        double coeff = area / VOL(m, li, lj, lk);
+                        for (int a = 0; a < dim; ++a) s->rhs[momId[a]][L] += coeff * tau[a];
+                        s->rhs[SIFV_F_RHOE][L] += coeff * work;
+                    }
+                    if (inR) {
+                        double coeff = area / VOL(m, i, j, k);
+                        for (int a = 0; a < dim; ++a) s->rhs[momId[a]][R] -= coeff * tau[a];
+                        s->rhs[SIFV_F_RHOE][R] -= coeff * work;
+                    }
+                }
+    }
+}
+
+/* computeViscousDt (src/RKTimeStepping.cpp:161-188): scalar mu only */
+static double viscous_dt(or_solver* s, double* const* f, double mu, double cfl, double maxDt) {
+    or_mesh* m = &s->m;
+    int dim = m->dim;
+    double dt = maxDt;
+    for (int k = 0; k < m->nz; ++k)
+        for (int j = 0; j < m->ny; ++j)
+            for (int i = 0; i < m->nx; ++i) {
+                size_t id = IDX(m, i, j, k);
+                double h = DX(m, i);
+                if (dim >= 2) h = dmin(h, DY(m, j));
+                if (dim >= 3) h = dmin(h, DZ(m, k));
+                double nu = mu / dmax(f[SIFV_F_RHO][id], 1e-14);
+                double dtCell = h * h / (2.0 * dim * nu);
+                dt = dmin(dt, cfl * dtCell);
+            }
+    return dt;
+}
+
+static int has_viscosity(const sifv_config* c) { return c->viscousMu > 0.0 || c->viscousNPhaseMu > 0; }
+
 static void compute_rhs(or_solver* s, double* const* f, const sifv_config* c, int semi) {
     or_mesh* m = &s->m;
     int dim = m->dim;
@@ -916,6 +1015,7 @@ static void compute_rhs(or_solver* s, double* const* f, const sifv_config* c, in
                     s->rhs[SIFV_F_RHOE][id] += rho * work;
                 }
     }
+    if (has_viscosity(c)) add_viscous(s, f, c);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1096,7 +1196,12 @@ double or_explicit_step(or_solver* s, double* const* f, const sifv_config* c, do
     if (c->explicitParams.constDt > 0)
         dt = c->explicitParams.constDt;
     else
+    {
         dt = or_acoustic_dt(s, f, c, c->explicitParams.cfl, c->explicitParams.maxDt);
+        /* src/ExplicitSolver.cpp:78-81 */
+        if (has_viscosity(c))
+            dt = dmin(dt, viscous_dt(s, f, c->viscousMu, c->explicitParams.cfl, c->explicitParams.maxDt));
+    }
     if (targetDt > 0) dt = dmin(dt, targetDt);
     int ord = c->RKOrder;
     int nph = n_phases(c);

@@ -60,6 +60,7 @@ class Config_C(C.Structure):
         ("explicitParams", _Explicit), ("semiImplicitParams", _Semi), ("igrParams", _IGR),
         ("multiPhaseParams", _Multi), ("bodyForceParams", _Body),
         ("viscousMu", C.c_double), ("surfaceTensionSigma", C.c_double),
+        ("viscousNPhaseMu", C.c_int), ("viscousPhaseMu", C.c_double * MAX_PHASES),
     ]
 
 

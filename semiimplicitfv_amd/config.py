@@ -239,6 +239,12 @@ class SimulationConfig:
         for i in range(3):
             c.bodyForceParams.a[i], c.bodyForceParams.b[i] = bf.a[i], bf.b[i]
             c.bodyForceParams.c[i], c.bodyForceParams.d[i] = bf.c[i], bf.d[i]
-        c.viscousMu = self.viscousParams.mu if not self.viscousParams.phaseMu else max(self.viscousParams.phaseMu)
+        c.viscousMu = self.viscousParams.mu
+        pm = list(self.viscousParams.phaseMu)
+        if len(pm) > _abi.MAX_PHASES:
+            raise ValueError("viscousParams.phaseMu: too many phases")
+        c.viscousNPhaseMu = len(pm)
+        for i, m in enumerate(pm):
+            c.viscousPhaseMu[i] = m
         c.surfaceTensionSigma = self.surfaceTensionParams.sigma
         return c

@@ -108,8 +108,13 @@ int validate_config(const sifv_config* c) {
     }
     if (c->viscousMu < 0.0) return fail(SIFV_ERR_INVALID, "viscousParams.mu must be >= 0");
     if (c->surfaceTensionSigma < 0.0) return fail(SIFV_ERR_INVALID, "surfaceTensionParams.sigma must be >= 0");
+    if (c->viscousNPhaseMu < 0 || c->viscousNPhaseMu > SIFV_MAX_PHASES)
+        return fail(SIFV_ERR_INVALID, "viscousParams.phaseMu: bad phase count");
     // terms outside the GPU path
-    if (c->viscousMu > 0.0) return fail(SIFV_ERR_UNSUPPORTED, "viscous stress is not on the GPU path");
+    if ((c->viscousMu > 0.0 || c->viscousNPhaseMu > 0) && c->semiImplicit)
+        return fail(SIFV_ERR_UNSUPPORTED, "viscous stress is on the GPU path of the explicit solver only");
+    if (c->viscousNPhaseMu > 0 && c->viscousNPhaseMu < c->multiPhaseParams.nPhases)
+        return fail(SIFV_ERR_UNSUPPORTED, "viscousParams.phaseMu needs one value per phase of a multiphase state");
     if (c->surfaceTensionSigma > 0.0) return fail(SIFV_ERR_UNSUPPORTED, "surface tension is not on the GPU path");
 
     return SIFV_OK;
@@ -158,6 +163,7 @@ struct sifv_handle {
     bool primFresh = true;
     int rowsOverride = 0;
     std::vector<double> nodes[3];   // physical node coordinates (VTK output)
+    double* visc = nullptr;         // viscous right-hand side of the stage (dim+1 fields)
     Timing timing;
     // sifv_explicit_advance: one RK step (dt finalize + stages) captured as a HIP
     // graph and replayed per step (off by default: measured slower than plain
@@ -308,6 +314,9 @@ bool has_body_force(const sifv_config& c) {
     return false;
 }
 
+// SimulationConfig::hasViscosity (include/SimulationConfig.hpp:96-98)
+bool has_viscosity(const sifv_config& c) { return c.viscousMu > 0.0 || c.viscousNPhaseMu > 0; }
+
 int timed_launch(sifv_handle* h, const void* fn, dim3 grid, dim3 block, void** args) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (h->timing.on) {
@@ -354,6 +363,8 @@ int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const d
     P.pAdv = h->jac[4];
     P.bodyForce = has_body_force(h->cfg) ? 1 : 0;
     P.accel = h->scal + 4;
+    P.visc = (!semi && has_viscosity(h->cfg)) ? h->visc : nullptr;
+    P.viscMu = (!semi && h->cfg.viscousMu > 0.0) ? h->cfg.viscousMu : 0.0;
     StageKernels<T> k = stage_kernels<T>(h, semi);
     if (!k.xy || (h->dim == 3 && !k.z)) return fail(SIFV_ERR_UNSUPPORTED, "no stage kernel for this configuration");
     const bool wantDt = last && !semi && h->cfg.explicitParams.constDt <= 0.0;
@@ -437,6 +448,8 @@ int launch_dt_cells(sifv_handle* h, bool advective, const double* rho = nullptr)
         A.alpha[p] = h->nph ? h->consField<double>(h->cur, h->dim + 2 + h->nph + pp) : nullptr;
     }
     A.advective = advective ? 1 : 0;
+    // computeViscousDt joins the acoustic step (src/ExplicitSolver.cpp:78-81)
+    A.viscMu = (!advective && !h->cfg.semiImplicit && h->cfg.viscousMu > 0.0) ? h->cfg.viscousMu : 0.0;
     size_t n = (size_t)h->nx * h->ny * h->nz;
     int nb = (int)std::min<size_t>((n + kRedThreads - 1) / kRedThreads, 2048);
     int rc = ensure_partials(h, nb);
@@ -565,6 +578,35 @@ int fill_dirs(sifv_handle* h, double** f, const int* comp, int nf, bool scalar, 
 
 // Enqueue one explicit step (dt finalize + RK stages).  loop: time-loop mode
 // with device-resident time and the active flag.
+int elementwise_blocks(const sifv_handle* h);
+
+// Viscous right-hand side of the stage whose input is conservative buffer
+// `src` (visc_kernel), into h->visc.
+template <typename T>
+int launch_visc(sifv_handle* h, int src, const int* active) {
+    if (!h->visc) {
+        HIPCHK(hipMalloc(&h->visc, sizeof(double) * (h->dim + 1) * h->total));
+        HIPCHK(hipMemset(h->visc, 0, sizeof(double) * (h->dim + 1) * h->total));
+    }
+    ViscFn<T> fn = visc_kernel_for<T>(h->dim, h->nph);
+    if (!fn) return fail(SIFV_ERR_UNSUPPORTED, "no viscous kernel for this configuration");
+    ViscArgs<T> A;
+    A.P.g = h->grid();
+    A.P.eos = h->eosd();
+    A.P.mix = h->mix();
+    A.P.total = h->total;
+    A.P.src = h->consField<T>(src, 0);
+    A.P.active = active;
+    A.mu = h->cfg.viscousMu;
+    A.nPhaseMu = h->cfg.viscousNPhaseMu;
+    for (int p = 0; p < kMaxPhases; ++p) A.phaseMu[p] = p < A.nPhaseMu ? h->cfg.viscousPhaseMu[p] : 0.0;
+    A.out = h->visc;
+    void* args[] = {&A};
+    HIPCHK(hipLaunchKernel(reinterpret_cast<const void*>(fn), dim3(elementwise_blocks(h)), dim3(256), args, 0,
+                           h->stream));
+    return SIFV_OK;
+}
+
 int enqueue_explicit_step(sifv_handle* h, double targetDt, bool loop, double endTime) {
     const sifv_config& c = h->cfg;
     int rc;
@@ -582,6 +624,10 @@ int enqueue_explicit_step(sifv_handle* h, double targetDt, bool loop, double end
         bool last = s == ord - 1;
         int irc = run_igr_any(h, src, active);
         if (irc) return irc;
+        if (has_viscosity(c)) {
+            irc = h->precision == SIFV_MIXED ? launch_visc<float>(h, src, active) : launch_visc<double>(h, src, active);
+            if (irc) return irc;
+        }
         return h->precision == SIFV_MIXED
                    ? launch_stage<float>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active, false)
                    : launch_stage<double>(h, src, u0, dst, kRK[ord - 1][s], ord == 1, last, active, false);
@@ -1068,6 +1114,7 @@ int adj_check(const sifv_handle* h) {
     for (int d = 0; d < 3; ++d)
         if (c.bodyForceParams.a[d] != 0.0 || c.bodyForceParams.b[d] != 0.0)
             return fail(SIFV_ERR_UNSUPPORTED, "the adjoint covers the flux terms only (no body force)");
+    if (has_viscosity(c)) return fail(SIFV_ERR_UNSUPPORTED, "the adjoint covers the flux terms only (no viscosity)");
     return SIFV_OK;
 }
 
@@ -1341,6 +1388,7 @@ int sifv_destroy(sifv_handle* h) {
     hipFree(h->scal);
     hipFree(h->flags);
     hipFree(h->part);
+    hipFree(h->visc);
     hipFree(h->jacPn);
     hipFree(h->jac32);
     for (double* c : h->adjCk) hipFree(c);

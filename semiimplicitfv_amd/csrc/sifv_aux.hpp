@@ -46,6 +46,7 @@ struct DtArgs {
     const double* alpha[kMaxPhases];
     const double* alphaRho[kMaxPhases];
     int advective;
+    double viscMu = 0.0;  // > 0: include the viscous limit (acoustic mode)
     double* partial;
 };
 
@@ -77,6 +78,7 @@ __global__ void __launch_bounds__(kRedThreads) dt_cells_kernel(DtArgs A) {
             }
             dc = h[0] / (fabs(A.vel[0][id]) + c);
             for (int d = 1; d < g.dim; ++d) dc = dmin_ref(dc, h[d] / (fabs(A.vel[d][id]) + c));
+            if (A.viscMu > 0.0) dc = dmin_ref(dc, visc_cell_dt(h, g.dim, A.viscMu, A.rho[id]));
         }
         m = dmin_ref(m, dc);
     }

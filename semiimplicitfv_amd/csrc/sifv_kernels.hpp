@@ -5,6 +5,7 @@
 
 #include "sifv_igr.hpp"
 #include "sifv_stage.hpp"
+#include "sifv_visc.hpp"
 
 namespace sifv {
 
@@ -38,5 +39,15 @@ IgrPreFn<double> igr_pre_kernel_for<double>(int dim, int nph);
 template <>
 IgrPreFn<float> igr_pre_kernel_for<float>(int dim, int nph);
 IgrSweepFn igr_sweep_kernel_for(int dim);
+
+// Viscous stress right-hand side per dimension / phase count.
+template <typename T>
+using ViscFn = void (*)(ViscArgs<T>);
+template <typename T>
+ViscFn<T> visc_kernel_for(int dim, int nph);
+template <>
+ViscFn<double> visc_kernel_for<double>(int dim, int nph);
+template <>
+ViscFn<float> visc_kernel_for<float>(int dim, int nph);
 
 }  // namespace sifv

@@ -102,9 +102,21 @@ struct StageParams {
     double* pres;         // semi-implicit: stage-start pressure (written)
     double* vel[3];       // semi-implicit: star velocity (written)
     double* pAdv;         // semi-implicit: advected pressure (written)
-    int bodyForce;        // hasBodyForce
-    const double* accel;  // device-resident body-force acceleration of the step
+    int bodyForce = 0;              // hasBodyForce
+    const double* accel = nullptr;  // device-resident body-force acceleration of the step
+    const double* visc = nullptr;   // viscous momentum/energy right-hand side (DIM+1 fields) or null
+    double viscMu = 0.0;            // > 0: the viscous limit joins the fused time-step reduction
 };
+
+// Viscous limit of one cell (computeViscousDt, src/RKTimeStepping.cpp:161-188),
+// without the cfl factor that the dt finalize applies.
+__device__ __forceinline__ double visc_cell_dt(const double* h, int dim, double mu, double rho) {
+    double hm = h[0];
+    if (dim >= 2) hm = dmin_ref(hm, h[1]);
+    if (dim >= 3) hm = dmin_ref(hm, h[2]);
+    const double nu = mu / dmax_ref(rho, 1e-14);
+    return hm * hm / (2.0 * dim * nu);
+}
 
 // Logical block id with all blocks of one XCD contiguous (blocks are
 // dispatched round-robin over the 8 XCDs of an MI355X), so neighbouring
@@ -375,6 +387,12 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
         }
         acc[DIM + 1] = (T)((double)acc[DIM + 1] + rho * work);
     }
+    if (P.visc) {
+        // viscous stress (src/ExplicitSolver.cpp:523-527)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) acc[1 + d] = (T)((double)acc[1 + d] + P.visc[(size_t)d * P.total + id]);
+        acc[DIM + 1] = (T)((double)acc[DIM + 1] + P.visc[(size_t)DIM * P.total + id]);
+    }
     double un[NCV];
     double U0v[NCV];
     const bool use0 = P.c2 != 0.0;
@@ -451,6 +469,7 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
     double dc = h[0] / (fabs(c.u[0]) + cs);
 #pragma unroll
     for (int d = 1; d < DIM; ++d) dc = dmin_ref(dc, h[d] / (fabs(c.u[d]) + cs));
+    if (P.viscMu > 0.0) dc = dmin_ref(dc, visc_cell_dt(h, DIM, P.viscMu, c.rho));
     return dc;
 }
 

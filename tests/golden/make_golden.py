@@ -24,7 +24,7 @@ from semiimplicitfv_amd.cases import _fill  # noqa: E402
 from semiimplicitfv_amd.timeloop import run_time_loop  # noqa: E402
 from tests.golden.problems import (body_force_cases, bubble_3d, jacobi_problem, liquid_gas_tube,  # noqa: E402
                                    random_ghost_case, ensemble_sample, igr_cases, recon_lines, riemann_pairs,
-                                   two_phase_cases)
+                                   two_phase_cases, viscous_cases)
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 
@@ -197,6 +197,24 @@ def g_body_force():
     save("body_force_ref.npz", **out)
 
 
+def g_viscous():
+    """Viscous cases, config.time advanced per step."""
+    out = {}
+    for key, (build, rs, nsteps, semi) in viscous_cases().items():
+        cfg, mesh, st, eos = build()
+        s = po.CpuSolver(mesh, cfg, rs, eos, "ref")
+        dts, t = [], 0.0
+        for k in range(nsteps):
+            cfg.step, cfg.time = k, t
+            dt = s.explicit_step(st, cfg)
+            dts.append(dt)
+            t += dt
+        out[f"{key}_dts"] = np.array(dts)
+        for fid in st.field_ids():
+            out[f"{key}_f{fid}"] = np.ascontiguousarray(mesh.interior(st.field(fid)))
+    save("viscous_ref.npz", **out)
+
+
 def g_ensemble():
     """Shock tubes of the batched ensemble sample run to t = 0.1 one by one
     (runTimeLoop dt rule): step counts, final times and interior fields."""
@@ -221,7 +239,7 @@ def g_ensemble():
 
 GENERATORS = {"recon": g_recon, "riemann": g_riemann, "sod": g_sod, "vortex": g_vortex, "ghost": g_ghost,
               "jacobi": g_jacobi, "semi": g_semi, "liquid_gas": g_liquid_gas, "two_phase": g_two_phase,
-              "igr": g_igr, "ensemble": g_ensemble, "body_force": g_body_force}
+              "igr": g_igr, "ensemble": g_ensemble, "body_force": g_body_force, "viscous": g_viscous}
 
 if __name__ == "__main__":
     assert po.available("ref"), "build oracle/_ref first (make -C oracle)"

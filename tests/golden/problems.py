@@ -356,6 +356,38 @@ def body_force_cases():
     return {"e1": (e1, 2, 40, False), "e2": (e2, 2, 25, False), "e3": (e3, 1, 12, False), "s2": (s2, 2, 3, True)}
 
 
+def viscous_cases():
+    """Seeded explicit cases with viscous stress (scalar mu, and per-phase mu
+    on a water/air state); reference outputs in viscous_ref.npz:
+    key -> (builder, Riemann kind (0 LF, 1 Rusanov, 2 HLLC), steps, semi)."""
+    NS = BoundaryCondition.NoSlipWall
+    SL = BoundaryCondition.SlipWall
+    OUT = BoundaryCondition.Outflow
+
+    def v1():
+        cfg, mesh, st, eos = make_case(1, (80,), bcs=[NS, OUT] + [SL] * 4, seed=31)
+        cfg.viscousParams.mu = 2e-3
+        return cfg, mesh, st, eos
+
+    def v2():
+        cfg, mesh, st, eos = make_case(2, (36, 24), bcs=[SL, SL, NS, NS, SL, SL], periods=(1, 0, 0), seed=32,
+                                       nonuniform=True)
+        cfg.viscousParams.mu = 5e-3
+        return cfg, mesh, st, eos
+
+    def v3():
+        cfg, mesh, st, eos = make_case(3, (10, 9, 8), bcs=[NS, SL, SL, NS, OUT, OUT], periods=(0, 0, 1), seed=33)
+        cfg.viscousParams.mu = 1e-2
+        return cfg, mesh, st, eos
+
+    def v2p():
+        cfg, mesh, st, eos = make_two_phase_case(2, (20, 16), bcs=[NS] * 6, seed=34)
+        cfg.viscousParams.phaseMu = [50.0, 1.0]
+        return cfg, mesh, st, eos
+
+    return {"v1": (v1, 2, 30, False), "v2": (v2, 2, 20, False), "v3": (v3, 1, 8, False), "v2p": (v2p, 2, 6, False)}
+
+
 def make_case(dim, n, bcs=None, periods=(0, 0, 0), recon=2, rk=3, seed=0, nonuniform=False, eos_kind="ideal",
               ng=None, cfl=0.5, shock=True):
     """A random but well-posed state: smooth background + optional jump."""

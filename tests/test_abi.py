@@ -15,7 +15,7 @@ def test_library_exports_every_declared_symbol():
     assert len(declared) >= 20
     missing = [s for s in declared if not hasattr(lib, s)]
     assert not missing, missing
-    assert lib.sifv_abi_version() == 1
+    assert lib.sifv_abi_version() == 2
 
 
 def _create(cfg, mesh=None):
@@ -48,7 +48,13 @@ def test_create_validates_like_the_reference(mutate, msg):
 
 
 def test_unsupported_terms_are_rejected():
-    cfg = SimulationConfig(dim=1, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5)
+    """Viscosity on the semi-implicit handle and surface tension are refused
+    before any device call."""
+    cfg = SimulationConfig(dim=1, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5, semiImplicit=True)
     cfg.viscousParams.mu = 1e-3
     rc, err = _create(cfg)
     assert rc == _abi.ERR_UNSUPPORTED and "viscous" in err
+    cfg = SimulationConfig(dim=1, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5)
+    cfg.surfaceTensionParams.sigma = 0.07
+    rc, err = _create(cfg)
+    assert rc == _abi.ERR_UNSUPPORTED and "surface tension" in err
