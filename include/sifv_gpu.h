@@ -76,7 +76,7 @@ typedef struct sifv_phase_eos { double gamma, pInf; } sifv_phase_eos;
 /* Mirror of SimulationConfig and its *Params structs
  * (include/SimulationConfig.hpp:22-91).  The body force is applied in the
  * stage update of the explicit and semi-implicit solvers at the step's config
- * time; the viscous stress in the explicit solver's stage update; surface
+ * time; the viscous stress in the stage update of both solvers; surface
  * tension is outside the GPU path and must be zero (the ensemble and the
  * adjoint carry the flux terms only). */
 typedef struct sifv_config {

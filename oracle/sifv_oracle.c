@@ -1326,6 +1326,9 @@ double or_semi_step(or_solver* s, double* const* f, const sifv_config* c, double
     const int mom[3] = {SIFV_F_RHOU, SIFV_F_RHOV, SIFV_F_RHOW};
     const int vel[3] = {SIFV_F_VELU, SIFV_F_VELV, SIFV_F_VELW};
     double dt = or_advective_dt(s, f, c->semiImplicitParams.cfl, c->semiImplicitParams.maxDt);
+    /* src/SemiImplicitSolver.cpp:112-115 */
+    if (has_viscosity(c))
+        dt = dmin(dt, viscous_dt(s, f, c->viscousMu, c->semiImplicitParams.cfl, c->semiImplicitParams.maxDt));
     if (targetDt > 0) dt = dmin(dt, targetDt);
     /* std::clamp(dt, minDt, maxDt) */
     if (dt < c->semiImplicitParams.minDt) dt = c->semiImplicitParams.minDt;

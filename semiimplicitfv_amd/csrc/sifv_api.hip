@@ -111,8 +111,6 @@ int validate_config(const sifv_config* c) {
     if (c->viscousNPhaseMu < 0 || c->viscousNPhaseMu > SIFV_MAX_PHASES)
         return fail(SIFV_ERR_INVALID, "viscousParams.phaseMu: bad phase count");
     // terms outside the GPU path
-    if ((c->viscousMu > 0.0 || c->viscousNPhaseMu > 0) && c->semiImplicit)
-        return fail(SIFV_ERR_UNSUPPORTED, "viscous stress is on the GPU path of the explicit solver only");
     if (c->viscousNPhaseMu > 0 && c->viscousNPhaseMu < c->multiPhaseParams.nPhases)
         return fail(SIFV_ERR_UNSUPPORTED, "viscousParams.phaseMu needs one value per phase of a multiphase state");
     if (c->surfaceTensionSigma > 0.0) return fail(SIFV_ERR_UNSUPPORTED, "surface tension is not on the GPU path");
@@ -363,7 +361,7 @@ int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const d
     P.pAdv = h->jac[4];
     P.bodyForce = has_body_force(h->cfg) ? 1 : 0;
     P.accel = h->scal + 4;
-    P.visc = (!semi && has_viscosity(h->cfg)) ? h->visc : nullptr;
+    P.visc = has_viscosity(h->cfg) ? h->visc : nullptr;
     P.viscMu = (!semi && h->cfg.viscousMu > 0.0) ? h->cfg.viscousMu : 0.0;
     StageKernels<T> k = stage_kernels<T>(h, semi);
     if (!k.xy || (h->dim == 3 && !k.z)) return fail(SIFV_ERR_UNSUPPORTED, "no stage kernel for this configuration");
@@ -434,7 +432,10 @@ const double kRK[3][3][4] = {
     {{1, 0, 1, 1}, {1, 3, 1, 4}, {2, 1, 2, 3}},
 };
 
-int launch_dt_cells(sifv_handle* h, bool advective, const double* rho = nullptr) {
+// withVisc: the step's dt, where computeViscousDt joins the acoustic
+// (src/ExplicitSolver.cpp:78-81) or advective (src/SemiImplicitSolver.cpp:112-115)
+// limit; the standalone dt queries leave it out.
+int launch_dt_cells(sifv_handle* h, bool advective, const double* rho = nullptr, bool withVisc = false) {
     DtArgs A;
     A.g = h->grid();
     A.eos = h->eosd();
@@ -448,8 +449,7 @@ int launch_dt_cells(sifv_handle* h, bool advective, const double* rho = nullptr)
         A.alpha[p] = h->nph ? h->consField<double>(h->cur, h->dim + 2 + h->nph + pp) : nullptr;
     }
     A.advective = advective ? 1 : 0;
-    // computeViscousDt joins the acoustic step (src/ExplicitSolver.cpp:78-81)
-    A.viscMu = (!advective && !h->cfg.semiImplicit && h->cfg.viscousMu > 0.0) ? h->cfg.viscousMu : 0.0;
+    A.viscMu = (withVisc && h->cfg.viscousMu > 0.0) ? h->cfg.viscousMu : 0.0;
     size_t n = (size_t)h->nx * h->ny * h->nz;
     int nb = (int)std::min<size_t>((n + kRedThreads - 1) / kRedThreads, 2048);
     int rc = ensure_partials(h, nb);
@@ -611,7 +611,7 @@ int enqueue_explicit_step(sifv_handle* h, double targetDt, bool loop, double end
     const sifv_config& c = h->cfg;
     int rc;
     if (c.explicitParams.constDt <= 0.0 && !h->dtNextValid) {
-        rc = launch_dt_cells(h, false);
+        rc = launch_dt_cells(h, false, nullptr, true);
         if (rc) return rc;
     }
     rc = launch_finalize(h, c.explicitParams.cfl, c.explicitParams.maxDt, c.explicitParams.minDt,
@@ -891,7 +891,7 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
     const auto& sp = c.semiImplicitParams;
     int rc = ensure_jacobi(h, sp.maxPressureIters);
     if (rc) return rc;
-    rc = launch_dt_cells(h, true, h->consField<double>(h->cur, 0));
+    rc = launch_dt_cells(h, true, h->consField<double>(h->cur, 0), true);
     if (!rc) rc = launch_finalize(h, sp.cfl, sp.maxDt, sp.minDt, -1.0, targetDt, true, false, 0.0);
     if (rc) return rc;
     const int ord = c.RKOrder;
@@ -908,6 +908,10 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
     for (int s = 0; s < ord; ++s) {
         rc = run_igr_any(h, src[s], nullptr);
         if (rc) return rc;
+        if (has_viscosity(c)) {
+            rc = launch_visc<double>(h, src[s], nullptr);
+            if (rc) return rc;
+        }
         rc = launch_stage<double>(h, src[s], u0[s], dst[s], kRK[ord - 1][s], ord == 1, false, nullptr, true);
         if (rc) return rc;
         double* rho = h->consField<double>(dst[s], 0);

@@ -78,8 +78,8 @@ __global__ void __launch_bounds__(kRedThreads) dt_cells_kernel(DtArgs A) {
             }
             dc = h[0] / (fabs(A.vel[0][id]) + c);
             for (int d = 1; d < g.dim; ++d) dc = dmin_ref(dc, h[d] / (fabs(A.vel[d][id]) + c));
-            if (A.viscMu > 0.0) dc = dmin_ref(dc, visc_cell_dt(h, g.dim, A.viscMu, A.rho[id]));
         }
+        if (A.viscMu > 0.0) dc = dmin_ref(dc, visc_cell_dt(h, g.dim, A.viscMu, A.rho[id]));
         m = dmin_ref(m, dc);
     }
     m = block_min(m, sh);

@@ -1,5 +1,6 @@
-// sifv_visc.hpp - viscous stress of the explicit solver on the GPU
-// (addViscousFluxes, src/ViscousFlux.cpp:25-303).
+// sifv_visc.hpp - viscous stress of the explicit and semi-implicit solvers on the GPU
+// (addViscousFluxes, src/ViscousFlux.cpp:25-303; called from
+// src/ExplicitSolver.cpp:523-527 and src/SemiImplicitSolver.cpp:509-514).
 //
 // visc_kernel writes, per interior cell, the viscous part of the momentum and
 // energy right-hand sides of the RK stage's state: the cell's 2*DIM faces in

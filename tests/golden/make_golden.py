@@ -203,13 +203,18 @@ def g_viscous():
     for key, (build, rs, nsteps, semi) in viscous_cases().items():
         cfg, mesh, st, eos = build()
         s = po.CpuSolver(mesh, cfg, rs, eos, "ref")
-        dts, t = [], 0.0
+        dts, its, t = [], [], 0.0
         for k in range(nsteps):
             cfg.step, cfg.time = k, t
-            dt = s.explicit_step(st, cfg)
+            if semi:
+                dt, it = s.semi_step(st, cfg)
+                its.append(it)
+            else:
+                dt = s.explicit_step(st, cfg)
             dts.append(dt)
             t += dt
         out[f"{key}_dts"] = np.array(dts)
+        out[f"{key}_iters"] = np.array(its, dtype=np.int64)
         for fid in st.field_ids():
             out[f"{key}_f{fid}"] = np.ascontiguousarray(mesh.interior(st.field(fid)))
     save("viscous_ref.npz", **out)

@@ -385,7 +385,34 @@ def viscous_cases():
         cfg.viscousParams.phaseMu = [50.0, 1.0]
         return cfg, mesh, st, eos
 
-    return {"v1": (v1, 2, 30, False), "v2": (v2, 2, 20, False), "v3": (v3, 1, 8, False), "v2p": (v2p, 2, 6, False)}
+    def semi(cfg):
+        cfg.semiImplicit = True
+        cfg.semiImplicitParams.cfl = 0.5
+        cfg.semiImplicitParams.maxDt = 1e-2
+        cfg.semiImplicitParams.pressureTol = 1e-9
+        cfg.semiImplicitParams.maxPressureIters = 300
+        cfg.validate()
+        return cfg
+
+    def s2():
+        cfg, mesh, st, eos = make_case(2, (30, 22), bcs=[SL, OUT, NS, NS, SL, SL], seed=35, nonuniform=True)
+        cfg.viscousParams.mu = 2e-2
+        return semi(cfg), mesh, st, eos
+
+    def s3():
+        cfg, mesh, st, eos = make_case(3, (9, 8, 10), bcs=[NS, NS, SL, OUT, NS, SL], periods=(0, 0, 0), seed=36)
+        cfg.viscousParams.mu = 0.2
+        return semi(cfg), mesh, st, eos
+
+    def s2p():
+        # rising-bubble ingredients: per-phase mu and gravity on a water/air state
+        cfg, mesh, st, eos = make_two_phase_case(2, (22, 26), bcs=[NS] * 6, seed=37, semi=True, p_jump=False)
+        cfg.viscousParams.phaseMu = [10.0, 0.1]
+        cfg.bodyForceParams.a = [0.0, -0.98, 0.0]
+        return cfg, mesh, st, eos
+
+    return {"v1": (v1, 2, 30, False), "v2": (v2, 2, 20, False), "v3": (v3, 1, 8, False), "v2p": (v2p, 2, 6, False),
+            "s2": (s2, 2, 4, True), "s3": (s3, 1, 3, True), "s2p": (s2p, 2, 3, True)}
 
 
 def make_case(dim, n, bcs=None, periods=(0, 0, 0), recon=2, rk=3, seed=0, nonuniform=False, eos_kind="ideal",

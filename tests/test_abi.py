@@ -48,12 +48,7 @@ def test_create_validates_like_the_reference(mutate, msg):
 
 
 def test_unsupported_terms_are_rejected():
-    """Viscosity on the semi-implicit handle and surface tension are refused
-    before any device call."""
-    cfg = SimulationConfig(dim=1, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5, semiImplicit=True)
-    cfg.viscousParams.mu = 1e-3
-    rc, err = _create(cfg)
-    assert rc == _abi.ERR_UNSUPPORTED and "viscous" in err
+    """Surface tension is refused before any device call."""
     cfg = SimulationConfig(dim=1, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5)
     cfg.surfaceTensionParams.sigma = 0.07
     rc, err = _create(cfg)

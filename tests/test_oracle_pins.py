@@ -385,33 +385,41 @@ def test_body_force_golden(oracle_c, key):
 
 
 def _run_viscous(po, key, drop=False):
-    build, rs, nsteps, _ = problems.viscous_cases()[key]
+    build, rs, nsteps, semi = problems.viscous_cases()[key]
     cfg, mesh, st, eos = build()
     if drop:
         cfg.viscousParams.mu, cfg.viscousParams.phaseMu = 0.0, []
     s = po.CpuSolver(mesh, cfg, rs, eos, "c")
-    dts, t = [], 0.0
+    dts, its, t = [], [], 0.0
     for k in range(nsteps):
         cfg.step, cfg.time = k, t
-        dts.append(s.explicit_step(st, cfg))
-        t += dts[-1]
-    return mesh, st, dts
+        if semi:
+            dt, it = s.semi_step(st, cfg)
+            its.append(it)
+        else:
+            dt = s.explicit_step(st, cfg)
+        dts.append(dt)
+        t += dt
+    return mesh, st, dts, its
 
 
 @pytest.mark.parametrize("key", list(problems.viscous_cases()))
 def test_viscous_golden(oracle_c, key):
     """Viscous stress (scalar mu 1D-3D with no-slip, slip, outflow and
-    periodic faces on stretched meshes; per-phase mu on a water/air state)
-    and the viscous time-step limit: oracle against the reference's own
-    output; without the term the state is far from the fixture."""
+    periodic faces on stretched meshes; per-phase mu on a water/air state;
+    explicit and semi-implicit, the latter with the viscous limit binding in
+    s2/s3 and gravity in s2p) and the viscous time-step limit: oracle
+    against the reference's own output; without the term the state is far
+    from the fixture."""
     d = load("viscous_ref.npz")
-    mesh, st, dts = _run_viscous(oracle_c, key)
+    mesh, st, dts, its = _run_viscous(oracle_c, key)
     assert np.allclose(dts, d[f"{key}_dts"], rtol=1e-13, atol=0.0)
+    assert list(its) == list(d[f"{key}_iters"])
     floor = 1e-10 if key.endswith("p") else 1e-12
     for fid in st.field_ids():
         l1, li = rel_errs(mesh.interior(st.field(fid)), d[f"{key}_f{fid}"])
         assert l1 <= floor and li <= 100 * floor, (fid, l1, li)
-    mesh0, st0, _ = _run_viscous(oracle_c, key, drop=True)
+    mesh0, st0, _, _ = _run_viscous(oracle_c, key, drop=True)
     far = max(rel_errs(mesh0.interior(st0.field(fid)), d[f"{key}_f{fid}"])[1] for fid in (1, 4))
     assert far > 1e-8, far
 
