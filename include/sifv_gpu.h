@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SIFV_ABI_VERSION 2
+#define SIFV_ABI_VERSION 3
 #define SIFV_MAX_PHASES 8
 
 /* ---- status codes ---------------------------------------------------- */
@@ -76,9 +76,9 @@ typedef struct sifv_phase_eos { double gamma, pInf; } sifv_phase_eos;
 /* Mirror of SimulationConfig and its *Params structs
  * (include/SimulationConfig.hpp:22-91).  The body force is applied in the
  * stage update of the explicit and semi-implicit solvers at the step's config
- * time; the viscous stress in the stage update of both solvers; surface
- * tension is outside the GPU path and must be zero (the ensemble and the
- * adjoint carry the flux terms only). */
+ * time; the viscous and capillary stresses in the stage update of both
+ * solvers (surface tension needs a multiphase state); the ensemble and the
+ * adjoint carry the flux terms only. */
 typedef struct sifv_config {
     int dim, nGhost, RKOrder, useIGR, semiImplicit, reconOrder;
     double wenoEps;
@@ -94,6 +94,8 @@ typedef struct sifv_config {
     /* ViscousParams::phaseMu (empty: viscousNPhaseMu = 0, the scalar mu) */
     int viscousNPhaseMu;
     double viscousPhaseMu[SIFV_MAX_PHASES];
+    /* SurfaceTensionParams::epsGradAlpha (> 0; the reference's default 1e-8) */
+    double surfaceTensionEps;
 } sifv_config;
 
 /* Mirror of the RectilinearMesh constructor arguments plus the boundary

@@ -72,6 +72,7 @@ inline sifv_config to_c(const SimulationConfig& c) {
     for (int p = 0; p < o.viscousNPhaseMu && p < SIFV_MAX_PHASES; ++p)
         o.viscousPhaseMu[p] = c.viscousParams.phaseMu[p];
     o.surfaceTensionSigma = c.surfaceTensionParams.sigma;
+    o.surfaceTensionEps = c.surfaceTensionParams.epsGradAlpha;
     return o;
 }
 

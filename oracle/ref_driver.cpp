@@ -89,6 +89,7 @@ SimulationConfig to_config(const sifv_config* c) {
     s.viscousParams.mu = c->viscousMu;
     s.viscousParams.phaseMu.assign(c->viscousPhaseMu, c->viscousPhaseMu + c->viscousNPhaseMu);
     s.surfaceTensionParams.sigma = c->surfaceTensionSigma;
+    s.surfaceTensionParams.epsGradAlpha = c->surfaceTensionEps;
     for (int q = 0; q < 3; ++q) {
         s.bodyForceParams.a[q] = c->bodyForceParams.a[q];
         s.bodyForceParams.b[q] = c->bodyForceParams.b[q];

@@ -872,6 +872,92 @@ static void add_viscous(or_solver* s, double* const* f, const sifv_config* c) {
     }
 }
 
+/* surface tension: addSurfaceTensionFluxes (src/SurfaceTension.cpp:7-255).
+ * Capillary stress of the face between L and R along `dir` from the
+ * gradient g of alpha[0] on the face (normal component from L and R,
+ * transverse components the mean of the central differences in L and R):
+ * T[dir] = sigma (|g|^2 - g_dir^2) / max(|g|, eps), T[a] = -sigma g_dir g_a /
+ * max(|g|, eps) (a != dir), work T . u_face; L gains +area/vol(L) * (T, work),
+ * R gains -area/vol(R).  Faces in the reference's loop order. */
+static void add_surface_tension(or_solver* s, double* const* f, const sifv_config* c) {
+    or_mesh* m = &s->m;
+    const int dim = m->dim;
+    const double sigma = c->surfaceTensionSigma, eps = c->surfaceTensionEps;
+    const double* al = f[SIFV_F_ALPHA];
+    const int velId[3] = {SIFV_F_VELU, SIFV_F_VELV, SIFV_F_VELW};
+    const int momId[3] = {SIFV_F_RHOU, SIFV_F_RHOV, SIFV_F_RHOW};
+    for (int dir = 0; dir < dim; ++dir) {
+        const int ei = dir == 0, ej = dir == 1, ek = dir == 2;
+        for (int k = 0; k < m->nz + ek; ++k)
+            for (int j = 0; j < m->ny + ej; ++j)
+                for (int i = 0; i < m->nx + ei; ++i) {
+                    const int li = i - ei, lj = j - ej, lk = k - ek;
+                    const size_t L = IDX(m, li, lj, lk), R = IDX(m, i, j, k);
+                    double g[3] = {0, 0, 0};
+                    double hn = dir == 0 ? 0.5 * (DX(m, li) + DX(m, i))
+                                         : dir == 1 ? 0.5 * (DY(m, lj) + DY(m, j)) : 0.5 * (DZ(m, lk) + DZ(m, k));
+                    g[dir] = (al[R] - al[L]) / hn;
+                    for (int b = 0; b < dim; ++b) {
+                        if (b == dir) continue;
+                        const int bi = b == 0, bj = b == 1, bk = b == 2;
+                        double hb = b == 0 ? DX(m, i) : b == 1 ? DY(m, j) : DZ(m, k);
+                        const size_t Lm = IDX(m, li - bi, lj - bj, lk - bk), Lp = IDX(m, li + bi, lj + bj, lk + bk);
+                        const size_t Rm = IDX(m, i - bi, j - bj, k - bk), Rp = IDX(m, i + bi, j + bj, k + bk);
+                        double dL = (al[Lp] - al[Lm]) / (2.0 * hb);
+                        double dR = (al[Rp] - al[Rm]) / (2.0 * hb);
+                        g[b] = 0.5 * (dL + dR);
+                    }
+                    /* |g| summed in x, y, z order */
+                    double ng = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+                    double nr = dmax(ng, eps);
+                    double T[3];
+                    for (int a = 0; a < 3; ++a) {
+                        if (a == dir) {
+                            /* the squares of the two transverse components, in axis order */
+                            const int b0 = dir == 0 ? 1 : 0, b1 = dir == 2 ? 1 : 2;
+                            T[a] = sigma * (g[b0] * g[b0] + g[b1] * g[b1]) / nr;
+                        } else {
+                            T[a] = -sigma * g[dir] * g[a] / nr;
+                        }
+                    }
+                    double uf[3] = {0, 0, 0};
+                    for (int a = 0; a < dim; ++a) uf[a] = 0.5 * (f[velId[a]][L] + f[velId[a]][R]);
+                    double work = T[0] * uf[0] + T[1] * uf[1] + T[2] * uf[2];
+                    double area = dir == 0 ? DY(m, j) * DZ(m, k) : dir == 1 ? DX(m, i) * DZ(m, k) : DX(m, i) * DY(m, j);
+                    const int inL = dir == 0 ? i >= 1 : dir == 1 ? j >= 1 : k >= 1;
+                    const int inR = dir == 0 ? i < m->nx : dir == 1 ? j < m->ny : k < m->nz;
+                    if (inL) {
+                        double coeff = area / VOL(m, li, lj, lk);
+                        for (int a = 0; a < dim; ++a) s->rhs[momId[a]][L] += coeff * T[a];
+                        s->rhs[SIFV_F_RHOE][L] += coeff * work;
+                    }
+                    if (inR) {
+                        double coeff = area / VOL(m, i, j, k);
+                        for (int a = 0; a < dim; ++a) s->rhs[momId[a]][R] -= coeff * T[a];
+                        s->rhs[SIFV_F_RHOE][R] -= coeff * work;
+                    }
+                }
+    }
+}
+
+/* computeCapillaryDt (src/RKTimeStepping.cpp:201-228) */
+static double capillary_dt(or_solver* s, double* const* f, double sigma, double cfl, double maxDt) {
+    or_mesh* m = &s->m;
+    int dim = m->dim;
+    double dt = maxDt;
+    for (int k = 0; k < m->nz; ++k)
+        for (int j = 0; j < m->ny; ++j)
+            for (int i = 0; i < m->nx; ++i) {
+                size_t id = IDX(m, i, j, k);
+                double h = DX(m, i);
+                if (dim >= 2) h = dmin(h, DY(m, j));
+                if (dim >= 3) h = dmin(h, DZ(m, k));
+                double dtCell = sqrt(dmax(f[SIFV_F_RHO][id], 1e-14) * h * h * h / sigma);
+                dt = dmin(dt, cfl * dtCell);
+            }
+    return dt;
+}
+
 /* computeViscousDt (src/RKTimeStepping.cpp:161-188): scalar mu only */
 static double viscous_dt(or_solver* s, double* const* f, double mu, double cfl, double maxDt) {
     or_mesh* m = &s->m;
@@ -1016,6 +1102,7 @@ static void compute_rhs(or_solver* s, double* const* f, const sifv_config* c, in
                 }
     }
     if (has_viscosity(c)) add_viscous(s, f, c);
+    if (c->surfaceTensionSigma > 0.0) add_surface_tension(s, f, c);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1201,6 +1288,10 @@ double or_explicit_step(or_solver* s, double* const* f, const sifv_config* c, do
         /* src/ExplicitSolver.cpp:78-81 */
         if (has_viscosity(c))
             dt = dmin(dt, viscous_dt(s, f, c->viscousMu, c->explicitParams.cfl, c->explicitParams.maxDt));
+        /* src/ExplicitSolver.cpp:82-85 */
+        if (c->surfaceTensionSigma > 0.0)
+            dt = dmin(dt, capillary_dt(s, f, c->surfaceTensionSigma, c->explicitParams.cfl,
+                                       c->explicitParams.maxDt));
     }
     if (targetDt > 0) dt = dmin(dt, targetDt);
     int ord = c->RKOrder;
@@ -1329,6 +1420,9 @@ double or_semi_step(or_solver* s, double* const* f, const sifv_config* c, double
     /* src/SemiImplicitSolver.cpp:112-115 */
     if (has_viscosity(c))
         dt = dmin(dt, viscous_dt(s, f, c->viscousMu, c->semiImplicitParams.cfl, c->semiImplicitParams.maxDt));
+    if (c->surfaceTensionSigma > 0.0)
+        dt = dmin(dt, capillary_dt(s, f, c->surfaceTensionSigma, c->semiImplicitParams.cfl,
+                                   c->semiImplicitParams.maxDt));
     if (targetDt > 0) dt = dmin(dt, targetDt);
     /* std::clamp(dt, minDt, maxDt) */
     if (dt < c->semiImplicitParams.minDt) dt = c->semiImplicitParams.minDt;

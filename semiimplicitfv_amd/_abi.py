@@ -61,6 +61,7 @@ class Config_C(C.Structure):
         ("multiPhaseParams", _Multi), ("bodyForceParams", _Body),
         ("viscousMu", C.c_double), ("surfaceTensionSigma", C.c_double),
         ("viscousNPhaseMu", C.c_int), ("viscousPhaseMu", C.c_double * MAX_PHASES),
+        ("surfaceTensionEps", C.c_double),
     ]
 
 

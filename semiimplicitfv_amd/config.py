@@ -247,4 +247,5 @@ class SimulationConfig:
         for i, m in enumerate(pm):
             c.viscousPhaseMu[i] = m
         c.surfaceTensionSigma = self.surfaceTensionParams.sigma
+        c.surfaceTensionEps = self.surfaceTensionParams.epsGradAlpha
         return c

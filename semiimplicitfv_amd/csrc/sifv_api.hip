@@ -108,12 +108,16 @@ int validate_config(const sifv_config* c) {
     }
     if (c->viscousMu < 0.0) return fail(SIFV_ERR_INVALID, "viscousParams.mu must be >= 0");
     if (c->surfaceTensionSigma < 0.0) return fail(SIFV_ERR_INVALID, "surfaceTensionParams.sigma must be >= 0");
+    if (c->surfaceTensionEps <= 0.0) return fail(SIFV_ERR_INVALID, "surfaceTensionParams.epsGradAlpha must be > 0");
     if (c->viscousNPhaseMu < 0 || c->viscousNPhaseMu > SIFV_MAX_PHASES)
         return fail(SIFV_ERR_INVALID, "viscousParams.phaseMu: bad phase count");
     // terms outside the GPU path
     if (c->viscousNPhaseMu > 0 && c->viscousNPhaseMu < c->multiPhaseParams.nPhases)
         return fail(SIFV_ERR_UNSUPPORTED, "viscousParams.phaseMu needs one value per phase of a multiphase state");
-    if (c->surfaceTensionSigma > 0.0) return fail(SIFV_ERR_UNSUPPORTED, "surface tension is not on the GPU path");
+    // the capillary stress reads alpha[0] (src/SurfaceTension.cpp:19), which a
+    // single-phase state does not have
+    if (c->surfaceTensionSigma > 0.0 && c->multiPhaseParams.nPhases == 0)
+        return fail(SIFV_ERR_UNSUPPORTED, "surface tension needs a multiphase state");
 
     return SIFV_OK;
 }
@@ -314,6 +318,9 @@ bool has_body_force(const sifv_config& c) {
 
 // SimulationConfig::hasViscosity (include/SimulationConfig.hpp:96-98)
 bool has_viscosity(const sifv_config& c) { return c.viscousMu > 0.0 || c.viscousNPhaseMu > 0; }
+// SimulationConfig::hasSurfaceTension (include/SimulationConfig.hpp:99)
+bool has_surface_tension(const sifv_config& c) { return c.surfaceTensionSigma > 0.0; }
+bool has_face_stress(const sifv_config& c) { return has_viscosity(c) || has_surface_tension(c); }
 
 int timed_launch(sifv_handle* h, const void* fn, dim3 grid, dim3 block, void** args) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -361,8 +368,9 @@ int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const d
     P.pAdv = h->jac[4];
     P.bodyForce = has_body_force(h->cfg) ? 1 : 0;
     P.accel = h->scal + 4;
-    P.visc = has_viscosity(h->cfg) ? h->visc : nullptr;
+    P.visc = has_face_stress(h->cfg) ? h->visc : nullptr;
     P.viscMu = (!semi && h->cfg.viscousMu > 0.0) ? h->cfg.viscousMu : 0.0;
+    P.capSigma = (!semi && has_surface_tension(h->cfg)) ? h->cfg.surfaceTensionSigma : 0.0;
     StageKernels<T> k = stage_kernels<T>(h, semi);
     if (!k.xy || (h->dim == 3 && !k.z)) return fail(SIFV_ERR_UNSUPPORTED, "no stage kernel for this configuration");
     const bool wantDt = last && !semi && h->cfg.explicitParams.constDt <= 0.0;
@@ -432,10 +440,11 @@ const double kRK[3][3][4] = {
     {{1, 0, 1, 1}, {1, 3, 1, 4}, {2, 1, 2, 3}},
 };
 
-// withVisc: the step's dt, where computeViscousDt joins the acoustic
-// (src/ExplicitSolver.cpp:78-81) or advective (src/SemiImplicitSolver.cpp:112-115)
-// limit; the standalone dt queries leave it out.
-int launch_dt_cells(sifv_handle* h, bool advective, const double* rho = nullptr, bool withVisc = false) {
+// withStress: the step's dt, where computeViscousDt and computeCapillaryDt
+// join the acoustic (src/ExplicitSolver.cpp:78-85) or advective
+// (src/SemiImplicitSolver.cpp:112-119) limit; the standalone dt queries leave
+// them out.
+int launch_dt_cells(sifv_handle* h, bool advective, const double* rho = nullptr, bool withStress = false) {
     DtArgs A;
     A.g = h->grid();
     A.eos = h->eosd();
@@ -449,7 +458,8 @@ int launch_dt_cells(sifv_handle* h, bool advective, const double* rho = nullptr,
         A.alpha[p] = h->nph ? h->consField<double>(h->cur, h->dim + 2 + h->nph + pp) : nullptr;
     }
     A.advective = advective ? 1 : 0;
-    A.viscMu = (withVisc && h->cfg.viscousMu > 0.0) ? h->cfg.viscousMu : 0.0;
+    A.viscMu = (withStress && h->cfg.viscousMu > 0.0) ? h->cfg.viscousMu : 0.0;
+    A.capSigma = (withStress && has_surface_tension(h->cfg)) ? h->cfg.surfaceTensionSigma : 0.0;
     size_t n = (size_t)h->nx * h->ny * h->nz;
     int nb = (int)std::min<size_t>((n + kRedThreads - 1) / kRedThreads, 2048);
     int rc = ensure_partials(h, nb);
@@ -597,9 +607,12 @@ int launch_visc(sifv_handle* h, int src, const int* active) {
     A.P.total = h->total;
     A.P.src = h->consField<T>(src, 0);
     A.P.active = active;
+    A.viscous = has_viscosity(h->cfg) ? 1 : 0;
     A.mu = h->cfg.viscousMu;
     A.nPhaseMu = h->cfg.viscousNPhaseMu;
     for (int p = 0; p < kMaxPhases; ++p) A.phaseMu[p] = p < A.nPhaseMu ? h->cfg.viscousPhaseMu[p] : 0.0;
+    A.sigma = h->cfg.surfaceTensionSigma;
+    A.capEps = h->cfg.surfaceTensionEps;
     A.out = h->visc;
     void* args[] = {&A};
     HIPCHK(hipLaunchKernel(reinterpret_cast<const void*>(fn), dim3(elementwise_blocks(h)), dim3(256), args, 0,
@@ -624,7 +637,7 @@ int enqueue_explicit_step(sifv_handle* h, double targetDt, bool loop, double end
         bool last = s == ord - 1;
         int irc = run_igr_any(h, src, active);
         if (irc) return irc;
-        if (has_viscosity(c)) {
+        if (has_face_stress(c)) {
             irc = h->precision == SIFV_MIXED ? launch_visc<float>(h, src, active) : launch_visc<double>(h, src, active);
             if (irc) return irc;
         }
@@ -908,7 +921,7 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
     for (int s = 0; s < ord; ++s) {
         rc = run_igr_any(h, src[s], nullptr);
         if (rc) return rc;
-        if (has_viscosity(c)) {
+        if (has_face_stress(c)) {
             rc = launch_visc<double>(h, src[s], nullptr);
             if (rc) return rc;
         }
@@ -1118,7 +1131,8 @@ int adj_check(const sifv_handle* h) {
     for (int d = 0; d < 3; ++d)
         if (c.bodyForceParams.a[d] != 0.0 || c.bodyForceParams.b[d] != 0.0)
             return fail(SIFV_ERR_UNSUPPORTED, "the adjoint covers the flux terms only (no body force)");
-    if (has_viscosity(c)) return fail(SIFV_ERR_UNSUPPORTED, "the adjoint covers the flux terms only (no viscosity)");
+    if (has_face_stress(c))
+        return fail(SIFV_ERR_UNSUPPORTED, "the adjoint covers the flux terms only (no viscous or capillary stress)");
     return SIFV_OK;
 }
 

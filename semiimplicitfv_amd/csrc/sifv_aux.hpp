@@ -46,7 +46,8 @@ struct DtArgs {
     const double* alpha[kMaxPhases];
     const double* alphaRho[kMaxPhases];
     int advective;
-    double viscMu = 0.0;  // > 0: include the viscous limit (acoustic mode)
+    double viscMu = 0.0;    // > 0: include the viscous limit
+    double capSigma = 0.0;  // > 0: include the capillary limit
     double* partial;
 };
 
@@ -80,6 +81,7 @@ __global__ void __launch_bounds__(kRedThreads) dt_cells_kernel(DtArgs A) {
             for (int d = 1; d < g.dim; ++d) dc = dmin_ref(dc, h[d] / (fabs(A.vel[d][id]) + c));
         }
         if (A.viscMu > 0.0) dc = dmin_ref(dc, visc_cell_dt(h, g.dim, A.viscMu, A.rho[id]));
+        if (A.capSigma > 0.0) dc = dmin_ref(dc, cap_cell_dt(h, g.dim, A.capSigma, A.rho[id]));
         m = dmin_ref(m, dc);
     }
     m = block_min(m, sh);

@@ -223,7 +223,7 @@ int sifv_ensemble_create(int ntubes, const int* ncells, const double* nodes, con
     if (precision != SIFV_FP64 && precision != SIFV_MIXED) return set_error(SIFV_ERR_INVALID, "unknown precision");
     for (int d = 0; d < 3; ++d)
         if (cfg->bodyForceParams.a[d] != 0.0 || cfg->bodyForceParams.b[d] != 0.0 || cfg->viscousMu > 0.0 ||
-            cfg->surfaceTensionSigma > 0.0)
+            cfg->viscousNPhaseMu > 0 || cfg->surfaceTensionSigma > 0.0)
             return set_error(SIFV_ERR_UNSUPPORTED, "ensemble runs carry the flux terms only");
     auto* e = new sifv_ensemble();
     e->ntubes = ntubes;

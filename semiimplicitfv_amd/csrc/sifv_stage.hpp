@@ -106,6 +106,7 @@ struct StageParams {
     const double* accel = nullptr;  // device-resident body-force acceleration of the step
     const double* visc = nullptr;   // viscous momentum/energy right-hand side (DIM+1 fields) or null
     double viscMu = 0.0;            // > 0: the viscous limit joins the fused time-step reduction
+    double capSigma = 0.0;          // > 0: the capillary limit joins it too
 };
 
 // Viscous limit of one cell (computeViscousDt, src/RKTimeStepping.cpp:161-188),
@@ -116,6 +117,15 @@ __device__ __forceinline__ double visc_cell_dt(const double* h, int dim, double 
     if (dim >= 3) hm = dmin_ref(hm, h[2]);
     const double nu = mu / dmax_ref(rho, 1e-14);
     return hm * hm / (2.0 * dim * nu);
+}
+
+// Capillary limit of one cell (computeCapillaryDt, src/RKTimeStepping.cpp:201-228),
+// without the cfl factor.
+__device__ __forceinline__ double cap_cell_dt(const double* h, int dim, double sigma, double rho) {
+    double hm = h[0];
+    if (dim >= 2) hm = dmin_ref(hm, h[1]);
+    if (dim >= 3) hm = dmin_ref(hm, h[2]);
+    return sqrt(dmax_ref(rho, 1e-14) * hm * hm * hm / sigma);
 }
 
 // Logical block id with all blocks of one XCD contiguous (blocks are
@@ -388,7 +398,7 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
         acc[DIM + 1] = (T)((double)acc[DIM + 1] + rho * work);
     }
     if (P.visc) {
-        // viscous stress (src/ExplicitSolver.cpp:523-527)
+        // viscous and capillary stress (src/ExplicitSolver.cpp:523-535)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) acc[1 + d] = (T)((double)acc[1 + d] + P.visc[(size_t)d * P.total + id]);
         acc[DIM + 1] = (T)((double)acc[DIM + 1] + P.visc[(size_t)DIM * P.total + id]);
@@ -470,6 +480,7 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
 #pragma unroll
     for (int d = 1; d < DIM; ++d) dc = dmin_ref(dc, h[d] / (fabs(c.u[d]) + cs));
     if (P.viscMu > 0.0) dc = dmin_ref(dc, visc_cell_dt(h, DIM, P.viscMu, c.rho));
+    if (P.capSigma > 0.0) dc = dmin_ref(dc, cap_cell_dt(h, DIM, P.capSigma, c.rho));
     return dc;
 }
 

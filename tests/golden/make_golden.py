@@ -22,7 +22,7 @@ from semiimplicitfv_amd import (BoundaryCondition, IdealGasEOS, ReconstructionOr
                                 SolutionState, StiffenedGasEOS, cases, uniform_mesh)
 from semiimplicitfv_amd.cases import _fill  # noqa: E402
 from semiimplicitfv_amd.timeloop import run_time_loop  # noqa: E402
-from tests.golden.problems import (body_force_cases, bubble_3d, jacobi_problem, liquid_gas_tube,  # noqa: E402
+from tests.golden.problems import (body_force_cases, bubble_3d, capillary_cases, jacobi_problem, liquid_gas_tube,  # noqa: E402
                                    random_ghost_case, ensemble_sample, igr_cases, recon_lines, riemann_pairs,
                                    two_phase_cases, viscous_cases)
 
@@ -220,6 +220,29 @@ def g_viscous():
     save("viscous_ref.npz", **out)
 
 
+def g_capillary():
+    """Surface-tension cases, config.time advanced per step."""
+    out = {}
+    for key, (build, rs, nsteps, semi) in capillary_cases().items():
+        cfg, mesh, st, eos = build()
+        s = po.CpuSolver(mesh, cfg, rs, eos, "ref")
+        dts, its, t = [], [], 0.0
+        for k in range(nsteps):
+            cfg.step, cfg.time = k, t
+            if semi:
+                dt, it = s.semi_step(st, cfg)
+                its.append(it)
+            else:
+                dt = s.explicit_step(st, cfg)
+            dts.append(dt)
+            t += dt
+        out[f"{key}_dts"] = np.array(dts)
+        out[f"{key}_iters"] = np.array(its, dtype=np.int64)
+        for fid in st.field_ids():
+            out[f"{key}_f{fid}"] = np.ascontiguousarray(mesh.interior(st.field(fid)))
+    save("capillary_ref.npz", **out)
+
+
 def g_ensemble():
     """Shock tubes of the batched ensemble sample run to t = 0.1 one by one
     (runTimeLoop dt rule): step counts, final times and interior fields."""
@@ -244,7 +267,8 @@ def g_ensemble():
 
 GENERATORS = {"recon": g_recon, "riemann": g_riemann, "sod": g_sod, "vortex": g_vortex, "ghost": g_ghost,
               "jacobi": g_jacobi, "semi": g_semi, "liquid_gas": g_liquid_gas, "two_phase": g_two_phase,
-              "igr": g_igr, "ensemble": g_ensemble, "body_force": g_body_force, "viscous": g_viscous}
+              "igr": g_igr, "ensemble": g_ensemble, "body_force": g_body_force, "viscous": g_viscous,
+              "capillary": g_capillary}
 
 if __name__ == "__main__":
     assert po.available("ref"), "build oracle/_ref first (make -C oracle)"

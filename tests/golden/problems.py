@@ -235,7 +235,7 @@ def liquid_gas_tube(n: int = 200):
 
 
 def make_two_phase_case(dim, n, bcs=None, periods=(0, 0, 0), recon=2, rk=3, seed=0, semi=False, p_jump=True,
-                        cfl=0.4, nonuniform=False):
+                        cfl=0.4, nonuniform=False, circle=False):
     """Water/air state with a smooth (tanh) interface, random-ish velocity
     and an optional pressure jump.  semi=True configures the semi-implicit
     solver (dt <= 2e-5, tolerance 1e-6, up to 400 Jacobi sweeps)."""
@@ -276,6 +276,10 @@ def make_two_phase_case(dim, n, bcs=None, periods=(0, 0, 0), recon=2, rk=3, seed
     Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
     h = 1.0 / max(n[:dim])
     s = (X - 0.45) + (0.3 * (Y - 0.5) if dim >= 2 else 0.0) + (0.2 * (Z - 0.5) if dim >= 3 else 0.0)
+    if circle:
+        # air bubble of radius 0.25 in water (water where s < 0 is outside)
+        r2 = (X - 0.5) ** 2 + (Y - 0.45) ** 2 + ((Z - 0.5) ** 2 if dim >= 3 else 0.0)
+        s = 0.25 - np.sqrt(r2)
     amin = cfg.multiPhaseParams.alphaMin
     aw = np.clip(0.5 * (1.0 - np.tanh(s / (2.0 * h))), amin, 1.0 - amin)
     aa = 1.0 - aw
@@ -413,6 +417,64 @@ def viscous_cases():
 
     return {"v1": (v1, 2, 30, False), "v2": (v2, 2, 20, False), "v3": (v3, 1, 8, False), "v2p": (v2p, 2, 6, False),
             "s2": (s2, 2, 4, True), "s3": (s3, 1, 3, True), "s2p": (s2p, 2, 3, True)}
+
+
+def capillary_cases():
+    """Seeded water/air cases with surface tension (planar and circular
+    interfaces, with and without per-phase viscosity and gravity); reference
+    outputs in capillary_ref.npz:
+    key -> (builder, Riemann kind (0 LF, 1 Rusanov, 2 HLLC), steps, semi)."""
+    NS = BoundaryCondition.NoSlipWall
+    SL = BoundaryCondition.SlipWall
+    OUT = BoundaryCondition.Outflow
+
+    def c1():
+        # 1D: the stress vanishes (no transverse gradient); the capillary
+        # limit binds the time step
+        cfg, mesh, st, eos = make_two_phase_case(1, (60,), bcs=[NS, OUT] + [SL] * 4, seed=41, p_jump=False)
+        cfg.surfaceTensionParams.sigma = 1e6
+        return cfg, mesh, st, eos
+
+    def c2():
+        cfg, mesh, st, eos = make_two_phase_case(2, (26, 22), bcs=[SL, OUT, NS, SL, SL, SL], seed=42, p_jump=False,
+                                                 circle=True, nonuniform=True)
+        cfg.surfaceTensionParams.sigma = 2e3
+        return cfg, mesh, st, eos
+
+    def c2e():
+        # planar interface, epsGradAlpha large enough to regularise |grad alpha|
+        cfg, mesh, st, eos = make_two_phase_case(2, (20, 18), bcs=[NS] * 6, seed=43, p_jump=False)
+        cfg.surfaceTensionParams.sigma = 5e2
+        cfg.surfaceTensionParams.epsGradAlpha = 2.0
+        return cfg, mesh, st, eos
+
+    def c3():
+        cfg, mesh, st, eos = make_two_phase_case(3, (10, 9, 11), bcs=[NS, SL, OUT, NS, SL, NS], seed=44,
+                                                 p_jump=False, circle=True)
+        cfg.surfaceTensionParams.sigma = 1e3
+        cfg.viscousParams.phaseMu = [5.0, 0.5]
+        return cfg, mesh, st, eos
+
+    def cs2():
+        # 2D_rising_bubble ingredients on the semi-implicit solver: surface
+        # tension, per-phase viscosity and gravity
+        cfg, mesh, st, eos = make_two_phase_case(2, (24, 26), bcs=[NS] * 6, seed=45, semi=True, p_jump=False,
+                                                 circle=True)
+        cfg.surfaceTensionParams.sigma = 1.96e2
+        cfg.viscousParams.phaseMu = [10.0, 0.1]
+        cfg.bodyForceParams.a = [0.0, -0.98, 0.0]
+        return cfg, mesh, st, eos
+
+    def cs3():
+        cfg, mesh, st, eos = make_two_phase_case(3, (9, 10, 8), bcs=[SL, NS, NS, SL, OUT, SL], seed=46, semi=True,
+                                                 p_jump=False, circle=True)
+        # the capillary limit binds below maxDt
+        cfg.surfaceTensionParams.sigma = 1e5
+        cfg.semiImplicitParams.maxDt = 1e-3
+        return cfg, mesh, st, eos
+
+    return {"c1": (c1, 2, 12, False), "c2": (c2, 2, 8, False), "c2e": (c2e, 1, 6, False), "c3": (c3, 2, 4, False),
+            "cs2": (cs2, 2, 3, True), "cs3": (cs3, 2, 2, True)}
 
 
 def make_case(dim, n, bcs=None, periods=(0, 0, 0), recon=2, rk=3, seed=0, nonuniform=False, eos_kind="ideal",

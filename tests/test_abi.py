@@ -15,7 +15,7 @@ def test_library_exports_every_declared_symbol():
     assert len(declared) >= 20
     missing = [s for s in declared if not hasattr(lib, s)]
     assert not missing, missing
-    assert lib.sifv_abi_version() == 2
+    assert lib.sifv_abi_version() == 3
 
 
 def _create(cfg, mesh=None):
@@ -35,6 +35,7 @@ def _create(cfg, mesh=None):
     (lambda c: setattr(c, "RKOrder", 4), "RKOrder must be 1, 2, or 3 (got 4)"),
     (lambda c: setattr(c.explicitParams, "cfl", 0.0), "explicitParams.cfl must be > 0"),
     (lambda c: setattr(c.explicitParams, "maxDt", 1e-13), "explicitParams.maxDt must be > minDt"),
+    (lambda c: setattr(c.surfaceTensionParams, "epsGradAlpha", 0.0), "surfaceTensionParams.epsGradAlpha must be > 0"),
 ])
 def test_create_validates_like_the_reference(mutate, msg):
     cfg = SimulationConfig(dim=1, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5)
@@ -48,7 +49,8 @@ def test_create_validates_like_the_reference(mutate, msg):
 
 
 def test_unsupported_terms_are_rejected():
-    """Surface tension is refused before any device call."""
+    """Surface tension on a single-phase state (no alpha[0]) is refused
+    before any device call."""
     cfg = SimulationConfig(dim=1, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5)
     cfg.surfaceTensionParams.sigma = 0.07
     rc, err = _create(cfg)

@@ -424,6 +424,46 @@ def test_viscous_golden(oracle_c, key):
     assert far > 1e-8, far
 
 
+def _run_capillary(po, key, drop=False):
+    build, rs, nsteps, semi = problems.capillary_cases()[key]
+    cfg, mesh, st, eos = build()
+    if drop:
+        cfg.surfaceTensionParams.sigma = 0.0
+    s = po.CpuSolver(mesh, cfg, rs, eos, "c")
+    dts, its, t = [], [], 0.0
+    for k in range(nsteps):
+        cfg.step, cfg.time = k, t
+        if semi:
+            dt, it = s.semi_step(st, cfg)
+            its.append(it)
+        else:
+            dt = s.explicit_step(st, cfg)
+        dts.append(dt)
+        t += dt
+    return mesh, st, dts, its
+
+
+@pytest.mark.parametrize("key", list(problems.capillary_cases()))
+def test_capillary_golden(oracle_c, key):
+    """Surface tension (capillary stress from grad alpha_0 on planar and
+    circular water/air interfaces, regularised |grad alpha|, with per-phase
+    viscosity and gravity; explicit 1D-3D and semi-implicit 2D/3D) and the
+    capillary time-step limit (binding in c1 and cs3): oracle against the
+    reference's own output at the water/air floor; without the term the state
+    (or, in 1D where the stress vanishes, the time step) is far from it."""
+    d = load("capillary_ref.npz")
+    mesh, st, dts, its = _run_capillary(oracle_c, key)
+    assert np.allclose(dts, d[f"{key}_dts"], rtol=1e-11, atol=0.0)
+    assert list(its) == list(d[f"{key}_iters"])
+    for fid in st.field_ids():
+        l1, li = rel_errs(mesh.interior(st.field(fid)), d[f"{key}_f{fid}"])
+        assert l1 <= 1e-10 and li <= 1e-9, (fid, l1, li)
+    mesh0, st0, dts0, _ = _run_capillary(oracle_c, key, drop=True)
+    far = max(rel_errs(mesh0.interior(st0.field(fid)), d[f"{key}_f{fid}"])[1] for fid in (1, 4))
+    far_dt = np.abs(np.array(dts0) / d[f"{key}_dts"] - 1.0).max()
+    assert max(far, far_dt) > 1e-6, (far, far_dt)
+
+
 @pytest.mark.parametrize("key", list(problems.igr_cases()))
 def test_igr_golden(oracle_c, key):
     """IGR (lexicographic Gauss-Seidel entropic pressure, warm start at step
