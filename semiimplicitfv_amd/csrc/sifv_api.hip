@@ -292,6 +292,8 @@ void zpass_shape(const sifv_handle* h, int& nstrip, int& nseg, int& rows) {
     nseg = (h->nz + rows - 1) / rows;
 }
 
+bool stage_lean(const sifv_config& c);
+
 template <typename T>
 StageKernels<T> stage_kernels(const sifv_handle* h, bool semi);
 
@@ -299,14 +301,14 @@ template <>
 StageKernels<double> stage_kernels<double>(const sifv_handle* h, bool semi) {
     int R = recon_radius(h->cfg.reconOrder);
     if (h->dim == 1) return stage_kernels_1d(R, h->riemann, h->nph, semi);
-    if (h->dim == 2) return stage_kernels_2d(R, h->riemann, h->nph, semi);
+    if (h->dim == 2) return stage_kernels_2d(R, h->riemann, h->nph, semi, stage_lean(h->cfg));
     return semi ? stage_kernels_3d_semi(R, h->riemann, h->nph) : stage_kernels_3d(R, h->riemann, h->nph);
 }
 
 template <>
 StageKernels<float> stage_kernels<float>(const sifv_handle* h, bool semi) {
     if (semi || h->nph) return StageKernels<float>();
-    return stage_kernels_fp32(h->dim, recon_radius(h->cfg.reconOrder), h->riemann);
+    return stage_kernels_fp32(h->dim, recon_radius(h->cfg.reconOrder), h->riemann, stage_lean(h->cfg));
 }
 
 // SimulationConfig::hasBodyForce (include/SimulationConfig.hpp:101-106)
@@ -321,6 +323,8 @@ bool has_viscosity(const sifv_config& c) { return c.viscousMu > 0.0 || c.viscous
 // SimulationConfig::hasSurfaceTension (include/SimulationConfig.hpp:99)
 bool has_surface_tension(const sifv_config& c) { return c.surfaceTensionSigma > 0.0; }
 bool has_face_stress(const sifv_config& c) { return has_viscosity(c) || has_surface_tension(c); }
+// No source term of the stage update beyond the flux terms (SRC = false kernels).
+bool stage_lean(const sifv_config& c) { return !has_body_force(c) && !has_face_stress(c); }
 
 int timed_launch(sifv_handle* h, const void* fn, dim3 grid, dim3 block, void** args) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -369,8 +373,6 @@ int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const d
     P.bodyForce = has_body_force(h->cfg) ? 1 : 0;
     P.accel = h->scal + 4;
     P.visc = has_face_stress(h->cfg) ? h->visc : nullptr;
-    P.viscMu = (!semi && h->cfg.viscousMu > 0.0) ? h->cfg.viscousMu : 0.0;
-    P.capSigma = (!semi && has_surface_tension(h->cfg)) ? h->cfg.surfaceTensionSigma : 0.0;
     StageKernels<T> k = stage_kernels<T>(h, semi);
     if (!k.xy || (h->dim == 3 && !k.z)) return fail(SIFV_ERR_UNSUPPORTED, "no stage kernel for this configuration");
     const bool wantDt = last && !semi && h->cfg.explicitParams.constDt <= 0.0;
@@ -380,9 +382,16 @@ int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const d
     P.nseg = nseg;
     P.nstrip = nstrip;
     int nb = nstrip * nseg * (h->dim == 3 ? h->nz : 1);
+    // viscous / capillary limits of the new state: their own reduction after
+    // the stage, into the partial slots after the fused acoustic minima
+    const double viscMu = h->cfg.viscousMu > 0.0 ? h->cfg.viscousMu : 0.0;
+    const double capSigma = has_surface_tension(h->cfg) ? h->cfg.surfaceTensionSigma : 0.0;
+    const bool stressDt = wantDt && (viscMu > 0.0 || capSigma > 0.0);
+    const size_t ncell = (size_t)h->nx * h->ny * h->nz;
+    const int nbs = stressDt ? (int)std::min<size_t>((ncell + kRedThreads - 1) / kRedThreads, 2048) : 0;
     P.dtPartial = nullptr;
     if (wantDt && h->dim < 3) {
-        int rc = ensure_partials(h, nb);
+        int rc = ensure_partials(h, nb + nbs);
         if (rc) return rc;
         P.dtPartial = h->partial;
     }
@@ -400,7 +409,7 @@ int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const d
         int nbz = zs * zn * h->ny;
         Z.dtPartial = nullptr;
         if (wantDt) {
-            rc = ensure_partials(h, nbz);
+            rc = ensure_partials(h, nbz + nbs);
             if (rc) return rc;
             Z.dtPartial = h->partial;
         }
@@ -408,6 +417,21 @@ int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const d
         rc = timed_launch(h, reinterpret_cast<const void*>(k.z), dim3(nbz), dim3(kWave), zargs);
         if (rc) return rc;
         if (wantDt) h->npartialValid = nbz;
+    }
+    if (stressDt) {
+        // room for the stage's partials plus these was ensured above
+        const int base = h->npartialValid;
+        if (base + nbs > h->npartialCap) return fail(SIFV_ERR_INVALID, "partial buffer too small");
+        StressDtArgs<T> S;
+        S.g = h->grid();
+        S.rho = P.dst;
+        S.active = active;
+        S.viscMu = viscMu;
+        S.capSigma = capSigma;
+        S.partial = h->partial + base;
+        hipLaunchKernelGGL(stress_dt_kernel<T>, dim3(nbs), dim3(kRedThreads), 0, h->stream, S);
+        HIPCHK(hipGetLastError());
+        h->npartialValid = base + nbs;
     }
     return SIFV_OK;
 }

@@ -88,6 +88,42 @@ __global__ void __launch_bounds__(kRedThreads) dt_cells_kernel(DtArgs A) {
     if (threadIdx.x == 0) A.partial[blockIdx.x] = m;
 }
 
+// Per-block minimum of the viscous and capillary cell limits over interior
+// cells from the conservative density of a new state (computeViscousDt /
+// computeCapillaryDt read state.rho, which cons->prim leaves as the
+// conservative density).  Launched after the last RK stage into the partial
+// slots after the stage kernel's fused acoustic minima, so that the next
+// dt_finalize takes the minimum over all three limits.
+template <typename T>
+struct StressDtArgs {
+    Grid g;
+    const T* rho;
+    const int* active;    // device flag; 0 skips (time loop finished)
+    double viscMu;        // > 0: viscous limit
+    double capSigma;      // > 0: capillary limit
+    double* partial;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) stress_dt_kernel(StressDtArgs<T> A) {
+    if (A.active && *A.active == 0) return;
+    __shared__ double sh[kRedThreads / kWave];
+    const Grid& g = A.g;
+    double m = 1.0e300;
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
+        const size_t id = cell_index(g, i, j, k);
+        const double h[3] = {g.dxw[i + g.ngx], g.dyw[j + g.ngy], g.dzw[k + g.ngz]};
+        const double rho = (double)A.rho[id];
+        double dc = 1.0e300;
+        if (A.viscMu > 0.0) dc = dmin_ref(dc, visc_cell_dt(h, g.dim, A.viscMu, rho));
+        if (A.capSigma > 0.0) dc = dmin_ref(dc, cap_cell_dt(h, g.dim, A.capSigma, rho));
+        m = dmin_ref(m, dc);
+    }
+    m = block_min(m, sh);
+    if (threadIdx.x == 0) A.partial[blockIdx.x] = m;
+}
+
 // Final time step of one step, on device:
 //   dt = constDt > 0 ? constDt : min(maxDt, cfl * min(partials))
 //   dt = min(dt, target) when target > 0       (ExplicitSolver::step,

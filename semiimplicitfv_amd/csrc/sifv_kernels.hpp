@@ -21,11 +21,13 @@ struct StageKernels {
 };
 
 // R = reconstruction radius (1, 2, 3), rs = Riemann solver (RS_*), nph = 0 or 2.
+// lean: no body force and no face stresses, so the 2D single-phase explicit
+// tables may hand out their SRC = false instantiations.
 StageKernels<double> stage_kernels_1d(int R, int rs, int nph, bool semi);
-StageKernels<double> stage_kernels_2d(int R, int rs, int nph, bool semi);
+StageKernels<double> stage_kernels_2d(int R, int rs, int nph, bool semi, bool lean);
 StageKernels<double> stage_kernels_3d(int R, int rs, int nph);
 StageKernels<double> stage_kernels_3d_semi(int R, int rs, int nph);
-StageKernels<float> stage_kernels_fp32(int dim, int R, int rs);
+StageKernels<float> stage_kernels_fp32(int dim, int R, int rs, bool lean);
 
 // IGR: pre-pass (rhs, diagonal, rho) per dimension / phase count, and the
 // wavefront Gauss-Seidel sweep per dimension.

@@ -105,8 +105,6 @@ struct StageParams {
     int bodyForce = 0;              // hasBodyForce
     const double* accel = nullptr;  // device-resident body-force acceleration of the step
     const double* visc = nullptr;   // viscous momentum/energy right-hand side (DIM+1 fields) or null
-    double viscMu = 0.0;            // > 0: the viscous limit joins the fused time-step reduction
-    double capSigma = 0.0;          // > 0: the capillary limit joins it too
 };
 
 // Viscous limit of one cell (computeViscousDt, src/RKTimeStepping.cpp:161-188),
@@ -373,7 +371,10 @@ __device__ __forceinline__ void accumulate(T* acc, const T* lo, const T* hi, T c
 
 // Sources and stage update of one owned cell; returns the acoustic cell dt
 // of the new state when requested.  acc holds the complete flux part.
-template <int DIM, int NPH, bool SEMI, typename T>
+// SRC = false: an instantiation for configurations without body force and
+// face stresses, which carries none of their code (P.bodyForce and P.visc
+// must then be off).
+template <int DIM, int NPH, bool SEMI, typename T, bool SRC = true>
 __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id, T* acc, const Cell<DIM, NPH>& cell,
                                               const double* h, double dt, bool wantDt, const T* Upre = nullptr,
                                               const T* U0pre = nullptr) {
@@ -384,7 +385,7 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
 #pragma unroll
     for (int ph = 0; ph < NPH; ++ph) acc[DIM + 2 + NPH + ph] = acc[DIM + 2 + NPH + ph] + (T)cell.a[ph] * acc[QD];
     if constexpr (SEMI) acc[QP] = acc[QP] + (T)cell.p * acc[QD];
-    if (P.bodyForce) {
+    if (SRC && P.bodyForce) {
         // body force on the stage state (src/ExplicitSolver.cpp:498-521,
         // src/SemiImplicitSolver.cpp:484-506)
         const double a[3] = {P.accel[0], P.accel[1], P.accel[2]};
@@ -397,7 +398,7 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
         }
         acc[DIM + 1] = (T)((double)acc[DIM + 1] + rho * work);
     }
-    if (P.visc) {
+    if (SRC && P.visc) {
         // viscous and capillary stress (src/ExplicitSolver.cpp:523-535)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) acc[1 + d] = (T)((double)acc[1 + d] + P.visc[(size_t)d * P.total + id]);
@@ -479,8 +480,6 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
     double dc = h[0] / (fabs(c.u[0]) + cs);
 #pragma unroll
     for (int d = 1; d < DIM; ++d) dc = dmin_ref(dc, h[d] / (fabs(c.u[d]) + cs));
-    if (P.viscMu > 0.0) dc = dmin_ref(dc, visc_cell_dt(h, DIM, P.viscMu, c.rho));
-    if (P.capSigma > 0.0) dc = dmin_ref(dc, cap_cell_dt(h, DIM, P.capSigma, c.rho));
     return dc;
 }
 
@@ -548,7 +547,7 @@ constexpr int stage_min_waves() {
 // ===========================================================================
 // x + y kernel (1D/2D full stage; 3D x-y part on one plane)
 // ===========================================================================
-template <int DIM, int R, int RS, int NPH, bool SEMI, int MODE, typename T>
+template <int DIM, int R, int RS, int NPH, bool SEMI, int MODE, typename T, bool SRC = true>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_min_waves<DIM, NPH, T>(), 8)))
 stage_xy_kernel(StageParams<T> P) {
     constexpr int NV = n_recon<DIM, NPH>();   // reconstructed variables
@@ -762,8 +761,9 @@ stage_xy_kernel(StageParams<T> P) {
             } else {
                 const double h[3] = {dxc, dyr, dzk};
                 double dc = (YDIR && PU)
-                                ? finish_cell<DIM, NPH, SEMI, T>(P, id, acc, cur, h, dt, P.dtPartial != nullptr, preU, preU0)
-                                : finish_cell<DIM, NPH, SEMI, T>(P, id, acc, cur, h, dt, P.dtPartial != nullptr);
+                                ? finish_cell<DIM, NPH, SEMI, T, SRC>(P, id, acc, cur, h, dt, P.dtPartial != nullptr, preU,
+                                                                      preU0)
+                                : finish_cell<DIM, NPH, SEMI, T, SRC>(P, id, acc, cur, h, dt, P.dtPartial != nullptr);
                 dtMin = dmin_ref(dtMin, dc);
             }
         }
