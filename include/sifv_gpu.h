@@ -153,8 +153,9 @@ int sifv_explicit_advance(sifv_handle* h, const sifv_config* cfg, int nsteps,
 
 /* ---- semi-implicit step ------------------------------------------------ */
 /* Replaces SemiImplicitSolver::step (src/SemiImplicitSolver.cpp:106-236)
- * with the Jacobi PressureSolver; *pressure_iters is
- * lastPressureIterations() of the last stage. */
+ * with the Jacobi PressureSolver, or the PCG one after
+ * sifv_set_pressure_solver; *pressure_iters is lastPressureIterations() of
+ * the last stage. */
 int sifv_semi_implicit_step(sifv_handle* h, const sifv_config* cfg, double targetDt,
                             double* dt_out, int* pressure_iters);
 
@@ -168,6 +169,28 @@ int sifv_jacobi_solve(sifv_handle* h, const double* rho, const double* rhoc2,
                       const double* rhs, double* pressure, double dt,
                       double tolerance, int maxIter, int* iters,
                       double* residual_history);
+
+/* Pressure solver kinds of the semi-implicit step. */
+#define SIFV_PS_JACOBI 0
+#define SIFV_PS_PCG 1
+
+/* Replaces HyprePressureSolver::solve (src/HyprePressureSolver.cpp:312-408):
+ * the same system as sifv_jacobi_solve, solved by conjugate gradients on
+ * its volume-weighted symmetric form with a diagonal preconditioner (the
+ * reference's StructPCG role, with a diagonal instead of a PFMG V-cycle);
+ * stops when ||rhs - A p||_2 <= tolerance ||rhs||_2 as the reference's
+ * two-norm setting does.  Only interior cells of pressure are written;
+ * *iters = iterations (0 when the initial guess passes, maxIter when not
+ * converged).  SIFV_ERR_UNSUPPORTED on a periodic axis whose ghost widths
+ * differ from the wrapped cells' (the system is not symmetric there). */
+int sifv_pcg_solve(sifv_handle* h, const double* rho, const double* rhoc2,
+                   const double* rhs, double* pressure, double dt,
+                   double tolerance, int maxIter, int* iters);
+
+/* Chooses the PressureSolver of sifv_semi_implicit_step (the reference's
+ * SemiImplicitSolver constructor argument, include/SemiImplicitSolver.hpp:24-31):
+ * SIFV_PS_JACOBI (default) or SIFV_PS_PCG (FP64 handles only). */
+int sifv_set_pressure_solver(sifv_handle* h, int kind);
 
 /* ---- ghost fill and time step ------------------------------------------ */
 /* Replaces RectilinearMesh::applyBoundaryConditions(state, varSet, halo)

@@ -74,6 +74,10 @@ def _lib(impl: str):
     if impl == "c":
         lib.or_jacobi.restype = i
         lib.or_jacobi.argtypes = [vp, pd, pd, pd, pd, d, d, i, pd]
+        lib.or_pcg.restype = i
+        lib.or_pcg.argtypes = [vp, pd, pd, pd, pd, d, d, i]
+        lib.or_set_pressure_solver.restype = None
+        lib.or_set_pressure_solver.argtypes = [vp, i]
         lib.or_recon.restype = d
         lib.or_recon.argtypes = [i, i, pd, d]
         lib.or_compute_rhs.restype = None
@@ -194,6 +198,17 @@ class CpuSolver:
                                       tol, maxIter, _abi.dptr(hist) if hist is not None else None)
         return self.lib.ref_jacobi(self.h, _abi.dptr(rho), _abi.dptr(rhoc2), _abi.dptr(rhs), _abi.dptr(p), dt, tol,
                                    maxIter)
+
+    def pcg(self, rho, rhoc2, rhs, p, dt, tol, maxIter):
+        """PCG pressure solve (oracle only): p updated in place (interior)."""
+        assert self.impl == "c"
+        return self.lib.or_pcg(self.h, _abi.dptr(rho), _abi.dptr(rhoc2), _abi.dptr(rhs), _abi.dptr(p), dt, tol,
+                               maxIter)
+
+    def set_pressure_solver(self, kind: int):
+        """Pressure solver of semi_step (oracle only): 0 Jacobi, 1 PCG."""
+        assert self.impl == "c"
+        self.lib.or_set_pressure_solver(self.h, kind)
 
     def compute_rhs(self, state, config):
         """Explicit RHS (oracle only); caller applies cons->prim + ghost fill first."""

@@ -104,6 +104,8 @@ struct or_solver {
     double* grad; /* 9 per cell */
     prim_t* faceL;
     prim_t* faceR;
+    int psolver;  /* pressure solver of the semi-implicit step: 0 Jacobi, 1 PCG */
+    double* cg[6]; /* PCG vectors r, z, p, q, d, s (lazily allocated) */
 };
 
 size_t or_total_cells(const or_solver* s) { return s->m.total; }
@@ -194,6 +196,7 @@ void or_destroy(or_solver* s) {
     free(s->pRhs);
     free(s->pressure_);
     free(s->pNew);
+    for (int q = 0; q < 6; ++q) free(s->cg[q]);
     free(s->grad);
     free(s->faceL);
     free(s->faceR);
@@ -1407,6 +1410,168 @@ int or_jacobi(or_solver* s, const double* rho, const double* rhoc2, const double
 }
 
 /* ------------------------------------------------------------------ */
+/* PCG pressure solve.  The system is the one every PressureSolver solves:
+ * (1 + c L) p = rhs with c = rhoc2 dt^2 and L the variable-coefficient
+ * Laplacian of pressureLaplacian (src/PressureLaplacian.cpp:6-60), with the
+ * face coefficients of non-periodic domain faces zeroed as
+ * HyprePressureSolver::assembleSystem does (src/HyprePressureSolver.cpp:161-290;
+ * the zero-gradient scalar ghosts of the Jacobi and Gauss-Seidel solvers give
+ * the same rows).  Each row scaled by V/c (V the cell volume):
+ *   S_ii = V/c + sum_f w_f,  S_i,nb(f) = -w_f,
+ *   w_f = area_f / (max(0.5 (rho_i + rho_nb), 1e-14) * 0.5 (h_i + h_nb)),
+ * which is V times the reference's row coefficient cL / cR up to rounding.
+ * Both rows of an interior face compute the same w_f, so S is symmetric
+ * (positive definite).  A periodic axis whose first ghost widths differ from
+ * the wrapped cells' widths beyond rounding (a stretched periodic axis: the mesh mirrors the
+ * boundary widths into the ghosts) leaves the two rows of the wrap face
+ * unequal, as in the reference's own matrix; conjugate gradients need the
+ * symmetric form, so such meshes are refused (return -1).  Conjugate gradients on S p = (V/c) rhs with the
+ * diagonal of S as preconditioner (the HYPRE_StructPCG role,
+ * src/HyprePressureSolver.cpp:119-138, with a diagonal instead of a PFMG
+ * V-cycle); stop when ||rhs - (1 + c L) p||_2 <= tol ||rhs||_2 (two-norm,
+ * relative to the right-hand side: src/HyprePressureSolver.cpp:350-355),
+ * compared as squares.  Returns the iteration count, 0 when the initial
+ * guess already passes, maxIter when not converged; rhs = 0 gives p = 0.
+ * Only interior cells of p are written.
+ * Sums run in the interior loop order (k, j, i). */
+/* ------------------------------------------------------------------ */
+static double pcg_row_w(const or_mesh* m, const double* rho, int dir, int side, int i, int j, int k) {
+    /* weight of the low (side 0) or high (side 1) face of cell (i,j,k) in the
+     * cell's own row, with the cell's own operand order as the reference's row */
+    const int n = dir == 0 ? m->nx : dir == 1 ? m->ny : m->nz;
+    const int c = dir == 0 ? i : dir == 1 ? j : k;
+    if (side == 0 && c == 0 && !m->wrap[2 * dir]) return 0.0;
+    if (side == 1 && c == n - 1 && !m->wrap[2 * dir + 1]) return 0.0;
+    const int sg = side ? 1 : -1;
+    const int ei = dir == 0 ? sg : 0, ej = dir == 1 ? sg : 0, ek = dir == 2 ? sg : 0;
+    const double rf = 0.5 * (rho[IDX(m, i, j, k)] + rho[IDX(m, i + ei, j + ej, k + ek)]);
+    double hf;
+    if (dir == 0) hf = side ? 0.5 * (DX(m, i) + DX(m, i + 1)) : 0.5 * (DX(m, i - 1) + DX(m, i));
+    else if (dir == 1) hf = side ? 0.5 * (DY(m, j) + DY(m, j + 1)) : 0.5 * (DY(m, j - 1) + DY(m, j));
+    else hf = side ? 0.5 * (DZ(m, k) + DZ(m, k + 1)) : 0.5 * (DZ(m, k - 1) + DZ(m, k));
+    const double area = dir == 0 ? DY(m, j) * DZ(m, k) : dir == 1 ? DX(m, i) * DZ(m, k) : DX(m, i) * DY(m, j);
+    return area / (dmax(rf, 1e-14) * hf);
+}
+
+/* out = S v over the interior (v's ghosts filled by the caller) */
+static void pcg_apply(const or_mesh* m, const double* rho, const double* d, const double* v, double* out) {
+    for (int k = 0; k < m->nz; ++k)
+        for (int j = 0; j < m->ny; ++j)
+            for (int i = 0; i < m->nx; ++i) {
+                const size_t id = IDX(m, i, j, k);
+                double acc = d[id] * v[id];
+                for (int dir = 0; dir < m->dim; ++dir) {
+                    const int ei = dir == 0, ej = dir == 1, ek = dir == 2;
+                    acc -= pcg_row_w(m, rho, dir, 0, i, j, k) * v[IDX(m, i - ei, j - ej, k - ek)];
+                    acc -= pcg_row_w(m, rho, dir, 1, i, j, k) * v[IDX(m, i + ei, j + ej, k + ek)];
+                }
+                out[id] = acc;
+            }
+}
+
+static int or_pcg_symmetric(const or_mesh* m) {
+    for (int dir = 0; dir < m->dim; ++dir) {
+        if (!m->wrap[2 * dir]) continue;
+        const int n = dir == 0 ? m->nx : dir == 1 ? m->ny : m->nz;
+        double lo, last, hi, first;
+        if (dir == 0) { lo = DX(m, -1); last = DX(m, n - 1); hi = DX(m, n); first = DX(m, 0); }
+        else if (dir == 1) { lo = DY(m, -1); last = DY(m, n - 1); hi = DY(m, n); first = DY(m, 0); }
+        else { lo = DZ(m, -1); last = DZ(m, n - 1); hi = DZ(m, n); first = DZ(m, 0); }
+        /* widths equal up to the rounding of the node coordinates */
+        if (fabs(lo - last) > 1e-10 * last || fabs(hi - first) > 1e-10 * first) return 0;
+    }
+    return 1;
+}
+
+int or_pcg(or_solver* s, const double* rho, const double* rhoc2, const double* rhs, double* x, double dt,
+           double tol, int maxIter) {
+    or_mesh* m = &s->m;
+    if (!or_pcg_symmetric(m)) {
+        snprintf(g_err, sizeof g_err, "PCG needs a symmetric system: a periodic axis with stretched wrap widths");
+        return -1;
+    }
+    for (int q = 0; q < 6; ++q)
+        if (!s->cg[q]) s->cg[q] = zalloc(m->total);
+    double *r = s->cg[0], *z = s->cg[1], *p = s->cg[2], *q = s->cg[3], *d = s->cg[4], *sc = s->cg[5];
+    const double dt2 = dt * dt;
+    const double tol2 = tol * tol;
+    double bb = 0.0;
+    for (int k = 0; k < m->nz; ++k)
+        for (int j = 0; j < m->ny; ++j)
+            for (int i = 0; i < m->nx; ++i) {
+                const size_t id = IDX(m, i, j, k);
+                sc[id] = VOL(m, i, j, k) / (rhoc2[id] * dt2);
+                double dd = sc[id];
+                for (int dir = 0; dir < m->dim; ++dir) {
+                    dd += pcg_row_w(m, rho, dir, 0, i, j, k);
+                    dd += pcg_row_w(m, rho, dir, 1, i, j, k);
+                }
+                d[id] = dd;
+                bb += rhs[id] * rhs[id];
+            }
+    if (bb == 0.0) {
+        for (int k = 0; k < m->nz; ++k)
+            for (int j = 0; j < m->ny; ++j)
+                for (int i = 0; i < m->nx; ++i) x[IDX(m, i, j, k)] = 0.0;
+        return 0;
+    }
+    /* the caller's ghosts stay as they are: the matrix product reads a copy */
+    memcpy(z, x, m->total * sizeof(double));
+    fill_scalar(m, z);
+    pcg_apply(m, rho, d, z, q);
+    double rr = 0.0, rz = 0.0;
+    for (int k = 0; k < m->nz; ++k)
+        for (int j = 0; j < m->ny; ++j)
+            for (int i = 0; i < m->nx; ++i) {
+                const size_t id = IDX(m, i, j, k);
+                r[id] = sc[id] * rhs[id] - q[id];
+                const double ro = r[id] / sc[id];
+                rr += ro * ro;
+                z[id] = r[id] / d[id];
+                rz += r[id] * z[id];
+                p[id] = z[id];
+            }
+    if (rr <= tol2 * bb) return 0;
+    for (int it = 0; it < maxIter; ++it) {
+        fill_scalar(m, p);
+        pcg_apply(m, rho, d, p, q);
+        double pq = 0.0;
+        for (int k = 0; k < m->nz; ++k)
+            for (int j = 0; j < m->ny; ++j)
+                for (int i = 0; i < m->nx; ++i) {
+                    const size_t id = IDX(m, i, j, k);
+                    pq += p[id] * q[id];
+                }
+        const double alpha = rz / pq;
+        double rzn = 0.0;
+        rr = 0.0;
+        for (int k = 0; k < m->nz; ++k)
+            for (int j = 0; j < m->ny; ++j)
+                for (int i = 0; i < m->nx; ++i) {
+                    const size_t id = IDX(m, i, j, k);
+                    x[id] += alpha * p[id];
+                    r[id] -= alpha * q[id];
+                    const double ro = r[id] / sc[id];
+                    rr += ro * ro;
+                    z[id] = r[id] / d[id];
+                    rzn += r[id] * z[id];
+                }
+        if (rr <= tol2 * bb) return it + 1;
+        const double beta = rzn / rz;
+        rz = rzn;
+        for (int k = 0; k < m->nz; ++k)
+            for (int j = 0; j < m->ny; ++j)
+                for (int i = 0; i < m->nx; ++i) {
+                    const size_t id = IDX(m, i, j, k);
+                    p[id] = z[id] + beta * p[id];
+                }
+    }
+    return maxIter;
+}
+
+void or_set_pressure_solver(or_solver* s, int kind) { s->psolver = kind; }
+
+/* ------------------------------------------------------------------ */
 /* semi-implicit step: SemiImplicitSolver::step, src/SemiImplicitSolver.cpp:106-236 */
 /* ------------------------------------------------------------------ */
 double or_semi_step(or_solver* s, double* const* f, const sifv_config* c, double targetDt,
@@ -1530,9 +1695,12 @@ double or_semi_step(or_solver* s, double* const* f, const sifv_config* c, double
                     s->pRhs[id] = s->pAdv[id] - s->rhoc2[id] * dt * s->divUstar[id];
                     s->pressure_[id] = f[SIFV_F_PRES][id];
                 }
-        lastIters = or_jacobi(s, f[SIFV_F_RHO], s->rhoc2, s->pRhs, s->pressure_, dt,
-                              c->semiImplicitParams.pressureTol,
-                              c->semiImplicitParams.maxPressureIters, NULL);
+        lastIters = s->psolver == 1
+                        ? or_pcg(s, f[SIFV_F_RHO], s->rhoc2, s->pRhs, s->pressure_, dt,
+                                 c->semiImplicitParams.pressureTol, c->semiImplicitParams.maxPressureIters)
+                        : or_jacobi(s, f[SIFV_F_RHO], s->rhoc2, s->pRhs, s->pressure_, dt,
+                                    c->semiImplicitParams.pressureTol, c->semiImplicitParams.maxPressureIters,
+                                    NULL);
         fill_scalar(m, s->pressure_);
         if (f[SIFV_F_SIGMA]) fill_scalar(m, f[SIFV_F_SIGMA]);
         for (int k = 0; k < m->nz; ++k)

@@ -33,6 +33,9 @@ double or_semi_step(or_solver* s, double* const* f, const sifv_config* cfg,
                     double targetDt, int* iters);
 int or_jacobi(or_solver* s, const double* rho, const double* rhoc2, const double* rhs,
               double* p, double dt, double tol, int maxIter, double* hist);
+int or_pcg(or_solver* s, const double* rho, const double* rhoc2, const double* rhs, double* p, double dt,
+           double tol, int maxIter);
+void or_set_pressure_solver(or_solver* s, int kind);
 void or_fill_ghosts(or_solver* s, double* const* f, int varset);
 void or_fill_scalar(or_solver* s, double* field);
 void or_cons_to_prim(or_solver* s, double* const* f, const sifv_config* cfg);

@@ -120,6 +120,8 @@ def _declare(lib):
         "sifv_explicit_advance": (i, [vp, P(Config_C), i, d, P(d), P(i)]),
         "sifv_semi_implicit_step": (i, [vp, P(Config_C), d, P(d), P(i)]),
         "sifv_jacobi_solve": (i, [vp, P(d), P(d), P(d), P(d), d, d, i, P(i), P(d)]),
+        "sifv_pcg_solve": (i, [vp, P(d), P(d), P(d), P(d), d, d, i, P(i)]),
+        "sifv_set_pressure_solver": (i, [vp, i]),
         "sifv_fill_ghosts": (i, [vp, i]),
         "sifv_fill_scalar_ghosts": (i, [vp, i]),
         "sifv_acoustic_dt": (i, [vp, d, d, P(d)]),

@@ -23,6 +23,7 @@
 #include "sifv_kernels.hpp"
 #include "sifv_semi.hpp"
 #include "sifv_adjoint.hpp"
+#include "sifv_pcg.hpp"
 
 using namespace sifv;
 
@@ -197,6 +198,13 @@ struct sifv_handle {
     bool sigPrim = false;
     bool unifY = false;       // all interior y (and z) widths are the same double
     bool jacMixed = false;    // semi-implicit pressure solve: FP32 sweeps + FP64 refinement
+    int psolver = SIFV_PS_JACOBI;  // pressure solver of the semi-implicit step
+    bool pcgSym = true;       // no periodic axis with stretched wrap widths (PCG usable)
+    double* pcgv[7] = {};     // PCG vectors r z p q d s and the partial sums
+    double* pcgSc = nullptr;  // PCG device scalars
+    int* pcgSt = nullptr;     // PCG device flags
+    int* pcgStHost = nullptr; // pinned copy of the flags
+    int pcgNblk = 0;
     double* jacPn = nullptr;  // FP64 Jacobi update of the refined iterate
     float* jac32 = nullptr;   // rho, co, r, d0, d1 (FP32)
     // discrete adjoint: checkpoints of the forward run and work arrays
@@ -919,6 +927,88 @@ int run_jacobi_mixed(sifv_handle* h, const double* rho, const double* dt, double
     return SIFV_OK;
 }
 
+// PCG pressure solve (sifv_pcg.hpp) of the system on rho / jac[2] (rhoc2) /
+// jac[3] (rhs) with the device time step dt, x = jac[0] in place (initial
+// guess in, answer out; interior cells only).  *n = iterations (0 when the
+// guess already passes, maxIter when not converged).
+int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int maxIter, int* n) {
+    if (!h->pcgSym)
+        return fail(SIFV_ERR_UNSUPPORTED,
+                    "PCG needs a symmetric system: a periodic axis with stretched wrap widths (use Jacobi)");
+    const int nb = elementwise_blocks(h);
+    if (!h->pcgv[0] || h->pcgNblk < nb) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int q = 0; q < 7; ++q) {
+            hipFree(h->pcgv[q]);
+            h->pcgv[q] = nullptr;
+        }
+        for (int q = 0; q < 6; ++q) {
+            HIPCHK(hipMalloc(&h->pcgv[q], sizeof(double) * h->total));
+            HIPCHK(hipMemset(h->pcgv[q], 0, sizeof(double) * h->total));
+        }
+        HIPCHK(hipMalloc(&h->pcgv[6], sizeof(double) * 3 * nb));
+        if (!h->pcgSc) HIPCHK(hipMalloc(&h->pcgSc, sizeof(double) * 8));
+        if (!h->pcgSt) HIPCHK(hipMalloc(&h->pcgSt, sizeof(int) * 4));
+        if (!h->pcgStHost) HIPCHK(hipHostMalloc(&h->pcgStHost, sizeof(int) * 4, hipHostMallocDefault));
+        h->pcgNblk = nb;
+    }
+    PcgArgs A;
+    A.g = h->grid();
+    A.rho = rho;
+    A.rhoc2 = h->jac[2];
+    A.rhs = h->jac[3];
+    A.dt = dt;
+    A.x = h->jac[0];
+    A.r = h->pcgv[0];
+    A.z = h->pcgv[1];
+    A.p = h->pcgv[2];
+    A.q = h->pcgv[3];
+    A.d = h->pcgv[4];
+    A.s = h->pcgv[5];
+    A.part = h->pcgv[6];
+    A.nblk = nb;
+    A.sc = h->pcgSc;
+    A.st = h->pcgSt;
+    A.tol2 = tol * tol;
+    A.maxIter = maxIter;
+    const dim3 blk(256);
+    void* args[] = {&A};
+    auto launch = [&](const void* fn, int grid) {
+        return timed_launch(h, fn, dim3(grid), blk, args);
+    };
+    const void* setup = h->dim == 1 ? reinterpret_cast<const void*>(pcg_setup_kernel<1>)
+                        : h->dim == 2 ? reinterpret_cast<const void*>(pcg_setup_kernel<2>)
+                                      : reinterpret_cast<const void*>(pcg_setup_kernel<3>);
+    const void* matvec = h->dim == 1 ? reinterpret_cast<const void*>(pcg_matvec_kernel<1>)
+                         : h->dim == 2 ? reinterpret_cast<const void*>(pcg_matvec_kernel<2>)
+                                       : reinterpret_cast<const void*>(pcg_matvec_kernel<3>);
+    int rc = launch(setup, nb);
+    if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_setup_fold_kernel), 1);
+    if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_zero_kernel), nb);
+    if (rc) return rc;
+    int it = 0, chunk = 4;
+    *n = maxIter;
+    while (true) {
+        HIPCHK(hipMemcpyAsync(h->pcgStHost, h->pcgSt, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (h->pcgStHost[0]) {
+            *n = h->pcgStHost[1];
+            return SIFV_OK;
+        }
+        if (it >= maxIter) return fail(SIFV_ERR_HIP, "PCG iteration count out of step with the device");
+        const int end = std::min(maxIter, it + chunk);
+        for (; it < end; ++it) {
+            rc = launch(matvec, nb);
+            if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_alpha_kernel), 1);
+            if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_update_kernel), nb);
+            if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_beta_kernel), 1);
+            if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_dir_kernel), nb);
+            if (rc) return rc;
+        }
+        chunk = std::min(chunk * 2, 64);
+    }
+}
+
 // One semi-implicit step (SemiImplicitSolver::step, src/SemiImplicitSolver.cpp:106-236):
 // advective dt, then per RK stage the fused advective stage kernel (star state,
 // advected pressure), PRIM ghosts of the star density / velocity, the pressure
@@ -988,7 +1078,15 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
         HIPCHK(hipGetLastError());
         int n = 0;
         const double* pfinal = nullptr;
-        if (h->jacMixed) {
+        if (h->psolver == SIFV_PS_PCG) {
+            rc = run_pcg(h, rho, h->scal, sp.pressureTol, sp.maxPressureIters, &n);
+            if (rc) return rc;
+            // fillScalarGhosts(pressure_) after the solve (src/SemiImplicitSolver.cpp:581)
+            double* pf[1] = {h->jac[0]};
+            int pc[1] = {0};
+            rc = fill_dirs(h, pf, pc, 1, true);
+            pfinal = h->jac[0];
+        } else if (h->jacMixed) {
             rc = run_jacobi_mixed(h, rho, h->scal, sp.pressureTol, sp.maxPressureIters, &n, &pfinal);
         } else {
             rc = run_jacobi(h, rho, h->scal, sp.pressureTol, sp.maxPressureIters, &n);
@@ -1352,6 +1450,18 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
     for (int i = 0; i < h->nxT; ++i) dx[i] = xe[i + 1] - xe[i];
     for (int j = 0; j < h->nyT; ++j) dy[j] = ye[j + 1] - ye[j];
     for (int k = 0; k < h->nzT; ++k) dz[k] = ze[k + 1] - ze[k];
+    // PCG needs the symmetric form: a periodic axis's first ghost widths must
+    // match the wrapped cells' widths up to the rounding of the nodes
+    {
+        const std::vector<double>* w[3] = {&dx, &dy, &dz};
+        const int n[3] = {h->nx, h->ny, h->nz}, ng[3] = {h->ngx, h->ngy, h->ngz};
+        for (int d = 0; d < h->dim; ++d) {
+            if (h->bc[2 * d] != BK_WRAP) continue;
+            const double lo = (*w[d])[ng[d] - 1], last = (*w[d])[ng[d] + n[d] - 1];
+            const double hi = (*w[d])[ng[d] + n[d]], first = (*w[d])[ng[d]];
+            if (std::fabs(lo - last) > 1e-10 * last || std::fabs(hi - first) > 1e-10 * first) h->pcgSym = false;
+        }
+    }
     h->unifY = true;
     for (int j = 1; j < h->ny; ++j) h->unifY = h->unifY && dy[h->ngy + j] == dy[h->ngy];
     for (int k = 1; k < h->nz; ++k) h->unifY = h->unifY && dz[h->ngz + k] == dz[h->ngz];
@@ -1431,6 +1541,10 @@ int sifv_destroy(sifv_handle* h) {
     hipFree(h->flags);
     hipFree(h->part);
     hipFree(h->visc);
+    for (int q = 0; q < 7; ++q) hipFree(h->pcgv[q]);
+    hipFree(h->pcgSc);
+    hipFree(h->pcgSt);
+    if (h->pcgStHost) hipHostFree(h->pcgStHost);
     hipFree(h->jacPn);
     hipFree(h->jac32);
     for (double* c : h->adjCk) hipFree(c);
@@ -1959,6 +2073,41 @@ int sifv_jacobi_solve(sifv_handle* h, const double* rho, const double* rhoc2, co
     if (iters) *iters = n;
     if (residual_history)
         for (int t = 0; t < n; ++t) residual_history[t] = h->hostResid[t];
+    return SIFV_OK;
+}
+
+int sifv_set_pressure_solver(sifv_handle* h, int kind) {
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    if (kind != SIFV_PS_JACOBI && kind != SIFV_PS_PCG) return fail(SIFV_ERR_INVALID, "unknown pressure solver");
+    if (kind == SIFV_PS_PCG && !h->pcgSym)
+        return fail(SIFV_ERR_UNSUPPORTED,
+                    "PCG needs a symmetric system: a periodic axis with stretched wrap widths (use Jacobi)");
+    if (kind == SIFV_PS_PCG && h->jacMixed)
+        return fail(SIFV_ERR_UNSUPPORTED, "the PCG pressure solve runs in FP64 (no mixed-precision form)");
+    h->psolver = kind;
+    return SIFV_OK;
+}
+
+int sifv_pcg_solve(sifv_handle* h, const double* rho, const double* rhoc2, const double* rhs, double* pressure,
+                   double dt, double tolerance, int maxIter, int* iters) {
+    if (!h || !rho || !rhoc2 || !rhs || !pressure) return fail(SIFV_ERR_INVALID, "null argument");
+    if (maxIter <= 0) return fail(SIFV_ERR_INVALID, "maxIter must be > 0");
+    if (!(tolerance > 0.0)) return fail(SIFV_ERR_INVALID, "tolerance must be > 0");
+    int rc = ensure_jacobi(h, 1);
+    if (rc) return rc;
+    const size_t nb = h->total * sizeof(double);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(h->jac[5], rho, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->jac[2], rhoc2, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->jac[3], rhs, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->jac[0], pressure, nb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->scal + 3, &dt, sizeof(double), hipMemcpyHostToDevice));
+    int n = 0;
+    rc = run_pcg(h, h->jac[5], h->scal + 3, tolerance, maxIter, &n);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(pressure, h->jac[0], nb, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (iters) *iters = n;
     return SIFV_OK;
 }
 

@@ -178,13 +178,26 @@ class ExplicitSolver(_GpuSolverBase):
 
 
 class SemiImplicitSolver(_GpuSolverBase):
-    """SemiImplicitSolver (include/SemiImplicitSolver.hpp:22-96) with the
-    Jacobi pressure solver, on the GPU."""
+    """SemiImplicitSolver (include/SemiImplicitSolver.hpp:22-96) on the GPU,
+    with the Jacobi or the PCG pressure solver (chosen by the pressure
+    solver's name(), as the reference dispatches on the object passed in)."""
+
+    _PS = {"Jacobi": 0, "PCG": 1}
 
     def __init__(self, mesh, riemannSolver, pressureSolver, eos, igrSolver, config, precision="fp64"):
         super().__init__(mesh, riemannSolver, eos, igrSolver, config, precision)
         self.pressureSolver_ = pressureSolver
         self.lastIters_ = 0
+        name = pressureSolver.name() if pressureSolver is not None else "Jacobi"
+        if name not in self._PS:
+            self.close()
+            raise ValueError(f"no GPU pressure solver named {name!r}")
+        if self._PS[name]:
+            try:
+                _abi.check(self._lib.sifv_set_pressure_solver(self._h, self._PS[name]))
+            except Exception:
+                self.close()
+                raise
 
     def step(self, config, mesh, state, targetDt=-1.0):
         self.upload(state)
@@ -226,4 +239,29 @@ class JacobiPressureSolver:
         _abi.check(lib.sifv_jacobi_solve(self.gpu._h, _abi.dptr(arrs[0]), _abi.dptr(arrs[1]), _abi.dptr(arrs[2]),
                                          _abi.dptr(pressure), float(dt), float(tolerance), int(maxIter),
                                          C.byref(it), _abi.dptr(hist) if hist is not None else None))
+        return it.value
+
+
+class PCGPressureSolver:
+    """PressureSolver::solve (include/PressureSolver.hpp:17-40) in the role of
+    HyprePressureSolver (src/HyprePressureSolver.cpp:312-408): conjugate
+    gradients on the GPU (sifv_pcg_solve), stopping on the relative two-norm
+    of the residual.  Needs a handle for the mesh (any solver built on the same
+    mesh) for standalone solves; passed to SemiImplicitSolver it selects the
+    PCG solve of the semi-implicit step."""
+
+    def __init__(self, gpu_solver: _GpuSolverBase | None = None):
+        self.gpu = gpu_solver
+
+    def name(self):
+        return "PCG"
+
+    def solve(self, mesh, rho, rhoc2, rhs, pressure, dt, tolerance, maxIter):
+        lib = _abi.load_library()
+        it = C.c_int()
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, rhoc2, rhs)]
+        if not (pressure.flags.c_contiguous and pressure.dtype == np.float64):
+            raise ValueError("pressure must be a contiguous float64 array (updated in place)")
+        _abi.check(lib.sifv_pcg_solve(self.gpu._h, _abi.dptr(arrs[0]), _abi.dptr(arrs[1]), _abi.dptr(arrs[2]),
+                                      _abi.dptr(pressure), float(dt), float(tolerance), int(maxIter), C.byref(it)))
         return it.value

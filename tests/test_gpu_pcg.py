@@ -1,0 +1,128 @@
+"""PCG pressure solve on the GPU (sifv_pcg_solve, and the semi-implicit
+step with SIFV_PS_PCG) against the oracle's PCG (oracle/sifv_oracle.c::or_pcg,
+itself pinned against a direct solve of an independently assembled system).
+
+1. Standalone solves: 2D stretched meshes (walls, or a uniform periodic
+   axis) at a mild and a stiff time step, and the 3D 1000:1 density-jump
+   system: the iteration count equals the oracle's (+-1: the dot products sum
+   in another order), the answer meets ||rhs - A p|| <= tol ||rhs|| in the
+   independent assembly, matches the oracle's answer to the scale of the
+   tolerance, and leaves the ghost cells as passed in.
+2. Semi-implicit steps with the PCG solve (12^3 water/air bubble, 3 steps):
+   pressure iterations equal the oracle's (+-1) and fewer than the Jacobi
+   step's, fields within 1e-8 of the oracle's PCG step (relative L1); the
+   oracle's PCG step equals its Jacobi step at a tight tolerance
+   (tests/test_oracle_pins.py::test_semi_step_pcg_matches_jacobi).
+3. A stretched periodic axis (non-symmetric rows at the wrap face) and the
+   FP32 pressure form are refused.
+"""
+import numpy as np
+import pytest
+
+from semiimplicitfv_amd import (HLLCSolver, IdealGasEOS, JacobiPressureSolver, PCGPressureSolver,
+                                SemiImplicitSolver)
+from semiimplicitfv_amd._abi import SifvError
+from tests.golden import problems
+from tests.helpers import rel_errs
+from tests.test_oracle_pins import _direct_system, _pressure_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_pcg(cfg, mesh):
+    eos = IdealGasEOS(1.4, 1.0, cfg)
+    g = SemiImplicitSolver(mesh, HLLCSolver(eos), PCGPressureSolver(), eos, None, cfg)
+    return g, PCGPressureSolver(g), eos
+
+
+@pytest.mark.parametrize("dt", [0.3, 30.0])
+@pytest.mark.parametrize("periodic_y", [True, False])
+def test_pcg_standalone_2d(gpu, oracle_c, dt, periodic_y):
+    cfg, mesh, rho, rhoc2, rhs = _pressure_problem(seed=11, nx=23, ny=17, periods=(0, 1 if periodic_y else 0, 0),
+                                                   uniform_y=periodic_y)
+    tol = 1e-10 if dt < 1.0 else 1e-8
+    g, ps, eos = _gpu_pcg(cfg, mesh)
+    pg = np.zeros(mesh.totalCells())
+    pg[:] = np.arange(pg.size) * 1e-3     # ghost cells must survive untouched
+    p0 = pg.copy()
+    ig = ps.solve(mesh, rho, rhoc2, rhs, pg, dt, tol, 2000)
+    g.close()
+    cpu = oracle_c.CpuSolver(mesh, cfg, 2, eos, "c")
+    pc = p0.copy()
+    ic = cpu.pcg(rho, rhoc2, rhs, pc, dt, tol, 2000)
+    assert 0 < ig < 2000 and abs(ig - ic) <= 1, (ig, ic)
+    ghost = np.ones(mesh.totalCells(), dtype=bool)
+    mesh.interior(ghost)[...] = False
+    assert np.array_equal(pg[ghost], p0[ghost])
+    A, b = _direct_system(mesh, rho, rhoc2, rhs, dt, periodic_y)
+    got = mesh.interior(pg)[0].reshape(-1)
+    assert np.linalg.norm(b - A @ got) <= tol * np.linalg.norm(b) * 1.01
+    ref = mesh.interior(pc)[0].reshape(-1)
+    assert np.abs(got - ref).max() <= 1e3 * tol * np.abs(ref).max()
+
+
+def test_pcg_standalone_3d(gpu, oracle_c):
+    """The 3D 1000:1 density-jump system (slip walls, y periodic)."""
+    cfg, mesh, rho, rhoc2, rhs, p0, dt = problems.jacobi_problem()
+    tol = 1e-9
+    g, ps, eos = _gpu_pcg(cfg, mesh)
+    pg = p0.copy()
+    ig = ps.solve(mesh, rho, rhoc2, rhs, pg, dt, tol, 5000)
+    g.close()
+    cpu = oracle_c.CpuSolver(mesh, cfg, 2, eos, "c")
+    pc = p0.copy()
+    ic = cpu.pcg(rho, rhoc2, rhs, pc, dt, tol, 5000)
+    assert 0 < ig < 5000 and abs(ig - ic) <= 1, (ig, ic)
+    l1, li = rel_errs(mesh.interior(pg), mesh.interior(pc))
+    assert l1 <= 1e-9 and li <= 1e-8, (l1, li)
+    # the converged answer also solves the Jacobi fixed point to its precision
+    pj = p0.copy()
+    cpu.jacobi(rho, rhoc2, rhs, pj, dt, 1e-9, 200000)
+    l1, li = rel_errs(mesh.interior(pg), mesh.interior(pj))
+    assert l1 <= 1e-8 and li <= 1e-7, (l1, li)
+
+
+def test_pcg_semi_implicit_steps(gpu, oracle_c):
+    runs = {}
+    for kind in ("gpu", "oracle", "jacobi"):
+        cfg, mesh, st, eos, rs = problems.bubble_3d(12, perturb=True)
+        cfg.semiImplicitParams.pressureTol = 1e-10
+        its = []
+        if kind == "gpu":
+            g = SemiImplicitSolver(mesh, rs, PCGPressureSolver(), eos, None, cfg)
+            g.upload(st)
+            for k in range(3):
+                cfg.step = k
+                g.step_resident(cfg)
+                its.append(g.lastPressureIterations())
+            g.download(st)
+            g.close()
+        else:
+            s = oracle_c.CpuSolver(mesh, cfg, rs.kind(), eos, "c")
+            s.set_pressure_solver(1 if kind == "oracle" else 0)
+            for k in range(3):
+                cfg.step = k
+                its.append(s.semi_step(st, cfg)[1])
+        runs[kind] = (mesh, st, its)
+    mesh, sg, ig = runs["gpu"]
+    _, so, io = runs["oracle"]
+    _, _, ij = runs["jacobi"]
+    assert all(abs(a - b) <= 1 for a, b in zip(ig, io)), (ig, io)
+    assert max(ig) < min(ij), (ig, ij)
+    for fid in sg.field_ids():
+        l1, li = rel_errs(mesh.interior(sg.field(fid)), mesh.interior(so.field(fid)))
+        assert l1 <= 1e-8, (fid, l1, li)
+
+
+def test_pcg_refusals(gpu):
+    cfg, mesh, rho, rhoc2, rhs = _pressure_problem(seed=5, nx=9, ny=7)   # stretched periodic y
+    eos = IdealGasEOS(1.4, 1.0, cfg)
+    with pytest.raises(SifvError):
+        SemiImplicitSolver(mesh, HLLCSolver(eos), PCGPressureSolver(), eos, None, cfg)
+    g = SemiImplicitSolver(mesh, HLLCSolver(eos), JacobiPressureSolver(), eos, None, cfg)
+    with pytest.raises(SifvError):
+        PCGPressureSolver(g).solve(mesh, rho, rhoc2, rhs, np.zeros(mesh.totalCells()), 1.0, 1e-6, 100)
+    g.close()
+    cfg, mesh, st, eos, rs = problems.bubble_3d(12)
+    with pytest.raises(SifvError):
+        SemiImplicitSolver(mesh, rs, PCGPressureSolver(), eos, None, cfg, precision="mixed")
