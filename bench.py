@@ -283,7 +283,8 @@ def bench_adjoint(n: int = 512, nsteps: int = 400, every: int = 20, ndir: int = 
                     "directions can cross one; the adjoint itself matches exact tangents to 1e-9 in the tests"}
 
 
-def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred: bool = False):
+def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred: bool = False,
+                 solver: str = "jacobi"):
     """3D water/air bubble (unit cube, n^3 cells, air sphere r = 0.2, alpha
     tanh over 2 cells, alphaMin 1e-8, slip walls, p = 1e5 at rest), semi-
     implicit SSP-RK3 with the Jacobi pressure solve (tol 1e-6, <= 1000
@@ -293,12 +294,14 @@ def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred:
     Jacobi sweeps refined by FP64 residual passes.  stirred: the velocity
     field of the 12^3 bubble fixture (u = 2 sin(pi x) cos(pi y), v = -2 cos(pi x)
     sin(pi y), w = 0.5 sin(pi z)), which makes every pressure solve run to its
-    1000-sweep cap."""
+    1000-sweep cap.  solver "pcg": the conjugate-gradient pressure solve
+    (stop on ||rhs - A p||_2 <= 1e-6 ||rhs||_2, <= 1000 iterations) instead of
+    Jacobi (stop on max |p_new - p| < 1e-6)."""
     import numpy as np
 
-    from semiimplicitfv_amd import (BoundaryCondition, HLLCSolver, JacobiPressureSolver, PhaseEOS,
-                                    ReconstructionOrder, SemiImplicitSolver, SimulationConfig, StiffenedGasEOS, _abi,
-                                    uniform_mesh)
+    from semiimplicitfv_amd import (BoundaryCondition, HLLCSolver, JacobiPressureSolver, PCGPressureSolver,
+                                    PhaseEOS, ReconstructionOrder, SemiImplicitSolver, SimulationConfig,
+                                    StiffenedGasEOS, _abi, uniform_mesh)
 
     WATER, AIR = PhaseEOS(4.4, 6.0e8), PhaseEOS(1.4, 0.0)
 
@@ -315,7 +318,8 @@ def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred:
         mesh.setBoundaryCondition(f, BoundaryCondition.SlipWall)
     eos = StiffenedGasEOS(4.4, 6.0e8, 1.0, cfg)
     t0 = time.perf_counter()
-    g = SemiImplicitSolver(mesh, HLLCSolver(eos), JacobiPressureSolver(), eos, None, cfg, precision=precision)
+    ps = PCGPressureSolver() if solver == "pcg" else JacobiPressureSolver()
+    g = SemiImplicitSolver(mesh, HLLCSolver(eos), ps, eos, None, cfg, precision=precision)
     xc, yc, zc = mesh.centroids()
     r = np.sqrt((xc[None, None, :] - 0.5) ** 2 + (yc[None, :, None] - 0.5) ** 2 + (zc[:, None, None] - 0.5) ** 2)
     amin = cfg.multiPhaseParams.alphaMin
@@ -363,10 +367,11 @@ def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred:
     pres = mesh.interior(g.download_field(_abi.F_PRES)).copy()
     finite = bool(np.isfinite(pres).all())
     g.close()
-    return {"workload": f"3D water/air bubble {n}^3, semi-implicit SSP-RK3 + Jacobi (tol 1e-6), WENO5+HLLC, FP64 "
+    solve = "PCG (relative residual 1e-6)" if solver == "pcg" else "Jacobi (tol 1e-6)"
+    return {"workload": f"3D water/air bubble {n}^3, semi-implicit SSP-RK3 + {solve}, WENO5+HLLC, FP64 "
                         + ("state, FP32 Jacobi sweeps + FP64 refinement" if precision == "mixed" else "throughout"),
             "value": n ** 3 * steps / el, "unit": "cell-updates/s", "ms_per_step": el / steps * 1e3,
-            "jacobi_iterations_last_stage": iters, "host_setup_s": setup_s, "finite": finite, "_pres": pres}
+            "pressure_iterations_last_stage": iters, "host_setup_s": setup_s, "finite": finite, "_pres": pres}
 
 
 def measured_traffic(n, precision):
@@ -487,13 +492,16 @@ def main():
         for tag, n, steps, stirred in (("at_rest", a.bubble_n, 3, False), ("stirred_256", 256, 1, True)):
             r64 = bench_bubble(n, steps, precision="fp64", stirred=stirred)
             r32 = bench_bubble(n, steps, precision="mixed", stirred=stirred)
-            p64, p32 = r64.pop("_pres"), r32.pop("_pres")
+            rcg = bench_bubble(n, steps, precision="fp64", stirred=stirred, solver="pcg")
+            p64, p32, pcg = r64.pop("_pres"), r32.pop("_pres"), rcg.pop("_pres")
             r32["pres_rel_L1_vs_fp64"] = float(np.abs(p32 - p64).sum() / np.abs(p64).sum())
             r32["ratio_vs_fp64"] = r32["value"] / r64["value"]
+            rcg["pres_rel_L1_vs_jacobi"] = float(np.abs(pcg - p64).sum() / np.abs(p64).sum())
+            rcg["ratio_vs_jacobi"] = rcg["value"] / r64["value"]
             if stirred:
                 r32["note"] = ("both solves stop at the 1000-sweep cap unconverged, so pres_rel_L1_vs_fp64 "
                                "compares two different partial iterates, not FP32 rounding")
-            out[tag] = {"fp64": r64, "fp32_sweeps": r32}
+            out[tag] = {"fp64": r64, "fp32_sweeps": r32, "pcg": rcg}
         return out
 
     for name, fn in (("ensemble", bench_ensemble), ("adjoint", bench_adjoint), ("bubble", bubble_pair),

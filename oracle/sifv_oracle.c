@@ -1431,7 +1431,8 @@ int or_jacobi(or_solver* s, const double* rho, const double* rhoc2, const double
  * V-cycle); stop when ||rhs - (1 + c L) p||_2 <= tol ||rhs||_2 (two-norm,
  * relative to the right-hand side: src/HyprePressureSolver.cpp:350-355),
  * compared as squares.  Returns the iteration count, 0 when the initial
- * guess already passes, maxIter when not converged; rhs = 0 gives p = 0.
+ * guess already passes, maxIter when not converged (or when p.Sp stops being
+ * positive); rhs = 0 gives p = 0.
  * Only interior cells of p are written.
  * Sums run in the interior loop order (k, j, i). */
 /* ------------------------------------------------------------------ */
@@ -1453,17 +1454,22 @@ static double pcg_row_w(const or_mesh* m, const double* rho, int dir, int side, 
     return area / (dmax(rf, 1e-14) * hf);
 }
 
-/* out = S v over the interior (v's ghosts filled by the caller) */
-static void pcg_apply(const or_mesh* m, const double* rho, const double* d, const double* v, double* out) {
+/* out = S v over the interior (v's ghosts filled by the caller), as
+ * (V/c) v_i + sum_f w_f (v_i - v_nb): the differences of neighbouring values
+ * are formed first, so a smooth v (a pressure ~1e5 with small variations)
+ * loses nothing to the cancellation that d v_i - sum w v_nb would suffer when
+ * the face terms outweigh V/c by many orders (fine meshes, large dt). */
+static void pcg_apply(const or_mesh* m, const double* rho, const double* sc, const double* v, double* out) {
     for (int k = 0; k < m->nz; ++k)
         for (int j = 0; j < m->ny; ++j)
             for (int i = 0; i < m->nx; ++i) {
                 const size_t id = IDX(m, i, j, k);
-                double acc = d[id] * v[id];
+                const double vi = v[id];
+                double acc = sc[id] * vi;
                 for (int dir = 0; dir < m->dim; ++dir) {
                     const int ei = dir == 0, ej = dir == 1, ek = dir == 2;
-                    acc -= pcg_row_w(m, rho, dir, 0, i, j, k) * v[IDX(m, i - ei, j - ej, k - ek)];
-                    acc -= pcg_row_w(m, rho, dir, 1, i, j, k) * v[IDX(m, i + ei, j + ej, k + ek)];
+                    acc += pcg_row_w(m, rho, dir, 0, i, j, k) * (vi - v[IDX(m, i - ei, j - ej, k - ek)]);
+                    acc += pcg_row_w(m, rho, dir, 1, i, j, k) * (vi - v[IDX(m, i + ei, j + ej, k + ek)]);
                 }
                 out[id] = acc;
             }
@@ -1518,7 +1524,7 @@ int or_pcg(or_solver* s, const double* rho, const double* rhoc2, const double* r
     /* the caller's ghosts stay as they are: the matrix product reads a copy */
     memcpy(z, x, m->total * sizeof(double));
     fill_scalar(m, z);
-    pcg_apply(m, rho, d, z, q);
+    pcg_apply(m, rho, sc, z, q);
     double rr = 0.0, rz = 0.0;
     for (int k = 0; k < m->nz; ++k)
         for (int j = 0; j < m->ny; ++j)
@@ -1534,7 +1540,7 @@ int or_pcg(or_solver* s, const double* rho, const double* rhoc2, const double* r
     if (rr <= tol2 * bb) return 0;
     for (int it = 0; it < maxIter; ++it) {
         fill_scalar(m, p);
-        pcg_apply(m, rho, d, p, q);
+        pcg_apply(m, rho, sc, p, q);
         double pq = 0.0;
         for (int k = 0; k < m->nz; ++k)
             for (int j = 0; j < m->ny; ++j)
@@ -1542,6 +1548,9 @@ int or_pcg(or_solver* s, const double* rho, const double* rhoc2, const double* r
                     const size_t id = IDX(m, i, j, k);
                     pq += p[id] * q[id];
                 }
+        /* breakdown (p.Sp not positive: rounding has exhausted the search
+         * directions): stop, reported as not converged */
+        if (!(pq > 0.0)) return maxIter;
         const double alpha = rz / pq;
         double rzn = 0.0;
         rr = 0.0;

@@ -26,7 +26,7 @@
 // iteration: the host enqueues iterations in chunks and polls the flag.
 //
 // Per iteration, per interior cell: pcg_matvec (read p with its 2*DIM
-// neighbours, rho likewise, d; write q: 32 B), pcg_update (read x p r q d s,
+// neighbours, rho likewise, s; write q: 32 B), pcg_update (read x p r q d s,
 // write x r z: 72 B), pcg_dir (read z p, write p: 24 B) -- 128 B/cell,
 // HBM-bound.
 #pragma once
@@ -86,18 +86,21 @@ __device__ __forceinline__ size_t pcg_face(const PcgArgs& A, int dir, int side, 
     return cell_index(g, wi, wj, wk);
 }
 
-// (S v)_i with the row's face weights; the diagonal d_i given.
+// (S v)_i = (V/c)_i v_i + sum_f w_f (v_i - v_nb): neighbour differences first,
+// so a smooth v (pressures ~1e5 with small variations) keeps its precision
+// where the face terms outweigh V/c by many orders (fine meshes, large dt).
 template <int DIM>
-__device__ __forceinline__ double pcg_row(const PcgArgs& A, const double* v, double di, int i, int j, int k,
+__device__ __forceinline__ double pcg_row(const PcgArgs& A, const double* v, double si, int i, int j, int k,
                                           size_t id) {
-    double acc = di * v[id];
+    const double vi = v[id];
+    double acc = si * vi;
 #pragma unroll
     for (int dir = 0; dir < DIM; ++dir) {
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             double w;
             const size_t nb = pcg_face<DIM>(A, dir, side, i, j, k, w);
-            acc -= w * v[nb];
+            acc += w * (vi - v[nb]);
         }
     }
     return acc;
@@ -162,7 +165,7 @@ __global__ void __launch_bounds__(256) pcg_setup_kernel(PcgArgs A) {
         A.d[id] = dd;
         const double rhs = A.rhs[id];
         bb += rhs * rhs;
-        const double r = sc * rhs - pcg_row<DIM>(A, A.x, dd, i, j, k, id);
+        const double r = sc * rhs - pcg_row<DIM>(A, A.x, sc, i, j, k, id);
         const double ro = r / sc;
         rr += ro * ro;
         const double z = r / dd;
@@ -216,7 +219,7 @@ __global__ void __launch_bounds__(256) pcg_matvec_kernel(PcgArgs A) {
     int i, j, k;
     for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
-        const double qv = pcg_row<DIM>(A, A.p, A.d[id], i, j, k, id);
+        const double qv = pcg_row<DIM>(A, A.p, A.s[id], i, j, k, id);
         A.q[id] = qv;
         pq += A.p[id] * qv;
     }
@@ -227,7 +230,16 @@ __global__ void __launch_bounds__(256) pcg_alpha_kernel(PcgArgs A) {
     if (A.st[0]) return;
     __shared__ double sh[256 / kWave];
     const double pq = pcg_fold(A, sh, 0);
-    if (threadIdx.x == 0) A.sc[1] = A.sc[0] / pq;
+    if (threadIdx.x == 0) {
+        if (pq > 0.0) {
+            A.sc[1] = A.sc[0] / pq;
+        } else {
+            // breakdown (rounding has exhausted the search directions): stop,
+            // reported as not converged
+            A.st[0] = 1;
+            A.st[1] = A.maxIter;
+        }
+    }
 }
 
 // x += alpha p, r -= alpha q, z = r / d; partials rr = sum (r / s)^2, rz = sum r z
