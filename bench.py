@@ -296,11 +296,12 @@ def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred:
     sin(pi y), w = 0.5 sin(pi z)), which makes every pressure solve run to its
     1000-sweep cap.  solver "pcg": the conjugate-gradient pressure solve
     (stop on ||rhs - A p||_2 <= 1e-6 ||rhs||_2, <= 1000 iterations) instead of
-    Jacobi (stop on max |p_new - p| < 1e-6)."""
+    Jacobi (stop on max |p_new - p| < 1e-6); solver "mgpcg": the same with the
+    multigrid V-cycle preconditioner."""
     import numpy as np
 
-    from semiimplicitfv_amd import (BoundaryCondition, HLLCSolver, JacobiPressureSolver, PCGPressureSolver,
-                                    PhaseEOS, ReconstructionOrder, SemiImplicitSolver, SimulationConfig,
+    from semiimplicitfv_amd import (BoundaryCondition, HLLCSolver, JacobiPressureSolver, MGPCGPressureSolver,
+                                    PCGPressureSolver, PhaseEOS, ReconstructionOrder, SemiImplicitSolver, SimulationConfig,
                                     StiffenedGasEOS, _abi, uniform_mesh)
 
     WATER, AIR = PhaseEOS(4.4, 6.0e8), PhaseEOS(1.4, 0.0)
@@ -318,7 +319,7 @@ def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred:
         mesh.setBoundaryCondition(f, BoundaryCondition.SlipWall)
     eos = StiffenedGasEOS(4.4, 6.0e8, 1.0, cfg)
     t0 = time.perf_counter()
-    ps = PCGPressureSolver() if solver == "pcg" else JacobiPressureSolver()
+    ps = {"pcg": PCGPressureSolver, "mgpcg": MGPCGPressureSolver}.get(solver, JacobiPressureSolver)()
     g = SemiImplicitSolver(mesh, HLLCSolver(eos), ps, eos, None, cfg, precision=precision)
     xc, yc, zc = mesh.centroids()
     r = np.sqrt((xc[None, None, :] - 0.5) ** 2 + (yc[None, :, None] - 0.5) ** 2 + (zc[:, None, None] - 0.5) ** 2)
@@ -367,7 +368,8 @@ def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred:
     pres = mesh.interior(g.download_field(_abi.F_PRES)).copy()
     finite = bool(np.isfinite(pres).all())
     g.close()
-    solve = "PCG (relative residual 1e-6)" if solver == "pcg" else "Jacobi (tol 1e-6)"
+    solve = {"pcg": "PCG (relative residual 1e-6)",
+             "mgpcg": "multigrid-preconditioned PCG (relative residual 1e-6)"}.get(solver, "Jacobi (tol 1e-6)")
     return {"workload": f"3D water/air bubble {n}^3, semi-implicit SSP-RK3 + {solve}, WENO5+HLLC, FP64 "
                         + ("state, FP32 Jacobi sweeps + FP64 refinement" if precision == "mixed" else "throughout"),
             "value": n ** 3 * steps / el, "unit": "cell-updates/s", "ms_per_step": el / steps * 1e3,
@@ -493,7 +495,11 @@ def main():
             r64 = bench_bubble(n, steps, precision="fp64", stirred=stirred)
             r32 = bench_bubble(n, steps, precision="mixed", stirred=stirred)
             rcg = bench_bubble(n, steps, precision="fp64", stirred=stirred, solver="pcg")
-            p64, p32, pcg = r64.pop("_pres"), r32.pop("_pres"), rcg.pop("_pres")
+            rmg = bench_bubble(n, steps, precision="fp64", stirred=stirred, solver="mgpcg")
+            p64, p32, pcg, pmg = r64.pop("_pres"), r32.pop("_pres"), rcg.pop("_pres"), rmg.pop("_pres")
+            rmg["pres_rel_L1_vs_jacobi"] = float(np.abs(pmg - p64).sum() / np.abs(p64).sum())
+            rmg["pres_rel_L1_vs_pcg"] = float(np.abs(pmg - pcg).sum() / np.abs(pcg).sum())
+            rmg["ratio_vs_jacobi"] = rmg["value"] / r64["value"]
             r32["pres_rel_L1_vs_fp64"] = float(np.abs(p32 - p64).sum() / np.abs(p64).sum())
             r32["ratio_vs_fp64"] = r32["value"] / r64["value"]
             rcg["pres_rel_L1_vs_jacobi"] = float(np.abs(pcg - p64).sum() / np.abs(p64).sum())
@@ -501,7 +507,7 @@ def main():
             if stirred:
                 r32["note"] = ("both solves stop at the 1000-sweep cap unconverged, so pres_rel_L1_vs_fp64 "
                                "compares two different partial iterates, not FP32 rounding")
-            out[tag] = {"fp64": r64, "fp32_sweeps": r32, "pcg": rcg}
+            out[tag] = {"fp64": r64, "fp32_sweeps": r32, "pcg": rcg, "mgpcg": rmg}
         return out
 
     for name, fn in (("ensemble", bench_ensemble), ("adjoint", bench_adjoint), ("bubble", bubble_pair),

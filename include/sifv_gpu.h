@@ -173,6 +173,7 @@ int sifv_jacobi_solve(sifv_handle* h, const double* rho, const double* rhoc2,
 /* Pressure solver kinds of the semi-implicit step. */
 #define SIFV_PS_JACOBI 0
 #define SIFV_PS_PCG 1
+#define SIFV_PS_MGPCG 2
 
 /* Replaces HyprePressureSolver::solve (src/HyprePressureSolver.cpp:312-408):
  * the same system as sifv_jacobi_solve, solved by conjugate gradients on
@@ -187,9 +188,21 @@ int sifv_pcg_solve(sifv_handle* h, const double* rho, const double* rhoc2,
                    const double* rhs, double* pressure, double dt,
                    double tolerance, int maxIter, int* iters);
 
+/* Replaces HyprePressureSolver::solve (src/HyprePressureSolver.cpp:312-408)
+ * in its StructPCG + PFMG form (src/HyprePressureSolver.cpp:123-138): as
+ * sifv_pcg_solve, preconditioned by one geometric-multigrid V-cycle
+ * (weighted-Jacobi relaxation, one pre- and one post-sweep, 2:1 coarsening of
+ * every axis down to at most 4 cells per axis, piecewise-constant transfers,
+ * rediscretised coarse operators) instead of the diagonal.  Same arguments,
+ * stopping rule, iteration count and refusals as sifv_pcg_solve. */
+int sifv_mgpcg_solve(sifv_handle* h, const double* rho, const double* rhoc2,
+                     const double* rhs, double* pressure, double dt,
+                     double tolerance, int maxIter, int* iters);
+
 /* Chooses the PressureSolver of sifv_semi_implicit_step (the reference's
  * SemiImplicitSolver constructor argument, include/SemiImplicitSolver.hpp:24-31):
- * SIFV_PS_JACOBI (default) or SIFV_PS_PCG (FP64 handles only). */
+ * SIFV_PS_JACOBI (default), SIFV_PS_PCG or SIFV_PS_MGPCG (the last two on
+ * FP64 handles only). */
 int sifv_set_pressure_solver(sifv_handle* h, int kind);
 
 /* ---- ghost fill and time step ------------------------------------------ */

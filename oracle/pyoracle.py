@@ -76,6 +76,10 @@ def _lib(impl: str):
         lib.or_jacobi.argtypes = [vp, pd, pd, pd, pd, d, d, i, pd]
         lib.or_pcg.restype = i
         lib.or_pcg.argtypes = [vp, pd, pd, pd, pd, d, d, i]
+        lib.or_mgpcg.restype = i
+        lib.or_mgpcg.argtypes = [vp, pd, pd, pd, pd, d, d, i]
+        lib.or_mg_precond.restype = i
+        lib.or_mg_precond.argtypes = [vp, pd, pd, d, pd, pd]
         lib.or_set_pressure_solver.restype = None
         lib.or_set_pressure_solver.argtypes = [vp, i]
         lib.or_recon.restype = d
@@ -205,8 +209,20 @@ class CpuSolver:
         return self.lib.or_pcg(self.h, _abi.dptr(rho), _abi.dptr(rhoc2), _abi.dptr(rhs), _abi.dptr(p), dt, tol,
                                maxIter)
 
+    def mgpcg(self, rho, rhoc2, rhs, p, dt, tol, maxIter):
+        """PCG with the multigrid V-cycle preconditioner (oracle only)."""
+        assert self.impl == "c"
+        return self.lib.or_mgpcg(self.h, _abi.dptr(rho), _abi.dptr(rhoc2), _abi.dptr(rhs), _abi.dptr(p), dt, tol,
+                                 maxIter)
+
+    def mg_precond(self, rho, rhoc2, dt, r, z):
+        """z = one multigrid V-cycle applied to r (oracle only); returns the
+        number of coarse levels, -1 when the mesh is refused."""
+        assert self.impl == "c"
+        return self.lib.or_mg_precond(self.h, _abi.dptr(rho), _abi.dptr(rhoc2), dt, _abi.dptr(r), _abi.dptr(z))
+
     def set_pressure_solver(self, kind: int):
-        """Pressure solver of semi_step (oracle only): 0 Jacobi, 1 PCG."""
+        """Pressure solver of semi_step (oracle only): 0 Jacobi, 1 PCG, 2 multigrid PCG."""
         assert self.impl == "c"
         self.lib.or_set_pressure_solver(self.h, kind)
 

@@ -104,13 +104,41 @@ struct or_solver {
     double* grad; /* 9 per cell */
     prim_t* faceL;
     prim_t* faceR;
-    int psolver;  /* pressure solver of the semi-implicit step: 0 Jacobi, 1 PCG */
+    int psolver;  /* pressure solver of the semi-implicit step: 0 Jacobi, 1 PCG, 2 multigrid PCG */
     double* cg[6]; /* PCG vectors r, z, p, q, d, s (lazily allocated) */
+    int mgLevels;  /* coarse levels of the multigrid preconditioner (0: not built) */
+    struct or_mglev* mg;
 };
 
 size_t or_total_cells(const or_solver* s) { return s->m.total; }
 
 static double* zalloc(size_t n) { return (double*)calloc(n, sizeof(double)); }
+
+/* One coarse level of the multigrid preconditioner: n cells per axis,
+ * compact x-fastest arrays (no ghosts).  s: mass term V/c summed over the
+ * children; w[d]: weight of the cell's low face along d (the wrap face at
+ * index 0 of a periodic axis, else 0 there); d: diagonal; b, x, t: right-hand
+ * side, iterate and output of the V-cycle on this level. */
+struct or_mglev {
+    int n[3];
+    size_t cells;
+    double *s, *w[3], *d, *b, *x, *t;
+};
+
+static void mg_free(struct or_solver* s) {
+    for (int l = 0; l < s->mgLevels; ++l) {
+        struct or_mglev* L = &s->mg[l];
+        free(L->s);
+        for (int d = 0; d < 3; ++d) free(L->w[d]);
+        free(L->d);
+        free(L->b);
+        free(L->x);
+        free(L->t);
+    }
+    free(s->mg);
+    s->mg = NULL;
+    s->mgLevels = 0;
+}
 
 or_solver* or_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann,
                      const sifv_eos* eos) {
@@ -197,6 +225,7 @@ void or_destroy(or_solver* s) {
     free(s->pressure_);
     free(s->pNew);
     for (int q = 0; q < 6; ++q) free(s->cg[q]);
+    mg_free(s);
     free(s->grad);
     free(s->faceL);
     free(s->faceR);
@@ -1489,8 +1518,231 @@ static int or_pcg_symmetric(const or_mesh* m) {
     return 1;
 }
 
-int or_pcg(or_solver* s, const double* rho, const double* rhoc2, const double* rhs, double* x, double dt,
-           double tol, int maxIter) {
+/* ------------------------------------------------------------------ */
+/* Geometric multigrid V-cycle used as the PCG preconditioner (the role of
+ * the PFMG preconditioner the reference's HyprePressureSolver attaches to
+ * StructPCG: one V-cycle, weighted-Jacobi relaxation, one pre- and one
+ * post-sweep, src/HyprePressureSolver.cpp:123-138).  Not PFMG itself: full
+ * 2:1 coarsening of every active axis with piecewise-constant transfers,
+ * coarse operators by rediscretisation.
+ *   coarse cell C = children {2c, 2c+1} per axis (those inside the finer grid);
+ *   s_C = sum of the children's s (the mass term V/c adds up exactly);
+ *   w_d(C) = f_d * sum of the finer low-face weights along d of the children
+ *            on C's low side, f_d = 1/2 when the axis was coarsened (the face
+ *            distance doubles), 1 when it has one cell;
+ *   d_C = s_C + sum_d (w_d(C) + w_d(C + e_d, wrapped)).
+ * Levels are added while the finest has no coarse level yet or the coarsest
+ * still has an axis longer than 4 cells.  V-cycle on level l with right-hand
+ * side b into out (zero initial guess):
+ *   x = om b / d;  b_{l+1}(C) = sum_children (b - A x);  V-cycle(l+1) -> t_{l+1};
+ *   x += t_{l+1}(parent);  out = x + om (b - A x) / d;
+ * on the coarsest level MG_COARSE_SWEEPS weighted-Jacobi sweeps from zero.
+ * om = 2/3 (1D), 4/5 (2D), 6/7 (3D).  Every step is linear and symmetric in
+ * the S-inner-product sense (Jacobi pre = post, restriction = transpose of
+ * the prolongation), so the preconditioner is a fixed symmetric operator.
+ * Children and sums run in (k, j, i) order. */
+#define MG_COARSE_SWEEPS 16
+#define MG_MAX_LEVELS 24
+
+static double mg_omega(int dim) { return dim == 1 ? 2.0 / 3.0 : dim == 2 ? 0.8 : 6.0 / 7.0; }
+
+static size_t mg_idx(const struct or_mglev* L, int i, int j, int k) {
+    return (size_t)i + (size_t)L->n[0] * ((size_t)j + (size_t)L->n[1] * (size_t)k);
+}
+
+/* (A v)_C on a coarse level */
+static double mg_row(const struct or_mglev* L, int dim, const double* v, int i, int j, int k) {
+    const size_t id = mg_idx(L, i, j, k);
+    const double vi = v[id];
+    double acc = L->s[id] * vi;
+    for (int dir = 0; dir < dim; ++dir) {
+        const int n = L->n[dir];
+        const int c = dir == 0 ? i : dir == 1 ? j : k;
+        const int cm = (c - 1 + n) % n, cp = (c + 1) % n;
+        const size_t lo = dir == 0 ? mg_idx(L, cm, j, k) : dir == 1 ? mg_idx(L, i, cm, k) : mg_idx(L, i, j, cm);
+        const size_t hi = dir == 0 ? mg_idx(L, cp, j, k) : dir == 1 ? mg_idx(L, i, cp, k) : mg_idx(L, i, j, cp);
+        acc += L->w[dir][id] * (vi - v[lo]);
+        acc += L->w[dir][hi] * (vi - v[hi]);
+    }
+    return acc;
+}
+
+/* Builds (once per mesh) the level dimensions and storage. */
+static void mg_alloc(or_solver* s) {
+    if (s->mgLevels) return;
+    const or_mesh* m = &s->m;
+    int n[3] = {m->nx, m->ny, m->nz};
+    struct or_mglev tmp[MG_MAX_LEVELS];
+    int L = 0;
+    while (L < MG_MAX_LEVELS) {
+        const int mx = n[0] > n[1] ? (n[0] > n[2] ? n[0] : n[2]) : (n[1] > n[2] ? n[1] : n[2]);
+        if (L > 0 && mx <= 4) break;
+        for (int d = 0; d < 3; ++d) n[d] = (n[d] + 1) / 2;
+        struct or_mglev* c = &tmp[L];
+        memset(c, 0, sizeof *c);
+        for (int d = 0; d < 3; ++d) c->n[d] = n[d];
+        c->cells = (size_t)n[0] * n[1] * n[2];
+        c->s = zalloc(c->cells);
+        for (int d = 0; d < 3; ++d) c->w[d] = zalloc(c->cells);
+        c->d = zalloc(c->cells);
+        c->b = zalloc(c->cells);
+        c->x = zalloc(c->cells);
+        c->t = zalloc(c->cells);
+        ++L;
+    }
+    s->mg = (struct or_mglev*)malloc(sizeof(struct or_mglev) * L);
+    memcpy(s->mg, tmp, sizeof(struct or_mglev) * L);
+    s->mgLevels = L;
+}
+
+/* Coarse operators for this solve: level 1 from the fine s and face weights
+ * (rho), deeper levels from the level above. */
+static void mg_setup(or_solver* s, const double* rho, const double* sc) {
+    const or_mesh* m = &s->m;
+    const int dim = m->dim;
+    for (int l = 0; l < s->mgLevels; ++l) {
+        struct or_mglev* C = &s->mg[l];
+        const struct or_mglev* F = l ? &s->mg[l - 1] : NULL;
+        const int nf[3] = {F ? F->n[0] : m->nx, F ? F->n[1] : m->ny, F ? F->n[2] : m->nz};
+        double fac[3];
+        for (int d = 0; d < 3; ++d) fac[d] = nf[d] != C->n[d] ? 0.5 : 1.0;
+        for (int ck = 0; ck < C->n[2]; ++ck)
+            for (int cj = 0; cj < C->n[1]; ++cj)
+                for (int ci = 0; ci < C->n[0]; ++ci) {
+                    double ss = 0.0, ww[3] = {0.0, 0.0, 0.0};
+                    for (int k = 2 * ck; k < 2 * ck + 2 && k < nf[2]; ++k)
+                        for (int j = 2 * cj; j < 2 * cj + 2 && j < nf[1]; ++j)
+                            for (int i = 2 * ci; i < 2 * ci + 2 && i < nf[0]; ++i) {
+                                const int at[3] = {i == 2 * ci, j == 2 * cj, k == 2 * ck};
+                                if (F) {
+                                    const size_t f = mg_idx(F, i, j, k);
+                                    ss += F->s[f];
+                                    for (int d = 0; d < dim; ++d)
+                                        if (at[d]) ww[d] += F->w[d][f];
+                                } else {
+                                    ss += sc[IDX(m, i, j, k)];
+                                    for (int d = 0; d < dim; ++d)
+                                        if (at[d]) ww[d] += pcg_row_w(m, rho, d, 0, i, j, k);
+                                }
+                            }
+                    const size_t c = mg_idx(C, ci, cj, ck);
+                    C->s[c] = ss;
+                    for (int d = 0; d < 3; ++d) C->w[d][c] = d < dim ? fac[d] * ww[d] : 0.0;
+                }
+        for (int ck = 0; ck < C->n[2]; ++ck)
+            for (int cj = 0; cj < C->n[1]; ++cj)
+                for (int ci = 0; ci < C->n[0]; ++ci) {
+                    const size_t c = mg_idx(C, ci, cj, ck);
+                    double dd = C->s[c];
+                    for (int d = 0; d < dim; ++d) {
+                        const int cc[3] = {ci, cj, ck};
+                        int up[3] = {ci, cj, ck};
+                        up[d] = (cc[d] + 1) % C->n[d];
+                        dd += C->w[d][c];
+                        dd += C->w[d][mg_idx(C, up[0], up[1], up[2])];
+                    }
+                    C->d[c] = dd;
+                }
+    }
+}
+
+/* out = src + om (b - A src) / d on a coarse level */
+static void mg_smooth(const struct or_mglev* L, int dim, double om, const double* src, double* out) {
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) {
+                const size_t id = mg_idx(L, i, j, k);
+                out[id] = src[id] + om * (L->b[id] - mg_row(L, dim, src, i, j, k)) / L->d[id];
+            }
+}
+
+/* V-cycle on coarse level l (right-hand side L->b) into L->t */
+static void mg_vcycle(or_solver* s, int l) {
+    const int dim = s->m.dim;
+    const double om = mg_omega(dim);
+    struct or_mglev* L = &s->mg[l];
+    for (size_t c = 0; c < L->cells; ++c) L->x[c] = om * L->b[c] / L->d[c];
+    if (l == s->mgLevels - 1) {
+        for (int sw = 1; sw < MG_COARSE_SWEEPS; ++sw) {
+            if (sw & 1) mg_smooth(L, dim, om, L->x, L->t);
+            else mg_smooth(L, dim, om, L->t, L->x);
+        }
+        return;
+    }
+    struct or_mglev* C = &s->mg[l + 1];
+    for (int ck = 0; ck < C->n[2]; ++ck)
+        for (int cj = 0; cj < C->n[1]; ++cj)
+            for (int ci = 0; ci < C->n[0]; ++ci) {
+                double acc = 0.0;
+                for (int k = 2 * ck; k < 2 * ck + 2 && k < L->n[2]; ++k)
+                    for (int j = 2 * cj; j < 2 * cj + 2 && j < L->n[1]; ++j)
+                        for (int i = 2 * ci; i < 2 * ci + 2 && i < L->n[0]; ++i)
+                            acc += L->b[mg_idx(L, i, j, k)] - mg_row(L, dim, L->x, i, j, k);
+                C->b[mg_idx(C, ci, cj, ck)] = acc;
+            }
+    mg_vcycle(s, l + 1);
+    for (int k = 0; k < L->n[2]; ++k)
+        for (int j = 0; j < L->n[1]; ++j)
+            for (int i = 0; i < L->n[0]; ++i) L->x[mg_idx(L, i, j, k)] += C->t[mg_idx(C, i / 2, j / 2, k / 2)];
+    mg_smooth(L, dim, om, L->x, L->t);
+}
+
+/* z = V-cycle(r) on the fine level; q is scratch (the fine iterate).  Fine
+ * products read q through its ghost fill. */
+static void mg_apply(or_solver* s, const double* rho, const double* sc, const double* d, const double* r, double* q,
+                     double* z) {
+    const or_mesh* m = &s->m;
+    const int dim = m->dim;
+    const double om = mg_omega(dim);
+    for (int k = 0; k < m->nz; ++k)
+        for (int j = 0; j < m->ny; ++j)
+            for (int i = 0; i < m->nx; ++i) {
+                const size_t id = IDX(m, i, j, k);
+                q[id] = om * r[id] / d[id];
+            }
+    fill_scalar(m, q);
+    struct or_mglev* C = &s->mg[0];
+    for (int ck = 0; ck < C->n[2]; ++ck)
+        for (int cj = 0; cj < C->n[1]; ++cj)
+            for (int ci = 0; ci < C->n[0]; ++ci) {
+                double acc = 0.0;
+                for (int k = 2 * ck; k < 2 * ck + 2 && k < m->nz; ++k)
+                    for (int j = 2 * cj; j < 2 * cj + 2 && j < m->ny; ++j)
+                        for (int i = 2 * ci; i < 2 * ci + 2 && i < m->nx; ++i) {
+                            const size_t id = IDX(m, i, j, k);
+                            const double vi = q[id];
+                            double a = sc[id] * vi;
+                            for (int dir = 0; dir < dim; ++dir) {
+                                const int ei = dir == 0, ej = dir == 1, ek = dir == 2;
+                                a += pcg_row_w(m, rho, dir, 0, i, j, k) * (vi - q[IDX(m, i - ei, j - ej, k - ek)]);
+                                a += pcg_row_w(m, rho, dir, 1, i, j, k) * (vi - q[IDX(m, i + ei, j + ej, k + ek)]);
+                            }
+                            acc += r[id] - a;
+                        }
+                C->b[mg_idx(C, ci, cj, ck)] = acc;
+            }
+    mg_vcycle(s, 0);
+    for (int k = 0; k < m->nz; ++k)
+        for (int j = 0; j < m->ny; ++j)
+            for (int i = 0; i < m->nx; ++i) q[IDX(m, i, j, k)] += C->t[mg_idx(C, i / 2, j / 2, k / 2)];
+    fill_scalar(m, q);
+    for (int k = 0; k < m->nz; ++k)
+        for (int j = 0; j < m->ny; ++j)
+            for (int i = 0; i < m->nx; ++i) {
+                const size_t id = IDX(m, i, j, k);
+                const double vi = q[id];
+                double a = sc[id] * vi;
+                for (int dir = 0; dir < dim; ++dir) {
+                    const int ei = dir == 0, ej = dir == 1, ek = dir == 2;
+                    a += pcg_row_w(m, rho, dir, 0, i, j, k) * (vi - q[IDX(m, i - ei, j - ej, k - ek)]);
+                    a += pcg_row_w(m, rho, dir, 1, i, j, k) * (vi - q[IDX(m, i + ei, j + ej, k + ek)]);
+                }
+                z[id] = vi + om * (r[id] - a) / d[id];
+            }
+}
+
+static int pcg_core(or_solver* s, const double* rho, const double* rhoc2, const double* rhs, double* x, double dt,
+                    double tol, int maxIter, int mg) {
     or_mesh* m = &s->m;
     if (!or_pcg_symmetric(m)) {
         snprintf(g_err, sizeof g_err, "PCG needs a symmetric system: a periodic axis with stretched wrap widths");
@@ -1537,6 +1789,19 @@ int or_pcg(or_solver* s, const double* rho, const double* rhoc2, const double* r
                 rz += r[id] * z[id];
                 p[id] = z[id];
             }
+    if (mg) {
+        mg_alloc(s);
+        mg_setup(s, rho, sc);
+        mg_apply(s, rho, sc, d, r, q, z);
+        rz = 0.0;
+        for (int k = 0; k < m->nz; ++k)
+            for (int j = 0; j < m->ny; ++j)
+                for (int i = 0; i < m->nx; ++i) {
+                    const size_t id = IDX(m, i, j, k);
+                    rz += r[id] * z[id];
+                    p[id] = z[id];
+                }
+    }
     if (rr <= tol2 * bb) return 0;
     for (int it = 0; it < maxIter; ++it) {
         fill_scalar(m, p);
@@ -1565,7 +1830,20 @@ int or_pcg(or_solver* s, const double* rho, const double* rhoc2, const double* r
                     z[id] = r[id] / d[id];
                     rzn += r[id] * z[id];
                 }
+        if (mg) {
+            mg_apply(s, rho, sc, d, r, q, z);
+            rzn = 0.0;
+            for (int k = 0; k < m->nz; ++k)
+                for (int j = 0; j < m->ny; ++j)
+                    for (int i = 0; i < m->nx; ++i) {
+                        const size_t id = IDX(m, i, j, k);
+                        rzn += r[id] * z[id];
+                    }
+        }
         if (rr <= tol2 * bb) return it + 1;
+        /* a preconditioned residual with r.z <= 0 (rounding at the end of a
+         * solve): stop, reported as not converged */
+        if (!(rzn > 0.0)) return maxIter;
         const double beta = rzn / rz;
         rz = rzn;
         for (int k = 0; k < m->nz; ++k)
@@ -1576,6 +1854,51 @@ int or_pcg(or_solver* s, const double* rho, const double* rhoc2, const double* r
                 }
     }
     return maxIter;
+}
+
+
+int or_pcg(or_solver* s, const double* rho, const double* rhoc2, const double* rhs, double* x, double dt,
+           double tol, int maxIter) {
+    return pcg_core(s, rho, rhoc2, rhs, x, dt, tol, maxIter, 0);
+}
+
+/* z = one V-cycle of the multigrid preconditioner applied to r (interior
+ * cells of padded fields), for the system of or_pcg on rho / rhoc2 / dt.
+ * Returns -1 where or_pcg refuses the mesh. */
+int or_mg_precond(or_solver* s, const double* rho, const double* rhoc2, double dt, const double* r, double* z) {
+    or_mesh* m = &s->m;
+    if (!or_pcg_symmetric(m)) {
+        snprintf(g_err, sizeof g_err, "PCG needs a symmetric system: a periodic axis with stretched wrap widths");
+        return -1;
+    }
+    for (int q = 0; q < 6; ++q)
+        if (!s->cg[q]) s->cg[q] = zalloc(m->total);
+    double *q = s->cg[3], *d = s->cg[4], *sc = s->cg[5];
+    const double dt2 = dt * dt;
+    for (int k = 0; k < m->nz; ++k)
+        for (int j = 0; j < m->ny; ++j)
+            for (int i = 0; i < m->nx; ++i) {
+                const size_t id = IDX(m, i, j, k);
+                sc[id] = VOL(m, i, j, k) / (rhoc2[id] * dt2);
+                double dd = sc[id];
+                for (int dir = 0; dir < m->dim; ++dir) {
+                    dd += pcg_row_w(m, rho, dir, 0, i, j, k);
+                    dd += pcg_row_w(m, rho, dir, 1, i, j, k);
+                }
+                d[id] = dd;
+            }
+    mg_alloc(s);
+    mg_setup(s, rho, sc);
+    mg_apply(s, rho, sc, d, r, q, z);
+    return s->mgLevels;
+}
+
+/* PCG as or_pcg with one multigrid V-cycle as the preconditioner instead of
+ * the diagonal (the StructPCG + PFMG pairing of the reference's
+ * HyprePressureSolver, src/HyprePressureSolver.cpp:123-138, 350-359). */
+int or_mgpcg(or_solver* s, const double* rho, const double* rhoc2, const double* rhs, double* x, double dt,
+             double tol, int maxIter) {
+    return pcg_core(s, rho, rhoc2, rhs, x, dt, tol, maxIter, 1);
 }
 
 void or_set_pressure_solver(or_solver* s, int kind) { s->psolver = kind; }
@@ -1704,9 +2027,10 @@ double or_semi_step(or_solver* s, double* const* f, const sifv_config* c, double
                     s->pRhs[id] = s->pAdv[id] - s->rhoc2[id] * dt * s->divUstar[id];
                     s->pressure_[id] = f[SIFV_F_PRES][id];
                 }
-        lastIters = s->psolver == 1
-                        ? or_pcg(s, f[SIFV_F_RHO], s->rhoc2, s->pRhs, s->pressure_, dt,
-                                 c->semiImplicitParams.pressureTol, c->semiImplicitParams.maxPressureIters)
+        lastIters = s->psolver >= 1
+                        ? pcg_core(s, f[SIFV_F_RHO], s->rhoc2, s->pRhs, s->pressure_, dt,
+                                   c->semiImplicitParams.pressureTol, c->semiImplicitParams.maxPressureIters,
+                                   s->psolver == 2)
                         : or_jacobi(s, f[SIFV_F_RHO], s->rhoc2, s->pRhs, s->pressure_, dt,
                                     c->semiImplicitParams.pressureTol, c->semiImplicitParams.maxPressureIters,
                                     NULL);

@@ -35,6 +35,9 @@ int or_jacobi(or_solver* s, const double* rho, const double* rhoc2, const double
               double* p, double dt, double tol, int maxIter, double* hist);
 int or_pcg(or_solver* s, const double* rho, const double* rhoc2, const double* rhs, double* p, double dt,
            double tol, int maxIter);
+int or_mgpcg(or_solver* s, const double* rho, const double* rhoc2, const double* rhs, double* p, double dt,
+             double tol, int maxIter);
+int or_mg_precond(or_solver* s, const double* rho, const double* rhoc2, double dt, const double* r, double* z);
 void or_set_pressure_solver(or_solver* s, int kind);
 void or_fill_ghosts(or_solver* s, double* const* f, int varset);
 void or_fill_scalar(or_solver* s, double* field);

@@ -10,7 +10,7 @@ from .config import (BoundaryCondition, ExplicitParams, IGRParams, MultiPhasePar
                      ReconstructionOrder, RiemannSolverType, SemiImplicitParams, SimulationConfig, VarSet)
 from .mesh import RectilinearMesh, runtime_linspace, uniform_mesh
 from .state import (HLLCSolver, IdealGasEOS, LFSolver, RusanovSolver, SolutionState, StiffenedGasEOS)
-from .solvers import ExplicitSolver, JacobiPressureSolver, PCGPressureSolver, SemiImplicitSolver
+from .solvers import ExplicitSolver, JacobiPressureSolver, MGPCGPressureSolver, PCGPressureSolver, SemiImplicitSolver
 from . import _abi
 from .ensemble import RiemannEnsemble
 
@@ -18,5 +18,5 @@ __all__ = [
     "BoundaryCondition", "ExplicitParams", "IGRParams", "MultiPhaseParams", "PhaseEOS", "ReconstructionOrder",
     "RiemannSolverType", "SemiImplicitParams", "SimulationConfig", "VarSet", "RectilinearMesh", "runtime_linspace",
     "uniform_mesh", "HLLCSolver", "IdealGasEOS", "LFSolver", "RusanovSolver", "SolutionState", "StiffenedGasEOS",
-    "ExplicitSolver", "JacobiPressureSolver", "PCGPressureSolver", "SemiImplicitSolver", "RiemannEnsemble",
+    "ExplicitSolver", "JacobiPressureSolver", "MGPCGPressureSolver", "PCGPressureSolver", "SemiImplicitSolver", "RiemannEnsemble",
 ]

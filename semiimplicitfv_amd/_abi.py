@@ -121,6 +121,7 @@ def _declare(lib):
         "sifv_semi_implicit_step": (i, [vp, P(Config_C), d, P(d), P(i)]),
         "sifv_jacobi_solve": (i, [vp, P(d), P(d), P(d), P(d), d, d, i, P(i), P(d)]),
         "sifv_pcg_solve": (i, [vp, P(d), P(d), P(d), P(d), d, d, i, P(i)]),
+        "sifv_mgpcg_solve": (i, [vp, P(d), P(d), P(d), P(d), d, d, i, P(i)]),
         "sifv_set_pressure_solver": (i, [vp, i]),
         "sifv_fill_ghosts": (i, [vp, i]),
         "sifv_fill_scalar_ghosts": (i, [vp, i]),

@@ -205,6 +205,9 @@ struct sifv_handle {
     int* pcgSt = nullptr;     // PCG device flags
     int* pcgStHost = nullptr; // pinned copy of the flags
     int pcgNblk = 0;
+    int mgLevels = 0;         // coarse levels of the multigrid preconditioner (0: not built)
+    MgLevel mg[kMgMaxLevels] = {};
+    double* mgBuf = nullptr;  // storage of every coarse level
     double* jacPn = nullptr;  // FP64 Jacobi update of the refined iterate
     float* jac32 = nullptr;   // rho, co, r, d0, d1 (FP32)
     // discrete adjoint: checkpoints of the forward run and work arrays
@@ -931,7 +934,58 @@ int run_jacobi_mixed(sifv_handle* h, const double* rho, const double* dt, double
 // jac[3] (rhs) with the device time step dt, x = jac[0] in place (initial
 // guess in, answer out; interior cells only).  *n = iterations (0 when the
 // guess already passes, maxIter when not converged).
-int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int maxIter, int* n) {
+// Coarse levels of the multigrid preconditioner (sifv_pcg.hpp): every axis
+// halved (rounding up) while the finest has no coarse level yet or the
+// coarsest still has an axis longer than 4 cells; 8 compact arrays per level.
+int ensure_mg(sifv_handle* h) {
+    if (h->mgLevels) return SIFV_OK;
+    int n[3] = {h->nx, h->ny, h->nz};
+    int L = 0;
+    size_t total = 0;
+    MgLevel lv[kMgMaxLevels] = {};
+    while (L < kMgMaxLevels) {
+        const int mx = std::max(n[0], std::max(n[1], n[2]));
+        if (L > 0 && mx <= 4) break;
+        for (int d = 0; d < 3; ++d) n[d] = (n[d] + 1) / 2;
+        for (int d = 0; d < 3; ++d) lv[L].n[d] = n[d];
+        lv[L].cells = n[0] * n[1] * n[2];
+        total += 8 * (size_t)lv[L].cells;
+        ++L;
+    }
+    double* buf = nullptr;
+    HIPCHK(hipMalloc(&buf, sizeof(double) * total));
+    HIPCHK(hipMemset(buf, 0, sizeof(double) * total));
+    double* q = buf;
+    for (int l = 0; l < L; ++l) {
+        const size_t c = lv[l].cells;
+        lv[l].s = q;
+        q += c;
+        for (int d = 0; d < 3; ++d) {
+            lv[l].w[d] = q;
+            q += c;
+        }
+        lv[l].d = q;
+        q += c;
+        lv[l].b = q;
+        q += c;
+        lv[l].x = q;
+        q += c;
+        lv[l].t = q;
+        q += c;
+    }
+    h->mgBuf = buf;
+    for (int l = 0; l < L; ++l) h->mg[l] = lv[l];
+    h->mgLevels = L;
+    return SIFV_OK;
+}
+
+template <typename F1, typename F2, typename F3>
+const void* by_dim(int dim, F1 f1, F2 f2, F3 f3) {
+    return dim == 1 ? reinterpret_cast<const void*>(f1)
+           : dim == 2 ? reinterpret_cast<const void*>(f2) : reinterpret_cast<const void*>(f3);
+}
+
+int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int maxIter, int* n, bool mg) {
     if (!h->pcgSym)
         return fail(SIFV_ERR_UNSUPPORTED,
                     "PCG needs a symmetric system: a periodic axis with stretched wrap widths (use Jacobi)");
@@ -982,7 +1036,86 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
     const void* matvec = h->dim == 1 ? reinterpret_cast<const void*>(pcg_matvec_kernel<1>)
                          : h->dim == 2 ? reinterpret_cast<const void*>(pcg_matvec_kernel<2>)
                                        : reinterpret_cast<const void*>(pcg_matvec_kernel<3>);
+    // multigrid V-cycle: z = V(r) on the fine level, kernels skipped once
+    // *gate is set (null gate during the set-up)
+    MgArgs M;
+    M.A = A;
+    M.gate = nullptr;
+    M.src = nullptr;
+    M.dst = nullptr;
+    M.slot = 0;
+    M.init = 0;
+    void* margs[] = {&M};
+    const int dim = h->dim;
+    auto cgrid = [](int cells) { return std::max(1, std::min((cells + 255) / 256, 8192)); };
+    auto mlaunch = [&](const void* fn, int grid) { return timed_launch(h, fn, dim3(grid), blk, margs); };
+    auto vcycle = [&]() -> int {
+        const int L = h->mgLevels;
+        int r = mlaunch(by_dim(dim, mg_pre_fine_kernel<1>, mg_pre_fine_kernel<2>, mg_pre_fine_kernel<3>), nb);
+        M.C = h->mg[0];
+        if (!r)
+            r = mlaunch(by_dim(dim, mg_restrict_fine_kernel<1>, mg_restrict_fine_kernel<2>,
+                               mg_restrict_fine_kernel<3>),
+                        cgrid(M.C.cells));
+        for (int l = 0; l < L && !r; ++l) {
+            M.C = h->mg[l];
+            r = mlaunch(by_dim(dim, mg_pre_kernel<1>, mg_pre_kernel<2>, mg_pre_kernel<3>), cgrid(M.C.cells));
+            if (!r && l + 1 < L) {
+                M.F = h->mg[l];
+                M.C = h->mg[l + 1];
+                r = mlaunch(by_dim(dim, mg_restrict_kernel<1>, mg_restrict_kernel<2>, mg_restrict_kernel<3>),
+                            cgrid(M.C.cells));
+            }
+        }
+        // coarsest: sweeps 2..kMgCoarseSweeps alternate x -> t, t -> x and end in t
+        M.C = h->mg[L - 1];
+        for (int sw = 1; sw < kMgCoarseSweeps && !r; ++sw) {
+            M.src = (sw & 1) ? M.C.x : M.C.t;
+            M.dst = (sw & 1) ? M.C.t : M.C.x;
+            r = mlaunch(by_dim(dim, mg_smooth_kernel<1>, mg_smooth_kernel<2>, mg_smooth_kernel<3>),
+                        cgrid(M.C.cells));
+        }
+        for (int l = L - 2; l >= 0 && !r; --l) {
+            M.F = h->mg[l];
+            M.C = h->mg[l + 1];
+            r = mlaunch(reinterpret_cast<const void*>(mg_prolong_kernel), cgrid(M.F.cells));
+            if (!r) {
+                M.C = h->mg[l];
+                M.src = M.C.x;
+                M.dst = M.C.t;
+                r = mlaunch(by_dim(dim, mg_smooth_kernel<1>, mg_smooth_kernel<2>, mg_smooth_kernel<3>),
+                            cgrid(M.C.cells));
+            }
+        }
+        M.C = h->mg[0];
+        if (!r) r = mlaunch(reinterpret_cast<const void*>(mg_prolong_fine_kernel), nb);
+        if (!r) r = mlaunch(by_dim(dim, mg_post_fine_kernel<1>, mg_post_fine_kernel<2>, mg_post_fine_kernel<3>), nb);
+        return r;
+    };
     int rc = launch(setup, nb);
+    if (!rc && mg) {
+        rc = ensure_mg(h);
+        for (int l = 0; l < h->mgLevels && !rc; ++l) {
+            M.C = h->mg[l];
+            if (l == 0) {
+                rc = mlaunch(by_dim(dim, mg_build_kernel<1, true>, mg_build_kernel<2, true>, mg_build_kernel<3, true>),
+                             cgrid(M.C.cells));
+            } else {
+                M.F = h->mg[l - 1];
+                rc = mlaunch(
+                    by_dim(dim, mg_build_kernel<1, false>, mg_build_kernel<2, false>, mg_build_kernel<3, false>),
+                    cgrid(M.C.cells));
+            }
+            if (!rc) rc = mlaunch(by_dim(dim, mg_diag_kernel<1>, mg_diag_kernel<2>, mg_diag_kernel<3>), cgrid(M.C.cells));
+        }
+        if (!rc) rc = vcycle();
+        M.slot = 2;
+        M.init = 1;
+        if (!rc) rc = mlaunch(reinterpret_cast<const void*>(mg_rz_kernel), nb);
+        M.gate = A.st;
+        M.slot = 1;
+        M.init = 0;
+    }
     if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_setup_fold_kernel), 1);
     if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_zero_kernel), nb);
     if (rc) return rc;
@@ -1001,6 +1134,8 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
             rc = launch(matvec, nb);
             if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_alpha_kernel), 1);
             if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_update_kernel), nb);
+            if (!rc && mg) rc = vcycle();
+            if (!rc && mg) rc = mlaunch(reinterpret_cast<const void*>(mg_rz_kernel), nb);
             if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_beta_kernel), 1);
             if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_dir_kernel), nb);
             if (rc) return rc;
@@ -1078,8 +1213,8 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
         HIPCHK(hipGetLastError());
         int n = 0;
         const double* pfinal = nullptr;
-        if (h->psolver == SIFV_PS_PCG) {
-            rc = run_pcg(h, rho, h->scal, sp.pressureTol, sp.maxPressureIters, &n);
+        if (h->psolver == SIFV_PS_PCG || h->psolver == SIFV_PS_MGPCG) {
+            rc = run_pcg(h, rho, h->scal, sp.pressureTol, sp.maxPressureIters, &n, h->psolver == SIFV_PS_MGPCG);
             if (rc) return rc;
             // fillScalarGhosts(pressure_) after the solve (src/SemiImplicitSolver.cpp:581)
             double* pf[1] = {h->jac[0]};
@@ -1542,6 +1677,7 @@ int sifv_destroy(sifv_handle* h) {
     hipFree(h->part);
     hipFree(h->visc);
     for (int q = 0; q < 7; ++q) hipFree(h->pcgv[q]);
+    hipFree(h->mgBuf);
     hipFree(h->pcgSc);
     hipFree(h->pcgSt);
     if (h->pcgStHost) hipHostFree(h->pcgStHost);
@@ -2078,18 +2214,19 @@ int sifv_jacobi_solve(sifv_handle* h, const double* rho, const double* rhoc2, co
 
 int sifv_set_pressure_solver(sifv_handle* h, int kind) {
     if (!h) return fail(SIFV_ERR_INVALID, "null handle");
-    if (kind != SIFV_PS_JACOBI && kind != SIFV_PS_PCG) return fail(SIFV_ERR_INVALID, "unknown pressure solver");
-    if (kind == SIFV_PS_PCG && !h->pcgSym)
+    if (kind != SIFV_PS_JACOBI && kind != SIFV_PS_PCG && kind != SIFV_PS_MGPCG)
+        return fail(SIFV_ERR_INVALID, "unknown pressure solver");
+    if (kind != SIFV_PS_JACOBI && !h->pcgSym)
         return fail(SIFV_ERR_UNSUPPORTED,
                     "PCG needs a symmetric system: a periodic axis with stretched wrap widths (use Jacobi)");
-    if (kind == SIFV_PS_PCG && h->jacMixed)
+    if (kind != SIFV_PS_JACOBI && h->jacMixed)
         return fail(SIFV_ERR_UNSUPPORTED, "the PCG pressure solve runs in FP64 (no mixed-precision form)");
     h->psolver = kind;
     return SIFV_OK;
 }
 
-int sifv_pcg_solve(sifv_handle* h, const double* rho, const double* rhoc2, const double* rhs, double* pressure,
-                   double dt, double tolerance, int maxIter, int* iters) {
+static int pcg_solve_host(sifv_handle* h, const double* rho, const double* rhoc2, const double* rhs,
+                          double* pressure, double dt, double tolerance, int maxIter, int* iters, bool mg) {
     if (!h || !rho || !rhoc2 || !rhs || !pressure) return fail(SIFV_ERR_INVALID, "null argument");
     if (maxIter <= 0) return fail(SIFV_ERR_INVALID, "maxIter must be > 0");
     if (!(tolerance > 0.0)) return fail(SIFV_ERR_INVALID, "tolerance must be > 0");
@@ -2103,12 +2240,22 @@ int sifv_pcg_solve(sifv_handle* h, const double* rho, const double* rhoc2, const
     HIPCHK(hipMemcpy(h->jac[0], pressure, nb, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->scal + 3, &dt, sizeof(double), hipMemcpyHostToDevice));
     int n = 0;
-    rc = run_pcg(h, h->jac[5], h->scal + 3, tolerance, maxIter, &n);
+    rc = run_pcg(h, h->jac[5], h->scal + 3, tolerance, maxIter, &n, mg);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(pressure, h->jac[0], nb, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (iters) *iters = n;
     return SIFV_OK;
+}
+
+int sifv_pcg_solve(sifv_handle* h, const double* rho, const double* rhoc2, const double* rhs, double* pressure,
+                   double dt, double tolerance, int maxIter, int* iters) {
+    return pcg_solve_host(h, rho, rhoc2, rhs, pressure, dt, tolerance, maxIter, iters, false);
+}
+
+int sifv_mgpcg_solve(sifv_handle* h, const double* rho, const double* rhoc2, const double* rhs, double* pressure,
+                     double dt, double tolerance, int maxIter, int* iters) {
+    return pcg_solve_host(h, rho, rhoc2, rhs, pressure, dt, tolerance, maxIter, iters, true);
 }
 
 // Runs nsteps explicit steps with the fixed dt from the resident state and

@@ -275,7 +275,9 @@ __global__ void __launch_bounds__(256) pcg_beta_kernel(PcgArgs A) {
         if (rr <= A.tol2 * A.sc[3]) {
             A.st[0] = 1;
             A.st[1] = it + 1;
-        } else if (it + 1 >= A.maxIter) {
+        } else if (it + 1 >= A.maxIter || !(rzn > 0.0)) {
+            // a preconditioned residual with r.z <= 0 (rounding at the end of
+            // a solve) stops the solve as not converged
             A.st[0] = 1;
             A.st[1] = A.maxIter;
         } else {
@@ -295,6 +297,289 @@ __global__ void __launch_bounds__(256) pcg_dir_kernel(PcgArgs A) {
     for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
         A.p[id] = A.z[id] + beta * A.p[id];
+    }
+}
+
+// ---- Multigrid V-cycle preconditioner --------------------------------------
+// One V-cycle of geometric multigrid on S replaces z = r / d when the solver
+// runs in its multigrid form (the role of the PFMG V-cycle that the reference's
+// HyprePressureSolver attaches to StructPCG: weighted-Jacobi relaxation, one
+// pre- and one post-sweep, src/HyprePressureSolver.cpp:123-138).  Full 2:1
+// coarsening of every axis with piecewise-constant transfers; a coarse cell C
+// owns the children {2c, 2c+1} per axis that exist on the finer level and
+//   s_C   = sum of the children's mass terms V/c,
+//   w_d,C = f_d * (sum of the finer low-face weights along d of the children
+//           on C's low side), f_d = 1/2 when the axis was coarsened, else 1,
+//   d_C   = s_C + sum_d (w_d,C + w_d,C+e_d (wrapped)).
+// Coarse levels are compact x-fastest arrays without ghosts (a low-face
+// weight at index 0 is the wrap face of a periodic axis, 0 otherwise, so the
+// wrapped neighbour index is always safe to read).  V-cycle on a level with
+// right-hand side b: x = om b / d; restrict b - A x by child sums; V-cycle on
+// the next level; x += t(parent); out = x + om (b - A x) / d; on the coarsest
+// level kMgCoarseSweeps weighted-Jacobi sweeps from zero.  om = 2/3, 4/5, 6/7
+// for 1D, 2D, 3D.  Each step is linear and symmetric, so the preconditioner
+// is a fixed SPD operator and conjugate gradients stay valid.
+// Fine-level work per V-cycle and cell: pre (read r d, write q: 24 B),
+// restrict (read q with neighbours, rho, s, r: 32 B), prolong (q: 16 B),
+// post (read q rho s r d, write z: 48 B).
+constexpr int kMgMaxLevels = 24;
+constexpr int kMgCoarseSweeps = 16;
+
+__host__ __device__ constexpr double mg_omega(int dim) {
+    return dim == 1 ? 2.0 / 3.0 : dim == 2 ? 0.8 : 6.0 / 7.0;
+}
+
+struct MgLevel {
+    int n[3];
+    int cells;
+    double *s, *w[3], *d, *b, *x, *t;
+};
+
+struct MgArgs {
+    PcgArgs A;            // fine level (padded fields)
+    MgLevel F;            // finer compact level (unused when fine is the padded grid)
+    MgLevel C;            // this / the coarser compact level
+    const int* gate;      // skip when *gate != 0 (solve finished); null: always run
+    const double* src;    // smoothing source
+    double* dst;          // smoothing destination
+    int slot;             // partial-sum slot of mg_rz_kernel
+    int init;             // mg_rz_kernel: also p = z
+};
+
+__device__ __forceinline__ bool mg_skip(const MgArgs& M) { return M.gate && *M.gate; }
+
+__device__ __forceinline__ int mg_idx(const MgLevel& L, int i, int j, int k) {
+    return i + L.n[0] * (j + L.n[1] * k);
+}
+
+template <int DIM>
+__device__ __forceinline__ double mg_row(const MgLevel& L, const double* v, int i, int j, int k) {
+    const int id = mg_idx(L, i, j, k);
+    const double vi = v[id];
+    double acc = L.s[id] * vi;
+#pragma unroll
+    for (int dir = 0; dir < DIM; ++dir) {
+        const int n = L.n[dir];
+        const int c = dir == 0 ? i : dir == 1 ? j : k;
+        const int cm = c == 0 ? n - 1 : c - 1, cp = c == n - 1 ? 0 : c + 1;
+        const int lo = dir == 0 ? mg_idx(L, cm, j, k) : dir == 1 ? mg_idx(L, i, cm, k) : mg_idx(L, i, j, cm);
+        const int hi = dir == 0 ? mg_idx(L, cp, j, k) : dir == 1 ? mg_idx(L, i, cp, k) : mg_idx(L, i, j, cp);
+        acc += L.w[dir][id] * (vi - v[lo]);
+        acc += L.w[dir][hi] * (vi - v[hi]);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ void mg_coords(const MgLevel& L, int c, int& i, int& j, int& k) {
+    i = c % L.n[0];
+    const int r = c / L.n[0];
+    j = r % L.n[1];
+    k = r / L.n[1];
+}
+
+// Coarse operator of level C from the fine grid (F unused) or from level F.
+template <int DIM, bool FROM_FINE>
+__global__ void __launch_bounds__(256) mg_build_kernel(MgArgs M) {
+    const MgLevel& C = M.C;
+    const Grid& g = M.A.g;
+    const int nf0 = FROM_FINE ? g.nx : M.F.n[0], nf1 = FROM_FINE ? g.ny : M.F.n[1], nf2 = FROM_FINE ? g.nz : M.F.n[2];
+    const double f0 = nf0 != C.n[0] ? 0.5 : 1.0, f1 = nf1 != C.n[1] ? 0.5 : 1.0, f2 = nf2 != C.n[2] ? 0.5 : 1.0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C.cells; c += gridDim.x * blockDim.x) {
+        int ci, cj, ck;
+        mg_coords(C, c, ci, cj, ck);
+        double ss = 0.0, w0 = 0.0, w1 = 0.0, w2 = 0.0;
+        for (int k = 2 * ck; k < 2 * ck + 2 && k < nf2; ++k)
+            for (int j = 2 * cj; j < 2 * cj + 2 && j < nf1; ++j)
+                for (int i = 2 * ci; i < 2 * ci + 2 && i < nf0; ++i) {
+                    if constexpr (FROM_FINE) {
+                        ss += M.A.s[cell_index(g, i, j, k)];
+                        double w;
+                        if (i == 2 * ci) {
+                            pcg_face<DIM>(M.A, 0, 0, i, j, k, w);
+                            w0 += w;
+                        }
+                        if (DIM >= 2 && j == 2 * cj) {
+                            pcg_face<DIM>(M.A, 1, 0, i, j, k, w);
+                            w1 += w;
+                        }
+                        if (DIM >= 3 && k == 2 * ck) {
+                            pcg_face<DIM>(M.A, 2, 0, i, j, k, w);
+                            w2 += w;
+                        }
+                    } else {
+                        const int f = mg_idx(M.F, i, j, k);
+                        ss += M.F.s[f];
+                        if (i == 2 * ci) w0 += M.F.w[0][f];
+                        if (DIM >= 2 && j == 2 * cj) w1 += M.F.w[1][f];
+                        if (DIM >= 3 && k == 2 * ck) w2 += M.F.w[2][f];
+                    }
+                }
+        C.s[c] = ss;
+        C.w[0][c] = f0 * w0;
+        C.w[1][c] = DIM >= 2 ? f1 * w1 : 0.0;
+        C.w[2][c] = DIM >= 3 ? f2 * w2 : 0.0;
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) mg_diag_kernel(MgArgs M) {
+    const MgLevel& C = M.C;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C.cells; c += gridDim.x * blockDim.x) {
+        int i, j, k;
+        mg_coords(C, c, i, j, k);
+        double dd = C.s[c];
+#pragma unroll
+        for (int dir = 0; dir < DIM; ++dir) {
+            const int n = C.n[dir];
+            const int cc = dir == 0 ? i : dir == 1 ? j : k;
+            const int cp = cc == n - 1 ? 0 : cc + 1;
+            const int up = dir == 0 ? mg_idx(C, cp, j, k) : dir == 1 ? mg_idx(C, i, cp, k) : mg_idx(C, i, j, cp);
+            dd += C.w[dir][c];
+            dd += C.w[dir][up];
+        }
+        C.d[c] = dd;
+    }
+}
+
+// fine pre-smooth: q = om r / d
+template <int DIM>
+__global__ void __launch_bounds__(256) mg_pre_fine_kernel(MgArgs M) {
+    if (mg_skip(M)) return;
+    const PcgArgs& A = M.A;
+    const Grid& g = A.g;
+    constexpr double om = mg_omega(DIM);
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
+        const size_t id = cell_index(g, i, j, k);
+        A.q[id] = om * A.r[id] / A.d[id];
+    }
+}
+
+// coarse right-hand side from the fine residual r - S q
+template <int DIM>
+__global__ void __launch_bounds__(256) mg_restrict_fine_kernel(MgArgs M) {
+    if (mg_skip(M)) return;
+    const PcgArgs& A = M.A;
+    const Grid& g = A.g;
+    const MgLevel& C = M.C;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C.cells; c += gridDim.x * blockDim.x) {
+        int ci, cj, ck;
+        mg_coords(C, c, ci, cj, ck);
+        double acc = 0.0;
+        for (int k = 2 * ck; k < 2 * ck + 2 && k < g.nz; ++k)
+            for (int j = 2 * cj; j < 2 * cj + 2 && j < g.ny; ++j)
+                for (int i = 2 * ci; i < 2 * ci + 2 && i < g.nx; ++i) {
+                    const size_t id = cell_index(g, i, j, k);
+                    acc += A.r[id] - pcg_row<DIM>(A, A.q, A.s[id], i, j, k, id);
+                }
+        C.b[c] = acc;
+    }
+}
+
+// q += t(parent)
+__global__ void __launch_bounds__(256) mg_prolong_fine_kernel(MgArgs M) {
+    if (mg_skip(M)) return;
+    const PcgArgs& A = M.A;
+    const Grid& g = A.g;
+    const MgLevel& C = M.C;
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
+        const size_t id = cell_index(g, i, j, k);
+        A.q[id] += C.t[mg_idx(C, i >> 1, j >> 1, k >> 1)];
+    }
+}
+
+// fine post-smooth: z = q + om (r - S q) / d
+template <int DIM>
+__global__ void __launch_bounds__(256) mg_post_fine_kernel(MgArgs M) {
+    if (mg_skip(M)) return;
+    const PcgArgs& A = M.A;
+    const Grid& g = A.g;
+    constexpr double om = mg_omega(DIM);
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
+        const size_t id = cell_index(g, i, j, k);
+        const double a = pcg_row<DIM>(A, A.q, A.s[id], i, j, k, id);
+        A.z[id] = A.q[id] + om * (A.r[id] - a) / A.d[id];
+    }
+}
+
+// coarse pre-smooth: x = om b / d
+template <int DIM>
+__global__ void __launch_bounds__(256) mg_pre_kernel(MgArgs M) {
+    if (mg_skip(M)) return;
+    const MgLevel& C = M.C;
+    constexpr double om = mg_omega(DIM);
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C.cells; c += gridDim.x * blockDim.x)
+        C.x[c] = om * C.b[c] / C.d[c];
+}
+
+// coarser right-hand side from level F's residual b - A x
+template <int DIM>
+__global__ void __launch_bounds__(256) mg_restrict_kernel(MgArgs M) {
+    if (mg_skip(M)) return;
+    const MgLevel& F = M.F;
+    const MgLevel& C = M.C;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C.cells; c += gridDim.x * blockDim.x) {
+        int ci, cj, ck;
+        mg_coords(C, c, ci, cj, ck);
+        double acc = 0.0;
+        for (int k = 2 * ck; k < 2 * ck + 2 && k < F.n[2]; ++k)
+            for (int j = 2 * cj; j < 2 * cj + 2 && j < F.n[1]; ++j)
+                for (int i = 2 * ci; i < 2 * ci + 2 && i < F.n[0]; ++i)
+                    acc += F.b[mg_idx(F, i, j, k)] - mg_row<DIM>(F, F.x, i, j, k);
+        C.b[c] = acc;
+    }
+}
+
+// level F: x += t(parent) from level C
+__global__ void __launch_bounds__(256) mg_prolong_kernel(MgArgs M) {
+    if (mg_skip(M)) return;
+    const MgLevel& F = M.F;
+    const MgLevel& C = M.C;
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < F.cells; f += gridDim.x * blockDim.x) {
+        int i, j, k;
+        mg_coords(F, f, i, j, k);
+        F.x[f] += C.t[mg_idx(C, i >> 1, j >> 1, k >> 1)];
+    }
+}
+
+// dst = src + om (b - A src) / d on level C
+template <int DIM>
+__global__ void __launch_bounds__(256) mg_smooth_kernel(MgArgs M) {
+    if (mg_skip(M)) return;
+    const MgLevel& C = M.C;
+    constexpr double om = mg_omega(DIM);
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C.cells; c += gridDim.x * blockDim.x) {
+        int i, j, k;
+        mg_coords(C, c, i, j, k);
+        M.dst[c] = M.src[c] + om * (C.b[c] - mg_row<DIM>(C, M.src, i, j, k)) / C.d[c];
+    }
+}
+
+// partial sums of r.z into slot M.slot (and p = z when M.init)
+__global__ void __launch_bounds__(256) mg_rz_kernel(MgArgs M) {
+    if (mg_skip(M)) return;
+    __shared__ double sh[256 / kWave];
+    const PcgArgs& A = M.A;
+    const Grid& g = A.g;
+    double rz = 0.0;
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
+        const size_t id = cell_index(g, i, j, k);
+        const double z = A.z[id];
+        rz += A.r[id] * z;
+        if (M.init) A.p[id] = z;
+    }
+    const double v[1] = {rz};
+    double t = wave_sum(v[0]);
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    if (l == 0) sh[w] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int u = 0; u < (int)(blockDim.x / kWave); ++u) s += sh[u];
+        A.part[(size_t)M.slot * A.nblk + blockIdx.x] = s;
     }
 }
 

@@ -179,10 +179,11 @@ class ExplicitSolver(_GpuSolverBase):
 
 class SemiImplicitSolver(_GpuSolverBase):
     """SemiImplicitSolver (include/SemiImplicitSolver.hpp:22-96) on the GPU,
-    with the Jacobi or the PCG pressure solver (chosen by the pressure
-    solver's name(), as the reference dispatches on the object passed in)."""
+    with the Jacobi, the PCG or the multigrid-preconditioned PCG pressure
+    solver (chosen by the pressure solver's name(), as the reference
+    dispatches on the object passed in)."""
 
-    _PS = {"Jacobi": 0, "PCG": 1}
+    _PS = {"Jacobi": 0, "PCG": 1, "MGPCG": 2}
 
     def __init__(self, mesh, riemannSolver, pressureSolver, eos, igrSolver, config, precision="fp64"):
         super().__init__(mesh, riemannSolver, eos, igrSolver, config, precision)
@@ -264,4 +265,27 @@ class PCGPressureSolver:
             raise ValueError("pressure must be a contiguous float64 array (updated in place)")
         _abi.check(lib.sifv_pcg_solve(self.gpu._h, _abi.dptr(arrs[0]), _abi.dptr(arrs[1]), _abi.dptr(arrs[2]),
                                       _abi.dptr(pressure), float(dt), float(tolerance), int(maxIter), C.byref(it)))
+        return it.value
+
+
+class MGPCGPressureSolver(PCGPressureSolver):
+    """PressureSolver::solve (include/PressureSolver.hpp:17-40) in the role of
+    HyprePressureSolver's StructPCG + PFMG pairing (src/HyprePressureSolver.cpp:
+    123-138, 312-408): conjugate gradients on the GPU preconditioned by one
+    geometric-multigrid V-cycle (sifv_mgpcg_solve), stopping on the relative
+    two-norm of the residual.  Passed to SemiImplicitSolver it selects the
+    multigrid PCG solve of the semi-implicit step."""
+
+    def name(self):
+        return "MGPCG"
+
+    def solve(self, mesh, rho, rhoc2, rhs, pressure, dt, tolerance, maxIter):
+        lib = _abi.load_library()
+        it = C.c_int()
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, rhoc2, rhs)]
+        if not (pressure.flags.c_contiguous and pressure.dtype == np.float64):
+            raise ValueError("pressure must be a contiguous float64 array (updated in place)")
+        _abi.check(lib.sifv_mgpcg_solve(self.gpu._h, _abi.dptr(arrs[0]), _abi.dptr(arrs[1]), _abi.dptr(arrs[2]),
+                                        _abi.dptr(pressure), float(dt), float(tolerance), int(maxIter),
+                                        C.byref(it)))
         return it.value

@@ -124,6 +124,38 @@ def jacobi_problem(n: int = 10, seed: int = 9):
     return cfg, mesh, rho, rhoc2, rhs, p0, 1e-4
 
 
+def pressure_problem_nd(shape, periods, seed: int = 21):
+    """Pressure system on a (nx, ny, nz) mesh of dimension 1-3 (trailing axes
+    of length 1 inactive): stretched widths on non-periodic axes, uniform on
+    periodic ones; a dense blob (1000:1 density, water/air rho c^2) in a light
+    background; rhs ~ 1e5 with noise."""
+    rng = np.random.default_rng(seed)
+    dim = 3 if shape[2] > 1 else 2 if shape[1] > 1 else 1
+    cfg = SimulationConfig(dim=dim, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5, semiImplicit=True)
+    nodes = []
+    for d in range(3):
+        n = shape[d]
+        if periods[d]:
+            nodes.append(np.linspace(0.0, 1.0, n + 1))
+        else:
+            w = 0.5 + rng.random(n)
+            nodes.append(np.concatenate([[0.0], np.cumsum(w / w.sum())]))
+    mesh = RectilinearMesh(cfg, nodes[0], nodes[1], nodes[2], periods=tuple(periods))
+    N = mesh.totalCells()
+    xc, yc, zc = mesh.centroids()
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    r2 = (X - 0.4) ** 2 + ((Y - 0.6) ** 2 if dim >= 2 else 0.0) + ((Z - 0.5) ** 2 if dim >= 3 else 0.0)
+    inside = r2 < 0.25 ** 2
+    rho = np.ones(N)
+    mesh.interior(rho)[...] = np.where(inside, 1000.0, 1.0) * (1.0 + 0.1 * rng.random(inside.shape))
+    rho = _fill_scalar_np(mesh, rho)
+    rhoc2 = np.zeros(N)
+    mesh.interior(rhoc2)[...] = np.where(inside, 2.25e9, 1.4e5)
+    rhs = np.zeros(N)
+    mesh.interior(rhs)[...] = 1e5 + 1e3 * rng.standard_normal(inside.shape)
+    return cfg, mesh, rho, rhoc2, rhs
+
+
 def _fill_scalar_np(mesh: RectilinearMesh, a: np.ndarray) -> np.ndarray:
     v = a.reshape(mesh.shape3()).copy()
     gx, gy, gz = mesh.ngx(), mesh.ngy(), mesh.ngz()
