@@ -201,6 +201,7 @@ struct sifv_handle {
     int psolver = SIFV_PS_JACOBI;  // pressure solver of the semi-implicit step
     bool pcgSym = true;       // no periodic axis with stretched wrap widths (PCG usable)
     double* pcgv[7] = {};     // PCG vectors r z p q d s and the partial sums
+    double* pcgW[3] = {};     // PCG face weights (one padded field per active axis)
     double* pcgSc = nullptr;  // PCG device scalars
     int* pcgSt = nullptr;     // PCG device flags
     int* pcgStHost = nullptr; // pinned copy of the flags
@@ -989,6 +990,9 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
     if (!h->pcgSym)
         return fail(SIFV_ERR_UNSUPPORTED,
                     "PCG needs a symmetric system: a periodic axis with stretched wrap widths (use Jacobi)");
+    // the face weights keep the high boundary face of each line in the ghost
+    // slot after its last cell
+    if (h->ng < 1) return fail(SIFV_ERR_UNSUPPORTED, "the PCG pressure solve needs at least one ghost layer");
     const int nb = elementwise_blocks(h);
     if (!h->pcgv[0] || h->pcgNblk < nb) {
         HIPCHK(hipStreamSynchronize(h->stream));
@@ -999,6 +1003,11 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
         for (int q = 0; q < 6; ++q) {
             HIPCHK(hipMalloc(&h->pcgv[q], sizeof(double) * h->total));
             HIPCHK(hipMemset(h->pcgv[q], 0, sizeof(double) * h->total));
+        }
+        for (int d = 0; d < h->dim; ++d) {
+            if (h->pcgW[d]) continue;
+            HIPCHK(hipMalloc(&h->pcgW[d], sizeof(double) * h->total));
+            HIPCHK(hipMemset(h->pcgW[d], 0, sizeof(double) * h->total));
         }
         HIPCHK(hipMalloc(&h->pcgv[6], sizeof(double) * 3 * nb));
         if (!h->pcgSc) HIPCHK(hipMalloc(&h->pcgSc, sizeof(double) * 8));
@@ -1020,6 +1029,7 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
     A.d = h->pcgv[4];
     A.s = h->pcgv[5];
     A.part = h->pcgv[6];
+    for (int d = 0; d < 3; ++d) A.W[d] = h->pcgW[d];
     A.nblk = nb;
     A.sc = h->pcgSc;
     A.st = h->pcgSt;
@@ -1092,7 +1102,8 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
         if (!r) r = mlaunch(by_dim(dim, mg_post_fine_kernel<1>, mg_post_fine_kernel<2>, mg_post_fine_kernel<3>), nb);
         return r;
     };
-    int rc = launch(setup, nb);
+    int rc = launch(by_dim(h->dim, pcg_weights_kernel<1>, pcg_weights_kernel<2>, pcg_weights_kernel<3>), nb);
+    if (!rc) rc = launch(setup, nb);
     if (!rc && mg) {
         rc = ensure_mg(h);
         for (int l = 0; l < h->mgLevels && !rc; ++l) {
@@ -1677,6 +1688,7 @@ int sifv_destroy(sifv_handle* h) {
     hipFree(h->part);
     hipFree(h->visc);
     for (int q = 0; q < 7; ++q) hipFree(h->pcgv[q]);
+    for (int d = 0; d < 3; ++d) hipFree(h->pcgW[d]);
     hipFree(h->mgBuf);
     hipFree(h->pcgSc);
     hipFree(h->pcgSt);

@@ -18,16 +18,23 @@
 // Layout: every vector is a padded field like the state's; only interior
 // cells are read or written (a periodic neighbour is read at its wrapped
 // interior index, so no ghost refresh is needed between kernels).  The face
-// weights are recomputed from rho in every product instead of stored: a
-// stored set would cost 2*DIM more doubles per cell per product.  Scalars
+// weights are computed once per solve (pcg_weights_kernel) into DIM padded
+// fields W[d]: W[d] at a cell holds the weight of its low face along d, and
+// the slot one past the last cell of a line holds the line's high boundary
+// face (0 on a wall, the wrap face of a periodic axis), so a row reads its
+// high weight from the next slot.  Each weight is the expression of the row
+// that owns the slot (the other row's operands in the same order), so
+// products equal those of recomputing the weights per row, up to the
+// compiler's choice of fused multiply-adds, without the 2*DIM FP64 divisions
+// per cell.  Scalars
 // (alpha, beta, the dot products) stay on the device; each product and update
 // kernel writes per-block partial sums that a one-block kernel folds in fixed
 // order, so a solve is deterministic run to run.  A device flag ends the
 // iteration: the host enqueues iterations in chunks and polls the flag.
 //
 // Per iteration, per interior cell: pcg_matvec (read p with its 2*DIM
-// neighbours, rho likewise, s; write q: 32 B), pcg_update (read x p r q d s,
-// write x r z: 72 B), pcg_dir (read z p, write p: 24 B) -- 128 B/cell,
+// neighbours, W, s; write q: 24 + 8 DIM B), pcg_update (read x p r q d s,
+// write x r z: 72 B), pcg_dir (read z p, write p: 24 B) -- 144 B/cell in 3D,
 // HBM-bound.
 #pragma once
 
@@ -43,6 +50,7 @@ struct PcgArgs {
     const double* dt;      // device-resident time step
     double* x;             // solution (initial guess in, answer out)
     double *r, *z, *p, *q, *d, *s;
+    double* W[3];          // face weights (see above)
     double* part;          // 3 x nblk per-block partial sums
     int nblk;
     double* sc;            // [0] rz [1] alpha [2] beta [3] bb [4] rr
@@ -86,24 +94,70 @@ __device__ __forceinline__ size_t pcg_face(const PcgArgs& A, int dir, int side, 
     return cell_index(g, wi, wj, wk);
 }
 
+// Interior index of the neighbour across the low (side 0) / high (side 1)
+// face along dir: the wrapped interior cell on a periodic domain face, the
+// cell itself on a wall (whose weight is 0).
+__device__ __forceinline__ size_t pcg_nb(const Grid& g, int dir, int side, int i, int j, int k, size_t id,
+                                         size_t stride) {
+    const int n = dir == 0 ? g.nx : dir == 1 ? g.ny : g.nz;
+    const int c = dir == 0 ? i : dir == 1 ? j : k;
+    if (side == 0) {
+        if (c > 0) return id - stride;
+        return g.sbc[2 * dir] == BK_WRAP ? id + (size_t)(n - 1) * stride : id;
+    }
+    if (c < n - 1) return id + stride;
+    return g.sbc[2 * dir + 1] == BK_WRAP ? id - (size_t)(n - 1) * stride : id;
+}
+
 // (S v)_i = (V/c)_i v_i + sum_f w_f (v_i - v_nb): neighbour differences first,
 // so a smooth v (pressures ~1e5 with small variations) keeps its precision
 // where the face terms outweigh V/c by many orders (fine meshes, large dt).
 template <int DIM>
 __device__ __forceinline__ double pcg_row(const PcgArgs& A, const double* v, double si, int i, int j, int k,
                                           size_t id) {
+    const Grid& g = A.g;
     const double vi = v[id];
     double acc = si * vi;
 #pragma unroll
     for (int dir = 0; dir < DIM; ++dir) {
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            double w;
-            const size_t nb = pcg_face<DIM>(A, dir, side, i, j, k, w);
-            acc += w * (vi - v[nb]);
-        }
+        const size_t stride = dir == 0 ? 1 : dir == 1 ? (size_t)g.nxT : (size_t)g.nxT * g.nyT;
+        const double wl = A.W[dir][id], wh = A.W[dir][id + stride];
+        acc += wl * (vi - v[pcg_nb(g, dir, 0, i, j, k, id, stride)]);
+        acc += wh * (vi - v[pcg_nb(g, dir, 1, i, j, k, id, stride)]);
     }
     return acc;
+}
+
+// Face weights W (low face of each cell; high face of the last cell of a
+// line in the slot after it), mass term s = V/c and diagonal d, from rho and
+// the widths with each row's own operand order.
+template <int DIM>
+__global__ void __launch_bounds__(256) pcg_weights_kernel(PcgArgs A) {
+    const Grid& g = A.g;
+    const double dt = *A.dt;
+    const double dt2 = dt * dt;
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
+        const size_t id = cell_index(g, i, j, k);
+        const double vol = g.dxw[i + g.ngx] * g.dyw[j + g.ngy] * g.dzw[k + g.ngz];
+        const double sc = vol / (A.rhoc2[id] * dt2);
+        double dd = sc;
+#pragma unroll
+        for (int dir = 0; dir < DIM; ++dir) {
+            const int n = dir == 0 ? g.nx : dir == 1 ? g.ny : g.nz;
+            const int c = dir == 0 ? i : dir == 1 ? j : k;
+            const size_t stride = dir == 0 ? 1 : dir == 1 ? (size_t)g.nxT : (size_t)g.nxT * g.nyT;
+            double wl, wh;
+            pcg_face<DIM>(A, dir, 0, i, j, k, wl);
+            pcg_face<DIM>(A, dir, 1, i, j, k, wh);
+            dd += wl;
+            dd += wh;
+            A.W[dir][id] = wl;
+            if (c == n - 1) A.W[dir][id + stride] = wh;
+        }
+        A.s[id] = sc;
+        A.d[id] = dd;
+    }
 }
 
 // Block sums of up to three values, written to part[q * nblk + blockIdx.x].
@@ -143,26 +197,12 @@ template <int DIM>
 __global__ void __launch_bounds__(256) pcg_setup_kernel(PcgArgs A) {
     __shared__ double sh[256 / kWave];
     const Grid& g = A.g;
-    const double dt = *A.dt;
-    const double dt2 = dt * dt;
     double bb = 0.0, rr = 0.0, rz = 0.0;
     int i, j, k;
     for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
-        const double vol = g.dxw[i + g.ngx] * g.dyw[j + g.ngy] * g.dzw[k + g.ngz];
-        const double sc = vol / (A.rhoc2[id] * dt2);
-        double dd = sc;
-#pragma unroll
-        for (int dir = 0; dir < DIM; ++dir) {
-#pragma unroll
-            for (int side = 0; side < 2; ++side) {
-                double w;
-                pcg_face<DIM>(A, dir, side, i, j, k, w);
-                dd += w;
-            }
-        }
-        A.s[id] = sc;
-        A.d[id] = dd;
+        const double sc = A.s[id];
+        const double dd = A.d[id];
         const double rhs = A.rhs[id];
         bb += rhs * rhs;
         const double r = sc * rhs - pcg_row<DIM>(A, A.x, sc, i, j, k, id);
@@ -392,20 +432,11 @@ __global__ void __launch_bounds__(256) mg_build_kernel(MgArgs M) {
             for (int j = 2 * cj; j < 2 * cj + 2 && j < nf1; ++j)
                 for (int i = 2 * ci; i < 2 * ci + 2 && i < nf0; ++i) {
                     if constexpr (FROM_FINE) {
-                        ss += M.A.s[cell_index(g, i, j, k)];
-                        double w;
-                        if (i == 2 * ci) {
-                            pcg_face<DIM>(M.A, 0, 0, i, j, k, w);
-                            w0 += w;
-                        }
-                        if (DIM >= 2 && j == 2 * cj) {
-                            pcg_face<DIM>(M.A, 1, 0, i, j, k, w);
-                            w1 += w;
-                        }
-                        if (DIM >= 3 && k == 2 * ck) {
-                            pcg_face<DIM>(M.A, 2, 0, i, j, k, w);
-                            w2 += w;
-                        }
+                        const size_t id = cell_index(g, i, j, k);
+                        ss += M.A.s[id];
+                        if (i == 2 * ci) w0 += M.A.W[0][id];
+                        if (DIM >= 2 && j == 2 * cj) w1 += M.A.W[1][id];
+                        if (DIM >= 3 && k == 2 * ck) w2 += M.A.W[2][id];
                     } else {
                         const int f = mg_idx(M.F, i, j, k);
                         ss += M.F.s[f];
