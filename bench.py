@@ -390,6 +390,31 @@ def measured_traffic(n, precision):
     return None
 
 
+# Issue model of one wave64 VALU instruction on MI355X: 4 cycles of a 16-lane
+# SIMD (FP64 and unpacked FP32 alike), 256 CUs x 4 SIMDs, 2.4 GHz peak clock.
+VALU_SIMDS = 1024
+VALU_CYCLES_PER_INST = 4
+CLOCK_HZ = 2.4e9
+
+
+def measured_valu(n, precision, mean_launch_s):
+    """VALU issue time per stage launch from the committed counter summary
+    (SQ_INSTS_VALU wave-instructions per dispatch) against the measured launch
+    time: the fraction of the launch the SIMDs spend issuing VALU work."""
+    name = f"stage_xy_weno5_hllc_{precision}_vortex{n}_pmc.md"
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("valu_wave_instructions_per_launch:"):
+            insts = float(line.split(":", 1)[1])
+            issue_s = insts * VALU_CYCLES_PER_INST / (VALU_SIMDS * CLOCK_HZ)
+            return {"wave_instructions": insts, "issue_ms": issue_s * 1e3,
+                    "frac": issue_s / mean_launch_s if mean_launch_s > 0 else None,
+                    "note": "VALU issue time / measured launch time; the FP64 stage kernel is VALU-bound"}
+    return None
+
+
 def cpu_baseline():
     """Reference step loop on the host cores (MPI ranks) over a bounded
     sample: 1024^2 vortex, 40 steps after one warm-up step."""
@@ -468,7 +493,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(a.n, a.precision),
                      "kernel": "stage_xy_kernel<2,3,HLLC>", "bytes_per_launch": bytes_per_launch,
-                     "mean_launch_ms": mean_launch_s * 1e3, "launches": res["launches"]},
+                     "mean_launch_ms": mean_launch_s * 1e3, "launches": res["launches"],
+                     "valu": measured_valu(a.n, a.precision, mean_launch_s)},
         "finite": res["finite"],
     }
     want = set() if a.no_variants or a.precision != "fp64" else set(a.variants.split(","))

@@ -390,8 +390,14 @@ __global__ void __launch_bounds__(256) jacobi32_kernel(Jacobi32Args A) {
 // loads by lanes 0 / 63 and rows 0 / kTileRows-1.  p goes through the scalar ghost
 // rule, rho through its filled ghosts.  Per cell the arithmetic is the elementwise
 // kernels' expression for expression, so both forms give identical bits.
-constexpr int kTileRows = 4;
-constexpr int kTilePlanes = 16;
+#ifndef SIFV_JACOBI_TILE_ROWS
+#define SIFV_JACOBI_TILE_ROWS 4
+#endif
+#ifndef SIFV_JACOBI_TILE_PLANES
+#define SIFV_JACOBI_TILE_PLANES 16
+#endif
+constexpr int kTileRows = SIFV_JACOBI_TILE_ROWS;
+constexpr int kTilePlanes = SIFV_JACOBI_TILE_PLANES;
 
 enum TileMode : int { TILE_SWEEP64 = 0, TILE_SWEEP32 = 1, TILE_PASS = 2 };
 
