@@ -171,18 +171,28 @@ public:
     }
 };
 
-// Drop-in for SemiImplicitSolver with the Jacobi pressure solver on the GPU
-// (include/SemiImplicitSolver.hpp:22-96).
+// Drop-in for SemiImplicitSolver (include/SemiImplicitSolver.hpp:22-96) with
+// its pressure solve on the GPU, chosen by the pressure solver's name() as
+// the reference dispatches on the object passed in: "PCG" -> conjugate
+// gradients, "MGPCG" or the reference's "HyprePCG+PFMG" -> the multigrid
+// preconditioned form, anything else -> Jacobi.
 class SemiImplicitSolver : public GpuSolverBase {
 public:
     SemiImplicitSolver(const RectilinearMesh&, std::shared_ptr<RiemannSolver> riemannSolver,
-                       std::shared_ptr<PressureSolver>, std::shared_ptr<EquationOfState> eos,
+                       std::shared_ptr<PressureSolver> pressureSolver, std::shared_ptr<EquationOfState> eos,
                        std::shared_ptr<IGRSolver>, const SimulationConfig& config, int precision = SIFV_FP64)
-        : GpuSolverBase(std::move(riemannSolver), std::move(eos), config, precision) {}
+        : GpuSolverBase(std::move(riemannSolver), std::move(eos), config, precision) {
+        const std::string n = pressureSolver ? pressureSolver->name() : std::string("Jacobi");
+        psKind_ = n == "PCG" ? SIFV_PS_PCG : (n == "MGPCG" || n == "HyprePCG+PFMG") ? SIFV_PS_MGPCG : SIFV_PS_JACOBI;
+    }
 
     double step(const SimulationConfig& config, const RectilinearMesh& mesh, SolutionState& state,
                 double targetDt = -1.0) {
         ensure(config, mesh);
+        if (!psSet_) {
+            if (psKind_ != SIFV_PS_JACOBI) check_status(sifv_set_pressure_solver(h_, psKind_));
+            psSet_ = true;
+        }
         upload(state);
         sifv_config c = to_c(config);
         double dt = 0.0;
@@ -194,38 +204,51 @@ public:
 
 private:
     int lastIters_ = 0;
+    int psKind_ = SIFV_PS_JACOBI;
+    bool psSet_ = false;
 };
 
 // PressureSolver backed by the GPU Jacobi sweep (src/JacobiPressureSolver.cpp:9-108):
 // drops into the reference's CPU SemiImplicitSolver through its vtable.
-class JacobiPressureSolver : public PressureSolver {
+// The GPU pressure solvers share one handle-owning base; kind selects the
+// entry point: SIFV_PS_JACOBI (sifv_jacobi_solve), SIFV_PS_PCG
+// (sifv_pcg_solve) or SIFV_PS_MGPCG (sifv_mgpcg_solve).
+class GpuPressureSolver : public PressureSolver {
 public:
-    JacobiPressureSolver(const SimulationConfig& config, std::shared_ptr<EquationOfState> eos,
-                         std::shared_ptr<RiemannSolver> rs)
-        : cfg_(config), eos_(std::move(eos)), rs_(std::move(rs)) {}
-    ~JacobiPressureSolver() override { sifv_destroy(h_); }
+    GpuPressureSolver(int kind, const SimulationConfig& config, std::shared_ptr<EquationOfState> eos,
+                      std::shared_ptr<RiemannSolver> rs)
+        : kind_(kind), cfg_(config), eos_(std::move(eos)), rs_(std::move(rs)) {}
+    ~GpuPressureSolver() override { sifv_destroy(h_); }
 
     int solve(const RectilinearMesh& mesh, const std::vector<double>& rho, const std::vector<double>& rhoc2,
               const std::vector<double>& rhs, std::vector<double>& pressure, double dt, double tolerance,
               int maxIter) override {
         ensure(mesh, nullptr);
-        int it = 0;
-        check_status(sifv_jacobi_solve(h_, rho.data(), rhoc2.data(), rhs.data(), pressure.data(), dt, tolerance,
-                                       maxIter, &it, nullptr));
-        return it;
+        return run(rho, rhoc2, rhs, pressure, dt, tolerance, maxIter);
     }
     int solve(const RectilinearMesh& mesh, const std::vector<double>& rho, const std::vector<double>& rhoc2,
               const std::vector<double>& rhs, std::vector<double>& pressure, double dt, double tolerance,
               int maxIter, HaloExchange& halo) override {
         ensure(mesh, &halo);
-        int it = 0;
-        check_status(sifv_jacobi_solve(h_, rho.data(), rhoc2.data(), rhs.data(), pressure.data(), dt, tolerance,
-                                       maxIter, &it, nullptr));
-        return it;
+        return run(rho, rhoc2, rhs, pressure, dt, tolerance, maxIter);
     }
-    std::string name() const override { return "Jacobi"; }
+    int kind() const { return kind_; }
 
 private:
+    int run(const std::vector<double>& rho, const std::vector<double>& rhoc2, const std::vector<double>& rhs,
+            std::vector<double>& pressure, double dt, double tolerance, int maxIter) {
+        int it = 0;
+        if (kind_ == SIFV_PS_PCG)
+            check_status(sifv_pcg_solve(h_, rho.data(), rhoc2.data(), rhs.data(), pressure.data(), dt, tolerance,
+                                        maxIter, &it));
+        else if (kind_ == SIFV_PS_MGPCG)
+            check_status(sifv_mgpcg_solve(h_, rho.data(), rhoc2.data(), rhs.data(), pressure.data(), dt, tolerance,
+                                          maxIter, &it));
+        else
+            check_status(sifv_jacobi_solve(h_, rho.data(), rhoc2.data(), rhs.data(), pressure.data(), dt,
+                                           tolerance, maxIter, &it, nullptr));
+        return it;
+    }
     void ensure(const RectilinearMesh& mesh, HaloExchange* halo) {
         if (h_) return;
         std::vector<double> x(mesh.nx() + 1), y(mesh.ny() + 1), z(mesh.nz() + 1);
@@ -253,10 +276,41 @@ private:
         int rt = n == "HLLC" ? SIFV_HLLC : (n == "Rusanov" ? SIFV_RUSANOV : SIFV_LF);
         check_status(sifv_create(&m, &c, rt, &e, SIFV_FP64, &h_));
     }
+    int kind_;
     SimulationConfig cfg_;
     std::shared_ptr<EquationOfState> eos_;
     std::shared_ptr<RiemannSolver> rs_;
     sifv_handle* h_ = nullptr;
+};
+
+// Drop-in for JacobiPressureSolver (src/JacobiPressureSolver.cpp:9-108).
+class JacobiPressureSolver : public GpuPressureSolver {
+public:
+    JacobiPressureSolver(const SimulationConfig& config, std::shared_ptr<EquationOfState> eos,
+                         std::shared_ptr<RiemannSolver> rs)
+        : GpuPressureSolver(SIFV_PS_JACOBI, config, std::move(eos), std::move(rs)) {}
+    std::string name() const override { return "Jacobi"; }
+};
+
+// In the HyprePressureSolver role (src/HyprePressureSolver.cpp:312-408):
+// conjugate gradients with a diagonal preconditioner.
+class PCGPressureSolver : public GpuPressureSolver {
+public:
+    PCGPressureSolver(const SimulationConfig& config, std::shared_ptr<EquationOfState> eos,
+                      std::shared_ptr<RiemannSolver> rs)
+        : GpuPressureSolver(SIFV_PS_PCG, config, std::move(eos), std::move(rs)) {}
+    std::string name() const override { return "PCG"; }
+};
+
+// In the HyprePressureSolver role with its StructPCG + PFMG pairing
+// (src/HyprePressureSolver.cpp:123-138): conjugate gradients preconditioned
+// by one geometric-multigrid V-cycle.
+class MGPCGPressureSolver : public GpuPressureSolver {
+public:
+    MGPCGPressureSolver(const SimulationConfig& config, std::shared_ptr<EquationOfState> eos,
+                        std::shared_ptr<RiemannSolver> rs)
+        : GpuPressureSolver(SIFV_PS_MGPCG, config, std::move(eos), std::move(rs)) {}
+    std::string name() const override { return "MGPCG"; }
 };
 
 }  // namespace gpu
