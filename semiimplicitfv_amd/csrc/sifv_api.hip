@@ -1035,6 +1035,7 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
     A.st = h->pcgSt;
     A.tol2 = tol * tol;
     A.maxIter = maxIter;
+    A.mgOmega = mg ? mg_omega(h->dim) : 0.0;
     const dim3 blk(256);
     void* args[] = {&A};
     auto launch = [&](const void* fn, int grid) {
@@ -1059,9 +1060,12 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
     const int dim = h->dim;
     auto cgrid = [](int cells) { return std::max(1, std::min((cells + 255) / 256, 8192)); };
     auto mlaunch = [&](const void* fn, int grid) { return timed_launch(h, fn, dim3(grid), blk, margs); };
-    auto vcycle = [&]() -> int {
+    // pre: run the fine pre-smooth (inside the iteration pcg_update_kernel
+    // has already written it)
+    auto vcycle = [&](bool pre) -> int {
         const int L = h->mgLevels;
-        int r = mlaunch(by_dim(dim, mg_pre_fine_kernel<1>, mg_pre_fine_kernel<2>, mg_pre_fine_kernel<3>), nb);
+        int r = pre ? mlaunch(by_dim(dim, mg_pre_fine_kernel<1>, mg_pre_fine_kernel<2>, mg_pre_fine_kernel<3>), nb)
+                    : SIFV_OK;
         M.C = h->mg[0];
         if (!r)
             r = mlaunch(by_dim(dim, mg_restrict_fine_kernel<1>, mg_restrict_fine_kernel<2>,
@@ -1119,10 +1123,9 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
             }
             if (!rc) rc = mlaunch(by_dim(dim, mg_diag_kernel<1>, mg_diag_kernel<2>, mg_diag_kernel<3>), cgrid(M.C.cells));
         }
-        if (!rc) rc = vcycle();
         M.slot = 2;
         M.init = 1;
-        if (!rc) rc = mlaunch(reinterpret_cast<const void*>(mg_rz_kernel), nb);
+        if (!rc) rc = vcycle(true);
         M.gate = A.st;
         M.slot = 1;
         M.init = 0;
@@ -1145,8 +1148,7 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
             rc = launch(matvec, nb);
             if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_alpha_kernel), 1);
             if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_update_kernel), nb);
-            if (!rc && mg) rc = vcycle();
-            if (!rc && mg) rc = mlaunch(reinterpret_cast<const void*>(mg_rz_kernel), nb);
+            if (!rc && mg) rc = vcycle(false);
             if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_beta_kernel), 1);
             if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_dir_kernel), nb);
             if (rc) return rc;

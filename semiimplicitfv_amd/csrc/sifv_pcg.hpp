@@ -57,6 +57,7 @@ struct PcgArgs {
     int* st;               // [0] done [1] iterations [2] iteration counter [3] zero x
     double tol2;           // tol^2
     int maxIter;
+    double mgOmega = 0.0;  // > 0: multigrid form, the update writes the V-cycle's pre-smooth q = om r / d
 };
 
 // Interior index of the neighbour across the low (side 0) / high (side 1)
@@ -297,9 +298,15 @@ __global__ void __launch_bounds__(256) pcg_update_kernel(PcgArgs A) {
         A.r[id] = r;
         const double ro = r / A.s[id];
         rr += ro * ro;
-        const double z = r / A.d[id];
-        A.z[id] = z;
-        rz += r * z;
+        if (A.mgOmega > 0.0) {
+            // the preconditioner's first fine-level pass (mg_pre_fine_kernel);
+            // q is free again until the next product
+            A.q[id] = A.mgOmega * r / A.d[id];
+        } else {
+            const double z = r / A.d[id];
+            A.z[id] = z;
+            rz += r * z;
+        }
     }
     pcg_partials(A, sh, 2, rr, rz, 0.0);
 }
@@ -359,9 +366,10 @@ __global__ void __launch_bounds__(256) pcg_dir_kernel(PcgArgs A) {
 // level kMgCoarseSweeps weighted-Jacobi sweeps from zero.  om = 2/3, 4/5, 6/7
 // for 1D, 2D, 3D.  Each step is linear and symmetric, so the preconditioner
 // is a fixed SPD operator and conjugate gradients stay valid.
-// Fine-level work per V-cycle and cell: pre (read r d, write q: 24 B),
-// restrict (read q with neighbours, rho, s, r: 32 B), prolong (q: 16 B),
-// post (read q rho s r d, write z: 48 B).
+// Fine-level work per V-cycle and cell: pre (q = om r / d, folded into
+// pcg_update_kernel inside the iteration), restrict (read q with neighbours,
+// W, s, r: 48 B in 3D), prolong (q: 16 B), post (read q W s r d, write z, with
+// the r.z partial sums: 64 B).
 constexpr int kMgMaxLevels = 24;
 constexpr int kMgCoarseSweeps = 16;
 
@@ -382,8 +390,8 @@ struct MgArgs {
     const int* gate;      // skip when *gate != 0 (solve finished); null: always run
     const double* src;    // smoothing source
     double* dst;          // smoothing destination
-    int slot;             // partial-sum slot of mg_rz_kernel
-    int init;             // mg_rz_kernel: also p = z
+    int slot;             // partial-sum slot of the r.z sums (mg_post_fine_kernel)
+    int init;             // mg_post_fine_kernel: also p = z
 };
 
 __device__ __forceinline__ bool mg_skip(const MgArgs& M) { return M.gate && *M.gate; }
@@ -520,18 +528,34 @@ __global__ void __launch_bounds__(256) mg_prolong_fine_kernel(MgArgs M) {
     }
 }
 
-// fine post-smooth: z = q + om (r - S q) / d
+// fine post-smooth: z = q + om (r - S q) / d, with the partial sums of r.z
+// into slot M.slot (and p = z when M.init)
 template <int DIM>
 __global__ void __launch_bounds__(256) mg_post_fine_kernel(MgArgs M) {
     if (mg_skip(M)) return;
+    __shared__ double sh[256 / kWave];
     const PcgArgs& A = M.A;
     const Grid& g = A.g;
     constexpr double om = mg_omega(DIM);
+    double rz = 0.0;
     int i, j, k;
     for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
         const double a = pcg_row<DIM>(A, A.q, A.s[id], i, j, k, id);
-        A.z[id] = A.q[id] + om * (A.r[id] - a) / A.d[id];
+        const double r = A.r[id];
+        const double z = A.q[id] + om * (r - a) / A.d[id];
+        A.z[id] = z;
+        rz += r * z;
+        if (M.init) A.p[id] = z;
+    }
+    const double t = wave_sum(rz);
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    if (l == 0) sh[w] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int u = 0; u < (int)(blockDim.x / kWave); ++u) s += sh[u];
+        A.part[(size_t)M.slot * A.nblk + blockIdx.x] = s;
     }
 }
 
@@ -588,30 +612,5 @@ __global__ void __launch_bounds__(256) mg_smooth_kernel(MgArgs M) {
     }
 }
 
-// partial sums of r.z into slot M.slot (and p = z when M.init)
-__global__ void __launch_bounds__(256) mg_rz_kernel(MgArgs M) {
-    if (mg_skip(M)) return;
-    __shared__ double sh[256 / kWave];
-    const PcgArgs& A = M.A;
-    const Grid& g = A.g;
-    double rz = 0.0;
-    int i, j, k;
-    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
-        const size_t id = cell_index(g, i, j, k);
-        const double z = A.z[id];
-        rz += A.r[id] * z;
-        if (M.init) A.p[id] = z;
-    }
-    const double v[1] = {rz};
-    double t = wave_sum(v[0]);
-    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
-    if (l == 0) sh[w] = t;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int u = 0; u < (int)(blockDim.x / kWave); ++u) s += sh[u];
-        A.part[(size_t)M.slot * A.nblk + blockIdx.x] = s;
-    }
-}
 
 }  // namespace sifv
