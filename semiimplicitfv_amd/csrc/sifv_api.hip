@@ -940,6 +940,7 @@ int run_jacobi_mixed(sifv_handle* h, const double* rho, const double* dt, double
 // coarsest still has an axis longer than 4 cells; 8 compact arrays per level.
 int ensure_mg(sifv_handle* h) {
     if (h->mgLevels) return SIFV_OK;
+    static_assert(kMgCoarsestCells == 64, "the coarsest level holds at most 4 x 4 x 4 cells");
     int n[3] = {h->nx, h->ny, h->nz};
     int L = 0;
     size_t total = 0;
@@ -953,6 +954,7 @@ int ensure_mg(sifv_handle* h) {
         total += 8 * (size_t)lv[L].cells;
         ++L;
     }
+    if (lv[L - 1].cells > kMgCoarsestCells) return fail(SIFV_ERR_HIP, "multigrid coarsest level too large");
     double* buf = nullptr;
     HIPCHK(hipMalloc(&buf, sizeof(double) * total));
     HIPCHK(hipMemset(buf, 0, sizeof(double) * total));
@@ -1071,24 +1073,21 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
             r = mlaunch(by_dim(dim, mg_restrict_fine_kernel<1>, mg_restrict_fine_kernel<2>,
                                mg_restrict_fine_kernel<3>),
                         cgrid(M.C.cells));
-        for (int l = 0; l < L && !r; ++l) {
+        for (int l = 0; l + 1 < L && !r; ++l) {
             M.C = h->mg[l];
             r = mlaunch(by_dim(dim, mg_pre_kernel<1>, mg_pre_kernel<2>, mg_pre_kernel<3>), cgrid(M.C.cells));
-            if (!r && l + 1 < L) {
+            if (!r) {
                 M.F = h->mg[l];
                 M.C = h->mg[l + 1];
                 r = mlaunch(by_dim(dim, mg_restrict_kernel<1>, mg_restrict_kernel<2>, mg_restrict_kernel<3>),
                             cgrid(M.C.cells));
             }
         }
-        // coarsest: sweeps 2..kMgCoarseSweeps alternate x -> t, t -> x and end in t
+        // coarsest: pre-smooth and all sweeps in one workgroup, answer in t
         M.C = h->mg[L - 1];
-        for (int sw = 1; sw < kMgCoarseSweeps && !r; ++sw) {
-            M.src = (sw & 1) ? M.C.x : M.C.t;
-            M.dst = (sw & 1) ? M.C.t : M.C.x;
-            r = mlaunch(by_dim(dim, mg_smooth_kernel<1>, mg_smooth_kernel<2>, mg_smooth_kernel<3>),
-                        cgrid(M.C.cells));
-        }
+        if (!r)
+            r = timed_launch(h, by_dim(dim, mg_coarsest_kernel<1>, mg_coarsest_kernel<2>, mg_coarsest_kernel<3>),
+                             dim3(1), dim3(kMgCoarsestCells), margs);
         for (int l = L - 2; l >= 0 && !r; --l) {
             M.F = h->mg[l];
             M.C = h->mg[l + 1];

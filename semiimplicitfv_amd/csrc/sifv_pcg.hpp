@@ -587,6 +587,41 @@ __global__ void __launch_bounds__(256) mg_restrict_kernel(MgArgs M) {
     }
 }
 
+// Coarsest level (at most 4 cells per axis, so at most kMgCoarsestCells):
+// x = om b / d, then kMgCoarseSweeps - 1 weighted-Jacobi sweeps alternating
+// x -> t, t -> x and ending in t, in one workgroup with the iterates in LDS;
+// the same arithmetic as mg_pre_kernel followed by mg_smooth_kernel launches.
+constexpr int kMgCoarsestCells = 64;
+
+template <int DIM>
+__global__ void __launch_bounds__(kMgCoarsestCells) mg_coarsest_kernel(MgArgs M) {
+    if (mg_skip(M)) return;
+    __shared__ double xs[kMgCoarsestCells], ts[kMgCoarsestCells];
+    const MgLevel& C = M.C;
+    constexpr double om = mg_omega(DIM);
+    const int c = threadIdx.x;
+    const bool on = c < C.cells;
+    int i = 0, j = 0, k = 0;
+    double b = 0.0, d = 1.0;
+    if (on) {
+        mg_coords(C, c, i, j, k);
+        b = C.b[c];
+        d = C.d[c];
+        xs[c] = om * b / d;
+    }
+    __syncthreads();
+    for (int sw = 1; sw < kMgCoarseSweeps; ++sw) {
+        const double* src = (sw & 1) ? xs : ts;
+        double* dst = (sw & 1) ? ts : xs;
+        if (on) dst[c] = src[c] + om * (b - mg_row<DIM>(C, src, i, j, k)) / d;
+        __syncthreads();
+    }
+    if (on) {
+        C.x[c] = xs[c];
+        C.t[c] = ts[c];
+    }
+}
+
 // level F: x += t(parent) from level C
 __global__ void __launch_bounds__(256) mg_prolong_kernel(MgArgs M) {
     if (mg_skip(M)) return;
