@@ -279,9 +279,11 @@ void stage_shape(const sifv_handle* h, int& nstrip, int& nseg, int& rows) {
     if (h->rowsOverride > 0) {
         rows = h->rowsOverride;
     } else {
-        // aim for >= ~8k waves so every CU holds several waves, but keep at
-        // least 8 rows per wave to amortise the 2-row warm-up
-        long target = 8192;
+        // aim for >= ~8k waves so every CU holds several waves (2D: ~16k,
+        // 16 rows per wave at 4096^2, the fastest of 8..128 rows measured for
+        // both the FP64 and the FP32-bulk kernels), but keep at least 8 rows
+        // per wave to amortise the 2-row warm-up
+        long target = h->dim == 2 ? 16384 : 8192;
         long r = (long)h->ny * nstrip * planes / target;
         rows = 8;
         while (rows < 64 && rows * 2 <= r) rows *= 2;
