@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
-    ap.add_argument("--variants", default="mixed,ensemble,adjoint,bubble,graphs,igr",
+    ap.add_argument("--variants", default="mixed,vortex1024,ensemble,adjoint,bubble,graphs,igr",
                     help="comma list of the side workloads to run with an FP64 main run")
     ap.add_argument("--bubble-n", type=int, default=512, help="cells per side of the 3D bubble workload")
     ap.add_argument("--rows", type=int, default=0, help="rows marched per wave (0 = built-in choice)")
@@ -68,6 +68,30 @@ def copy_probe(mib: int, reps: int = 5):
     hip.hipFree(a)
     hip.hipFree(b)
     print(json.dumps({"copy_probe_bytes": n, "reps": reps}))
+
+
+def accuracy_1024(steps: int = 1000):
+    """FP32-bulk against FP64 on the 1024^2 vortex after `steps` steps: rel L1
+    and Linf of rho, rhoU, rhoV, rhoE (the FP64 run is within 1e-12 of the
+    FP64 oracle by the parity tests)."""
+    import numpy as np
+
+    from semiimplicitfv_amd import ExplicitSolver, cases
+
+    fields = {}
+    for prec in ("fp64", "mixed"):
+        cfg, mesh, st, eos, rs = cases.isentropic_vortex_2d(1024)
+        g = ExplicitSolver(mesh, rs, eos, None, cfg, precision=prec)
+        g.upload(st)
+        g.advance(cfg, steps, 1e30, 0.0)
+        fields[prec] = {f: mesh.interior(g.download_field(f)).copy() for f in (0, 1, 2, 4)}
+        g.close()
+    out = {"steps": steps}
+    for f, name in ((0, "rho"), (1, "rhoU"), (2, "rhoV"), (4, "rhoE")):
+        a, b = fields["mixed"][f], fields["fp64"][f]
+        out[name] = {"rel_L1": float(np.abs(a - b).sum() / np.abs(b).sum()),
+                     "rel_Linf": float(np.abs(a - b).max() / np.abs(b).max())}
+    return out
 
 
 def run_gpu(n, steps, warmup, precision, rows=0):
@@ -415,32 +439,49 @@ def measured_valu(n, precision, mean_launch_s):
     return None
 
 
-def cpu_baseline():
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(n: int = 4096, steps: int = 3):
     """Reference step loop on the host cores (MPI ranks) over a bounded
-    sample: 1024^2 vortex, 40 steps after one warm-up step."""
+    sample of the GPU line's own workload: the n^2 vortex, `steps` steps
+    after one warm-up step.  Ranks: every core this process may use, capped
+    at 16, the host-core share that goes with one GPU on the MI355X boxes
+    (their process guard and OMP_NUM_THREADS=16 set that share)."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
-    n, steps = 1024, 40
-    ncores = min(16, len(os.sched_getaffinity(0)))
+    avail = len(os.sched_getaffinity(0))
+    ncores = min(16, avail)
     mpiexec = shutil.which("mpiexec") or ("/opt/conda/bin/mpiexec" if os.path.exists("/opt/conda/bin/mpiexec")
                                           else None)
-    sample = f"2D vortex {n}^2, WENO5+HLLC+RK3 FP64, {steps} steps (reference step loop, 1 warm-up step)"
+    model = cpu_model()
+    sample = (f"2D vortex {n}^2 (the GPU line's workload), WENO5+HLLC+RK3 FP64, {steps} steps "
+              f"(reference step loop, 1 warm-up step), {model}, {ncores} of {avail} visible cores")
     if os.path.exists(exe):
         tries = []
         if mpiexec and ncores > 1:
             tries.append(([mpiexec, "-n", str(ncores), exe, "vortex2d", str(n), str(steps), "hllc"], ncores))
-        tries.append(([exe, "vortex2d", str(n), str(max(4, steps // 8)), "hllc"], 1))
+        tries.append(([exe, "vortex2d", str(min(n, 1024)), "4", "hllc"], 1))
         for cmd, cores in tries:
             try:
                 r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=300)
                 if r.returncode == 0:
                     line = [l for l in r.stdout.decode().splitlines() if l.startswith("{")][-1]
                     d = json.loads(line)
-                    smp = sample if cores > 1 else sample.replace(f"{steps} steps", f"{d['steps']} steps")
+                    smp = sample if cores > 1 else (f"2D vortex {min(n, 1024)}^2, WENO5+HLLC+RK3 FP64, {d['steps']} "
+                                                    f"steps (reference step loop), {model}, 1 core")
                     return {"value": d["cell_updates_per_s"], "unit": "cell-updates/s", "cores": d["ranks"],
                             "kind": "reference", "sample": smp + f", {d['ranks']} MPI rank(s), {d['seconds']:.2f} s"}
             except Exception:
                 continue
-    # fallback: the oracle restatement, one core
+    # fallback: the oracle restatement, one core (smaller sample: one core)
     from oracle import pyoracle
     from semiimplicitfv_amd import cases
 
@@ -455,7 +496,7 @@ def cpu_baseline():
         cpu.explicit_step(st, cfg)
     el = time.perf_counter() - t0
     return {"value": 256 * 256 * k / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"2D vortex 256^2, {k} steps, C restatement, 1 core, {el:.2f} s"}
+            "sample": f"2D vortex 256^2, {k} steps, C restatement, 1 core of {model}, {el:.2f} s"}
 
 
 def main():
@@ -512,7 +553,23 @@ def main():
         out["variants"]["mixed"] = {"value": mv, "ratio_vs_fp64": mv / value, "rho_rel_L1_vs_fp64": l1,
                                     "rho_rel_Linf_vs_fp64": linf, "roofline_achieved_GBs": mb / mm / 1e9,
                                     "mean_launch_ms": mm * 1e3,
+                                    "accuracy_1024_1000_steps": accuracy_1024(),
                                     "note": "error against the FP64 GPU run, itself within 1e-12 of the FP64 oracle"}
+    if "vortex1024" in want:
+        t0 = time.perf_counter()
+        r64 = run_gpu(1024, 400, 20, "fp64")
+        r32 = run_gpu(1024, 400, 20, "mixed")
+        v64 = r64["cells"] * 400 / r64["seconds"]
+        v32 = r32["cells"] * 400 / r32["seconds"]
+        out["variants"]["vortex1024"] = {
+            "workload": "2D isentropic vortex 1024x1024, WENO5+HLLC+SSP-RK3, CFL 0.6, 400 steps",
+            "fp64": {"value": v64, "mean_launch_ms": r64["kernel_ms"] / max(r64["launches"], 1),
+                     "ratio_vs_4096": v64 / value},
+            "mixed": {"value": v32, "mean_launch_ms": r32["kernel_ms"] / max(r32["launches"], 1),
+                      "ratio_vs_fp64": v32 / v64},
+            "note": ("1M cells give ~3k waves of 62 columns x 6 rows at one wave per SIMD slot: the 2-row warm-up "
+                     "of each wave's y-window is 1/3 of its rows (1/8 at 4096^2, 16 rows)"),
+            "wall_s": time.perf_counter() - t0}
     def bubble_pair():
         import numpy as np
 
@@ -546,7 +603,7 @@ def main():
                 out["variants"][name] = {"error": repr(e)}
             out["variants"][name]["wall_s"] = time.perf_counter() - t0
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(a.n)
     print(json.dumps(out))
 
 

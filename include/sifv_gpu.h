@@ -56,8 +56,13 @@ enum { SIFV_XLOW = 0, SIFV_XHIGH = 1, SIFV_YLOW = 2, SIFV_YHIGH = 3,
  * (src/StiffenedGasEOS.cpp) */
 enum { SIFV_EOS_IDEAL = 0, SIFV_EOS_STIFFENED = 1 };
 /* Arithmetic of the bulk WENO/flux work.  SIFV_FP64 is the plain method;
- * SIFV_MIXED stores the state and runs WENO/flux arithmetic in FP32 while
- * the EOS energy<->pressure conversion, dt and conservation sums stay FP64. */
+ * SIFV_MIXED (explicit solver) stores the state in FP32 and runs WENO/flux
+ * arithmetic in FP32; a stiffened gas or two-phase state keeps its
+ * energy<->pressure conversion in FP64 and stores its energy without the
+ * stiffened-gas constant sum_k alpha_k gamma_k pInf_k/(gamma_k-1), so the FP32
+ * state keeps the pressure part; an ideal gas converts and updates in FP32.
+ * With the semi-implicit solver SIFV_MIXED keeps the FP64 state and runs the
+ * Jacobi sweeps in FP32 with FP64 residual refinement. */
 enum { SIFV_FP64 = 0, SIFV_MIXED = 1 };
 
 /* Field ids for upload/download (SolutionState members,

@@ -15,6 +15,7 @@
 #include <filesystem>
 #include <fstream>
 #include <iomanip>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -219,6 +220,18 @@ struct sifv_handle {
     double* adjGbar = nullptr;
     int adjN = 0, adjK = 0, adjNb = 0;
     double adjDt = 0.0;
+    // FP32 state of a stiffened gas (single or two phase, explicit): the
+    // stored energy excludes the stiffened-gas constant (Eos::eoff / Mix::eoff);
+    // uploads of the total energy wait in ePend (FP64) until the volume
+    // fractions they are taken relative to are known (settle_energy)
+    bool eoff = false;
+    double* ePend = nullptr;
+    bool ePending = false;
+    // device the handle was created on, and the allocation-free mode of
+    // sifv_explicit_advance_noalloc (dt partials in the library's static
+    // device array instead of `partial`; nothing may allocate)
+    int device = 0;
+    bool noalloc = false;
 
     size_t tsize() const { return precision == SIFV_MIXED ? sizeof(float) : sizeof(double); }
     Grid grid() const {
@@ -236,7 +249,12 @@ struct sifv_handle {
         e.gamma = eos.gamma;
         e.pInf = eos.kind == SIFV_EOS_STIFFENED ? eos.pInf : 0.0;
         e.stiffened = eos.kind == SIFV_EOS_STIFFENED;
+        e.eoff = (eoff && nph == 0 && e.stiffened) ? 1 : 0;
         return e;
+    }
+    // single-phase offset gamma pInf / (gamma - 1) of the stored energy
+    double eoffConst() const {
+        return (eoff && nph == 0 && eos.kind == SIFV_EOS_STIFFENED) ? eos.gamma * eos.pInf / (eos.gamma - 1.0) : 0.0;
     }
     Mix mix() const {
         Mix m;
@@ -246,6 +264,11 @@ struct sifv_handle {
             m.pInf[p] = p < nph ? cfg.multiPhaseParams.phases[p].pInf : 0.0;
         }
         m.alphaMin = cfg.multiPhaseParams.alphaMin;
+        m.eoff = (eoff && nph > 0) ? 1 : 0;
+        for (int p = 0; p < kMaxPhases; ++p) {
+            m.eC[p] = p < nph ? m.gamma[p] * m.pInf[p] / (m.gamma[p] - 1.0) : 0.0;
+            m.gi[p] = p < nph ? 1.0 / (m.gamma[p] - 1.0) : 0.0;
+        }
         return m;
     }
     template <typename T>
@@ -258,6 +281,7 @@ int run_igr_any(sifv_handle* h, int src, const int* active);
 
 int ensure_partials(sifv_handle* h, int n) {
     if (n <= h->npartialCap) return SIFV_OK;
+    if (h->noalloc) return fail(SIFV_ERR_INVALID, "dt partials exceed the allocation-free scratch");
     if (h->partial) HIPCHK(hipFree(h->partial));
     h->partial = nullptr;
     HIPCHK(hipMalloc(&h->partial, sizeof(double) * n));
@@ -281,12 +305,19 @@ void stage_shape(const sifv_handle* h, int& nstrip, int& nseg, int& rows) {
     } else {
         // aim for >= ~8k waves so every CU holds several waves (2D: ~16k,
         // 16 rows per wave at 4096^2, the fastest of 8..128 rows measured for
-        // both the FP64 and the FP32-bulk kernels), but keep at least 8 rows
-        // per wave to amortise the 2-row warm-up
+        // both the FP64 and the FP32-bulk kernels); meshes too small for 8 rows
+        // per wave at that count get about one wave per SIMD slot (3072 at three
+        // waves per SIMD) with 4..8 rows, trading the 2-row warm-up against
+        // filling the chip (1024^2: 6 rows, measured fastest of 4..16)
         long target = h->dim == 2 ? 16384 : 8192;
-        long r = (long)h->ny * nstrip * planes / target;
-        rows = 8;
-        while (rows < 64 && rows * 2 <= r) rows *= 2;
+        const long cols = (long)h->ny * nstrip * planes;
+        long r = cols / target;
+        if (r >= 8) {
+            rows = 8;
+            while (rows < 64 && rows * 2 <= r) rows *= 2;
+        } else {
+            rows = (int)std::max<long>(4, std::min<long>(8, (cols + 3071) / 3072));
+        }
     }
     rows = std::min(rows, h->ny);
     nseg = (h->ny + rows - 1) / rows;
@@ -321,7 +352,9 @@ StageKernels<double> stage_kernels<double>(const sifv_handle* h, bool semi) {
 
 template <>
 StageKernels<float> stage_kernels<float>(const sifv_handle* h, bool semi) {
-    if (semi || h->nph) return StageKernels<float>();
+    if (semi) return StageKernels<float>();
+    if (h->nph == 2) return stage_kernels_fp32_mp(h->dim, recon_radius(h->cfg.reconOrder), h->riemann);
+    if (h->nph) return StageKernels<float>();
     return stage_kernels_fp32(h->dim, recon_radius(h->cfg.reconOrder), h->riemann, stage_lean(h->cfg));
 }
 
@@ -379,7 +412,8 @@ int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const d
     P.active = active;
     P.eps = h->cfg.wenoEps;
     P.igr = h->cfg.useIGR ? 1 : 0;
-    P.weno = (h->cfg.reconOrder <= SIFV_WENO5) ? 1 : 0;
+    // FP32 WENO of a stiffened gas: indicators scaled before squaring
+    P.weno = (h->cfg.reconOrder <= SIFV_WENO5) ? (h->eoff ? 2 : 1) : 0;
     P.sigPrim = h->sigPrim ? 1 : 0;
     P.pres = h->pres;
     for (int d = 0; d < 3; ++d) P.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
@@ -490,10 +524,14 @@ int launch_dt_cells(sifv_handle* h, bool advective, const double* rho = nullptr,
     A.rho = rho ? rho : h->rhoP;
     for (int d = 0; d < 3; ++d) A.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
     A.pres = h->pres;
+    A.f32 = h->precision == SIFV_MIXED ? 1 : 0;
     for (int p = 0; p < kMaxPhases; ++p) {
         int pp = p < h->nph ? p : 0;
-        A.alphaRho[p] = h->nph ? h->consField<double>(h->cur, h->dim + 2 + pp) : nullptr;
-        A.alpha[p] = h->nph ? h->consField<double>(h->cur, h->dim + 2 + h->nph + pp) : nullptr;
+        const bool f = A.f32 != 0;
+        A.alphaRho[p] = (h->nph && !f) ? h->consField<double>(h->cur, h->dim + 2 + pp) : nullptr;
+        A.alpha[p] = (h->nph && !f) ? h->consField<double>(h->cur, h->dim + 2 + h->nph + pp) : nullptr;
+        A.alphaRho32[p] = (h->nph && f) ? h->consField<float>(h->cur, h->dim + 2 + pp) : nullptr;
+        A.alpha32[p] = (h->nph && f) ? h->consField<float>(h->cur, h->dim + 2 + h->nph + pp) : nullptr;
     }
     A.advective = advective ? 1 : 0;
     A.viscMu = (withStress && h->cfg.viscousMu > 0.0) ? h->cfg.viscousMu : 0.0;
@@ -570,9 +608,45 @@ int prim_field_set(sifv_handle* h, double** f, int* comp) {
     return n;
 }
 
+// FP32 stiffened-gas state (sifv_handle::eoff): total energies of the current
+// state into ePend (toStored false) or the stored energies from ePend.
+int energy_convert(sifv_handle* h, bool toStored) {
+    if (!h->ePend) {
+        if (h->noalloc) return fail(SIFV_ERR_INVALID, "energy staging buffer missing");
+        HIPCHK(hipMalloc(&h->ePend, sizeof(double) * h->total));
+    }
+    EnergyArgs A;
+    const Mix m = h->mix();
+    A.total = h->total;
+    A.E = h->consField<float>(h->cur, h->dim + 1);
+    A.nph = h->nph;
+    for (int p = 0; p < kMaxPhases; ++p) {
+        A.alpha[p] = p < h->nph ? h->consField<float>(h->cur, h->dim + 2 + h->nph + p) : nullptr;
+        A.eC[p] = m.eC[p];
+    }
+    A.c = h->eoffConst();
+    A.e64 = h->ePend;
+    A.toStored = toStored ? 1 : 0;
+    const int nb = (int)std::min<size_t>((h->total + 255) / 256, 8192);
+    hipLaunchKernelGGL(energy_offset_kernel, dim3(nb), dim3(256), 0, h->stream, A);
+    HIPCHK(hipGetLastError());
+    return SIFV_OK;
+}
+
+// Stores an uploaded total energy relative to the volume fractions now in the
+// state (called before anything reads the state).
+int settle_energy(sifv_handle* h) {
+    if (!h->ePending) return SIFV_OK;
+    int rc = energy_convert(h, true);
+    if (rc) return rc;
+    h->ePending = false;
+    return SIFV_OK;
+}
+
 int refresh_prim(sifv_handle* h) {
+    int rc = settle_energy(h);
+    if (rc) return rc;
     if (h->primFresh) return SIFV_OK;
-    int rc;
     if (h->cfg.semiImplicit) {
         // the semi-implicit step leaves the solved pressure and the corrected
         // velocity in the primitive arrays (no closing conversion,
@@ -633,6 +707,7 @@ int elementwise_blocks(const sifv_handle* h);
 template <typename T>
 int launch_visc(sifv_handle* h, int src, const int* active) {
     if (!h->visc) {
+        if (h->noalloc) return fail(SIFV_ERR_INVALID, "face-stress buffer missing");
         HIPCHK(hipMalloc(&h->visc, sizeof(double) * (h->dim + 1) * h->total));
         HIPCHK(hipMemset(h->visc, 0, sizeof(double) * (h->dim + 1) * h->total));
     }
@@ -1471,7 +1546,7 @@ void drop_step_graph(sifv_handle* h) {
 // events are recorded, and IGR (whose warm start reads the step counter and
 // whose sweeps carry epochs) is off.
 bool step_graph_usable(const sifv_handle* h) {
-    return h->useGraphs && !h->timing.on && h->cfg.RKOrder >= 2 && !h->cfg.useIGR &&
+    return h->useGraphs && !h->noalloc && !h->timing.on && h->cfg.RKOrder >= 2 && !h->cfg.useIGR &&
            (h->cfg.explicitParams.constDt > 0.0 || h->dtNextValid);
 }
 
@@ -1545,8 +1620,6 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
         if (mesh->bc[f] < 0 || mesh->bc[f] > 4) return fail(SIFV_ERR_INVALID, "unknown boundary condition");
     if (cfg->multiPhaseParams.nPhases != 0 && cfg->multiPhaseParams.nPhases != 2)
         return fail(SIFV_ERR_UNSUPPORTED, "the GPU path carries single-phase or two-phase states");
-    if (precision == SIFV_MIXED && !cfg->semiImplicit && cfg->multiPhaseParams.nPhases > 0)
-        return fail(SIFV_ERR_UNSUPPORTED, "the FP32-bulk explicit form is available for single-phase states only");
 
     auto* h = new sifv_handle();
     h->dim = mesh->dim;
@@ -1569,6 +1642,7 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
     h->precision = h->jacMixed ? SIFV_FP64 : precision;
     h->nph = cfg->multiPhaseParams.nPhases;
     h->nc = h->dim + 2 + 2 * h->nph;
+    h->eoff = h->precision == SIFV_MIXED && (h->nph > 0 || eos->kind == SIFV_EOS_STIFFENED);
     const int nn[3] = {h->nx, h->ny, h->nz};
     for (int d = 0; d < 3; ++d) h->periods[d] = d < h->dim && mesh->periods[d] != 0;
     for (int f = 0; f < 6; ++f) {
@@ -1655,6 +1729,13 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
         h->nfc = h->dim + 2 + 2 * h->nph + 2;
         CK(hipMalloc(&h->part, h->tsize() * h->total * h->nfc));
     }
+    // buffers the time loop would otherwise allocate on first use
+    if (has_face_stress(*cfg)) {
+        CK(hipMalloc(&h->visc, sizeof(double) * (h->dim + 1) * h->total));
+        CK(hipMemset(h->visc, 0, sizeof(double) * (h->dim + 1) * h->total));
+    }
+    if (h->eoff) CK(hipMalloc(&h->ePend, sizeof(double) * h->total));
+    CK(hipGetDevice(&h->device));
 #undef CK
     if (cfg->useIGR) {
         rc = setup_igr(h);
@@ -1682,6 +1763,7 @@ int sifv_destroy(sifv_handle* h) {
     hipFree(h->d_dz);
     for (int b = 0; b < 3; ++b) hipFree(h->cons[b]);
     hipFree(h->rhoP);
+    hipFree(h->ePend);
     for (int d = 0; d < 3; ++d) hipFree(h->vel[d]);
     hipFree(h->pres);
     hipFree(h->sigma);
@@ -1759,6 +1841,21 @@ int sifv_upload(sifv_handle* h, int field, const double* host, size_t n) {
     if (n != h->total) return fail(SIFV_ERR_INVALID, "array length != totalCells");
     HIPCHK(hipStreamSynchronize(h->stream));
     int q = cons_slot(h, field);
+    if (h->eoff && field == SIFV_F_RHOE) {
+        // total energy: kept in FP64 until stored relative to the volume fractions
+        if (!h->ePend) HIPCHK(hipMalloc(&h->ePend, sizeof(double) * h->total));
+        HIPCHK(hipMemcpy(h->ePend, host, n * sizeof(double), hipMemcpyHostToDevice));
+        h->ePending = true;
+        h->dtNextValid = false;
+        return SIFV_OK;
+    }
+    if (h->eoff && h->nph > 0 && !h->ePending && field >= SIFV_F_ALPHA && field < SIFV_F_ALPHA + SIFV_MAX_PHASES) {
+        // new volume fractions keep the total energy: restate it first
+        int rc = energy_convert(h, false);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(h->stream));
+        h->ePending = true;
+    }
     if (q >= 0) {
         if (h->precision == SIFV_MIXED) {
             std::vector<float> tmp(host, host + n);
@@ -1785,6 +1882,12 @@ int sifv_download(sifv_handle* h, int field, double* host, size_t n) {
     int q = cons_slot(h, field);
     if (field == SIFV_F_RHO) {
         HIPCHK(hipMemcpy(host, h->rhoP, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else if (h->eoff && field == SIFV_F_RHOE) {
+        // stored energy + stiffened-gas offset, in FP64
+        rc = energy_convert(h, false);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemcpy(host, h->ePend, n * sizeof(double), hipMemcpyDeviceToHost));
     } else if (q >= 0) {
         if (h->precision == SIFV_MIXED) {
             std::vector<float> tmp(n);
@@ -1805,7 +1908,9 @@ int sifv_explicit_step(sifv_handle* h, const sifv_config* cfg, double targetDt, 
     if (!h) return fail(SIFV_ERR_INVALID, "null handle");
     if (h->cfg.semiImplicit) return fail(SIFV_ERR_INVALID, "handle was created for the semi-implicit solver");
     check_step_config(h, cfg);
-    int rc = enqueue_explicit_step(h, targetDt, false, 0.0);
+    int rc = settle_energy(h);
+    if (rc) return rc;
+    rc = enqueue_explicit_step(h, targetDt, false, 0.0);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(h->hostPinned, h->scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -1818,12 +1923,13 @@ int sifv_explicit_step(sifv_handle* h, const sifv_config* cfg, double targetDt, 
     return SIFV_OK;
 }
 
-int sifv_explicit_advance(sifv_handle* h, const sifv_config* cfg, int nsteps, double endTime,
-                          double* time_inout, int* steps_done) {
-    if (!h || !time_inout) return fail(SIFV_ERR_INVALID, "null argument");
-    if (h->cfg.semiImplicit) return fail(SIFV_ERR_INVALID, "handle was created for the semi-implicit solver");
-    if (nsteps < 0) return fail(SIFV_ERR_INVALID, "nsteps < 0");
+namespace {
+// Body of sifv_explicit_advance (shared with the allocation-free form).
+int advance_impl(sifv_handle* h, const sifv_config* cfg, int nsteps, double endTime, double* time_inout,
+                 int* steps_done) {
     check_step_config(h, cfg);
+    int src = settle_energy(h);
+    if (src) return src;
     hipLaunchKernelGGL(loop_reset_kernel, dim3(1), dim3(1), 0, h->stream, h->scal, h->flags, *time_inout);
     HIPCHK(hipGetLastError());
     // runTimeLoop advances config.step once per step (src/RKTimeStepping.cpp:266);
@@ -1832,7 +1938,10 @@ int sifv_explicit_advance(sifv_handle* h, const sifv_config* cfg, int nsteps, do
     for (int s = 0; s < nsteps; ++s) {
         h->cfg.step = step0 + s;
         int rc = step_graph_usable(h) ? launch_step_graph(h, endTime) : enqueue_explicit_step(h, -1.0, true, endTime);
-        if (rc) return rc;
+        if (rc) {
+            h->cfg.step = step0;
+            return rc;
+        }
     }
     h->cfg.step = step0;
     HIPCHK(hipMemcpyAsync(h->hostPinned, h->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1847,6 +1956,16 @@ int sifv_explicit_advance(sifv_handle* h, const sifv_config* cfg, int nsteps, do
     if (steps_done) *steps_done = hflags[1];
     if (!std::isfinite(h->hostPinned[1])) return fail(SIFV_ERR_NONFINITE, "non-finite time");
     return SIFV_OK;
+}
+
+}  // namespace
+
+int sifv_explicit_advance(sifv_handle* h, const sifv_config* cfg, int nsteps, double endTime,
+                          double* time_inout, int* steps_done) {
+    if (!h || !time_inout) return fail(SIFV_ERR_INVALID, "null argument");
+    if (h->cfg.semiImplicit) return fail(SIFV_ERR_INVALID, "handle was created for the semi-implicit solver");
+    if (nsteps < 0) return fail(SIFV_ERR_INVALID, "nsteps < 0");
+    return advance_impl(h, cfg, nsteps, endTime, time_inout, steps_done);
 }
 
 int sifv_fill_ghosts(sifv_handle* h, int varset) {

@@ -45,6 +45,9 @@ struct DtArgs {
     const double* pres;
     const double* alpha[kMaxPhases];
     const double* alphaRho[kMaxPhases];
+    const float* alpha32[kMaxPhases];     // FP32 state: volume fractions and partial
+    const float* alphaRho32[kMaxPhases];  // densities are read from these instead
+    int f32 = 0;
     int advective;
     double viscMu = 0.0;    // > 0: include the viscous limit
     double capSigma = 0.0;  // > 0: include the capillary limit
@@ -69,8 +72,8 @@ __global__ void __launch_bounds__(kRedThreads) dt_cells_kernel(DtArgs A) {
             if (A.mix.nph > 0) {
                 double a[kMaxPhases], ar[kMaxPhases];
                 for (int ph = 0; ph < A.mix.nph; ++ph) {
-                    a[ph] = A.alpha[ph][id];
-                    ar[ph] = A.alphaRho[ph][id];
+                    a[ph] = A.f32 ? (double)A.alpha32[ph][id] : A.alpha[ph][id];
+                    ar[ph] = A.f32 ? (double)A.alphaRho32[ph][id] : A.alphaRho[ph][id];
                 }
                 c = mix_sound(A.mix, A.rho[id], A.pres[id], a, ar, A.mix.nph);
             } else {
@@ -229,7 +232,8 @@ __global__ void __launch_bounds__(256) cons_to_prim_kernel(C2PArgs<T> A) {
                 sInv += a / gm1;
                 sPi += a * A.mix.gamma[ph] * A.mix.pInf[ph] / gm1;
             }
-            A.pres[id] = (((double)A.cons[dim + 1][id] - ke) - sPi) / sInv;
+            A.pres[id] = A.mix.eoff ? ((double)A.cons[dim + 1][id] - ke) / sInv
+                                    : (((double)A.cons[dim + 1][id] - ke) - sPi) / sInv;
         } else {
             double mom[3], u[3], p;
             for (int d = 0; d < dim; ++d) mom[d] = (double)A.cons[1 + d][id];
@@ -241,6 +245,31 @@ __global__ void __launch_bounds__(256) cons_to_prim_kernel(C2PArgs<T> A) {
             for (int d = 0; d < dim; ++d) A.vel[d][id] = u[d];
             A.pres[id] = p;
         }
+    }
+}
+
+// Stored <-> total energy of an FP32 state whose energy field excludes the
+// stiffened-gas constant (Eos::eoff / Mix::eoff), over every padded cell:
+// toStored: E (FP64, e64) -> E - offset (FP32, into the state);
+// otherwise: E_stored + offset -> e64.  The offset uses the volume fractions
+// stored in the same state.
+struct EnergyArgs {
+    size_t total;
+    float* E;                          // stored energy field
+    const float* alpha[kMaxPhases];    // stored volume fractions (multiphase)
+    int nph;
+    double eC[kMaxPhases];             // multiphase: gamma_k pInf_k / (gamma_k - 1)
+    double c;                          // single phase: gamma pInf / (gamma - 1)
+    double* e64;
+    int toStored;
+};
+
+__global__ void __launch_bounds__(256) energy_offset_kernel(EnergyArgs A) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < A.total; t += (size_t)gridDim.x * blockDim.x) {
+        double o = A.c;
+        for (int ph = 0; ph < A.nph; ++ph) o += (double)A.alpha[ph][t] * A.eC[ph];
+        if (A.toStored) A.E[t] = (float)(A.e64[t] - o);
+        else A.e64[t] = (double)A.E[t] + o;
     }
 }
 

@@ -94,17 +94,91 @@ __device__ __forceinline__ int map_axis_scalar(int c, int n, int lo, int hi) {
     return c;
 }
 
+// SIFV_FAST_DIV = 1: reciprocals, divisions and square roots whose operands
+// are clamped into the normal range (densities and pressures floored at
+// 1e-14, sound speeds > 0) use the hardware approximations refined by Newton
+// steps instead of the IEEE sequences with range scaling and special-case
+// fix-ups (within one ulp of the IEEE result), and wave-speed bounds use the
+// hardware min/max (equal to the reference's std::min/std::max for every
+// non-NaN operand).
+#ifndef SIFV_FAST_DIV
+#define SIFV_FAST_DIV 1
+#endif
+
 __device__ __forceinline__ double dmax_ref(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double dmin_ref(double a, double b) { return (b < a) ? b : a; }
 template <typename T>
 __device__ __forceinline__ T tmax(T a, T b) { return (a < b) ? b : a; }
 template <typename T>
 __device__ __forceinline__ T tmin(T a, T b) { return (b < a) ? b : a; }
+// min/max of operands that are never NaN in a finite state (wave speeds,
+// clamps of densities and pressures)
+template <typename T>
+__device__ __forceinline__ T hmax(T a, T b) {
+    if constexpr (!SIFV_FAST_DIV) return tmax(a, b);
+    else if constexpr (sizeof(T) == 8) return __builtin_fmax(a, b);
+    else return __builtin_fmaxf(a, b);
+}
+template <typename T>
+__device__ __forceinline__ T hmin(T a, T b) {
+    if constexpr (!SIFV_FAST_DIV) return tmin(a, b);
+    else if constexpr (sizeof(T) == 8) return __builtin_fmin(a, b);
+    else return __builtin_fminf(a, b);
+}
 
-// Equation of state constants of the single-phase path.
+// 1/x for a normal x in [1e-290, 1e290]: the hardware reciprocal refined by two
+// Newton-Raphson steps r <- r + r (1 - x r) (error well below one ulp of r).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// a / b and 1 / b, sqrt(x) for normal operands (see SIFV_FAST_DIV); the
+// quotient gets one residual correction q + r (a - b q).
+__device__ __forceinline__ double div_fast(double a, double b) {
+    if constexpr (SIFV_FAST_DIV) {
+        const double r = rcp_nr(b);
+        const double q = a * r;
+        return fma(fma(-b, q, a), r, q);
+    } else {
+        return a / b;
+    }
+}
+__device__ __forceinline__ double rcp_fast(double b) {
+    if constexpr (SIFV_FAST_DIV) return rcp_nr(b);
+    else return 1.0 / b;
+}
+// sqrt of a positive normal x: hardware reciprocal square root refined by the
+// coupled Newton iteration for g = sqrt(x), h = 1 / (2 sqrt(x)), then two
+// residual corrections of g.
+__device__ __forceinline__ double sqrt_fast(double x) {
+    if constexpr (SIFV_FAST_DIV) {
+        const double y = __builtin_amdgcn_rsq(x);
+        double g = x * y, h = 0.5 * y;
+        const double r = fma(-g, h, 0.5);
+        g = fma(g, r, g);
+        h = fma(h, r, h);
+        double d = fma(-g, g, x);
+        g = fma(d, h, g);
+        d = fma(-g, g, x);
+        return fma(d, h, g);
+    } else {
+        return sqrt(x);
+    }
+}
+
+// Equation of state constants of the single-phase path.  eoff: the stored
+// total energy excludes the constant gamma*pInf/(gamma-1) (FP32 storage of a
+// stiffened gas, where the stored value would otherwise be dominated by that
+// constant and lose the pressure to rounding); the conversion then reads
+// p = (gamma-1)(E_stored - ke), which is p = (gamma-1)(E - ke) - gamma*pInf.
 struct Eos {
     double gamma, pInf;
     int stiffened;
+    int eoff = 0;
 };
 
 // cons -> prim of one cell in FP64: IdealGasEOS::toPrimitive
@@ -116,7 +190,7 @@ template <int DIM>
 __device__ __forceinline__ void cons_to_prim(const Eos& e, double rho, const double* mom, double E,
                                              double* u, double& p) {
     double rs = dmax_ref(rho, 1e-14);
-    const double ri = 1.0 / rs;
+    const double ri = rcp_fast(rs);
     double ke = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -126,7 +200,7 @@ __device__ __forceinline__ void cons_to_prim(const Eos& e, double rho, const dou
     ke *= 0.5 * rs;
     double ein = E - ke;
     if (e.stiffened)
-        p = (e.gamma - 1.0) * ein - e.gamma * e.pInf;
+        p = e.eoff ? (e.gamma - 1.0) * ein : (e.gamma - 1.0) * ein - e.gamma * e.pInf;
     else
         p = dmax_ref((e.gamma - 1.0) * ein, 1e-14);
 }
@@ -144,7 +218,7 @@ struct Recon;
 
 template <typename T>
 struct Recon<1, T> {
-    __device__ __forceinline__ static void run(const T* v, T, bool, T& minus, T& plus) {
+    __device__ __forceinline__ static void run(const T* v, T, int, T& minus, T& plus) {
         minus = v[0];
         plus = v[0];
     }
@@ -172,25 +246,37 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 
 // Division / reciprocal / square root in the working precision: IEEE FP64,
 // hardware (1-ulp) FP32 instructions.
-template <typename T>
+// FAST = false keeps the IEEE FP64 operations (two-phase states, whose
+// water/air rounding floor is set by the pInf cancellation).
+template <typename T, bool FAST = true>
 __device__ __forceinline__ T t_rcp(T x) {
-    if constexpr (sizeof(T) == 8) return T(1.0) / x;
-    else return frcp(x);
+    if constexpr (sizeof(T) == 4) return frcp(x);
+    else if constexpr (FAST) return rcp_fast(x);
+    else return T(1.0) / x;
 }
-template <typename T>
+template <typename T, bool FAST = true>
 __device__ __forceinline__ T t_div(T a, T b) {
-    if constexpr (sizeof(T) == 8) return a / b;
-    else return a * frcp(b);
+    if constexpr (sizeof(T) == 4) return a * frcp(b);
+    else if constexpr (FAST) return div_fast(a, b);
+    else return a / b;
 }
-template <typename T>
+// square root of a positive normal operand (sound speeds)
+template <typename T, bool FAST = true>
 __device__ __forceinline__ T t_sqrt(T x) {
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_sqrtf(x);
+    else if constexpr (FAST) return sqrt_fast(x);
+    else return sqrt(x);
+}
+// square root that may see 0 (velocity magnitudes)
+template <typename T>
+__device__ __forceinline__ T t_sqrt0(T x) {
     if constexpr (sizeof(T) == 8) return sqrt(x);
     else return __builtin_amdgcn_sqrtf(x);
 }
 
 template <typename T>
 struct Recon<2, T> {
-    __device__ __forceinline__ static void run(const T* v, T eps, bool weno, T& minus, T& plus) {
+    __device__ __forceinline__ static void run(const T* v, T eps, int weno, T& minus, T& plus) {
         const T h = T(0.5), th = T(1.5);
         T pl0 = -h * v[0] + th * v[1];
         T pl1 = h * v[1] + h * v[2];
@@ -221,16 +307,6 @@ struct Recon<2, T> {
     }
 };
 
-// 1/x for a normal x in [1e-290, 1e290]: the hardware reciprocal refined by two
-// Newton-Raphson steps r <- r + r (1 - x r) (error well below one ulp of r).
-__device__ __forceinline__ double rcp_nr(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
-
 // WENO5 of cell c = v[2].  With the first differences a..d of the stencil, the
 // quadratic of sub-stencil k (cells k..k+2) has second difference t_k and slope
 // term S_k (S_0 = 3b - a, S_1 = b + c, S_2 = 3c - d); its values at the cell's
@@ -243,7 +319,7 @@ __device__ __forceinline__ double rcp_nr(double x) {
 // image, with 3/10, 6/10, 1/10.
 template <typename T>
 struct Recon<3, T> {
-    __device__ __forceinline__ static void run(const T* v, T eps, bool weno, T& minus, T& plus) {
+    __device__ __forceinline__ static void run(const T* v, T eps, int weno, T& minus, T& plus) {
         const T a = v[1] - v[0], b = v[2] - v[1], c = v[3] - v[2], d = v[4] - v[3];
         const T t0 = b - a, t1 = c - b, t2 = d - c;
         const T S0 = T(3.0) * b - a, S1 = b + c, S2 = T(3.0) * c - d;
@@ -276,11 +352,35 @@ struct Recon<3, T> {
                 plus = v[2] + (nP * dM) * inv;
                 minus = v[2] + (nM * dP) * inv;
             } else {
-                plus = v[2] + nP / (dP * T(12.0));
-                minus = v[2] + nM / (dM * T(12.0));
+                // products out of range (eps + beta below ~1e-36 or above
+                // ~1e36): the weights d_k / Q_k of the reference, computed
+                // from the ratios e_min / e_k in (0, 1] so that neither the
+                // squares nor their reciprocals leave the FP64 range
+                const T em = tmin(tmin(e0, e1), e2);
+                const T r0 = em / e0, r1 = em / e1, r2 = em / e2;
+                const T q0 = r0 * r0, q1 = r1 * r1, q2 = r2 * r2;
+                const T aP0 = d0 * q0, a1 = d1 * q1, aP2 = d2 * q2;
+                const T aM0 = d2 * q0, aM2 = d0 * q2;
+                plus = v[2] + (aP0 * Gp0 + a1 * Gp1 + aP2 * Gp2) / ((aP0 + a1 + aP2) * T(12.0));
+                minus = v[2] + (aM0 * Gm0 + a1 * Gm1 + aM2 * Gm2) / ((aM0 + a1 + aM2) * T(12.0));
             }
         } else {
-            const T q0 = frcp(Q0), q1 = frcp(Q1), q2 = frcp(Q2);
+            // weno == 2: the indicators are scaled by their minimum first
+            // (weights depend only on the ratios of eps + beta_k), so that a
+            // pressure jump of 1e9..1e19 (stiffened gas) cannot overflow the
+            // FP32 square; the smallest indicator keeps the weight d_k
+            T q0, q1, q2;
+            if (weno == 2) {
+                const T em = tmin(tmin(e0, e1), e2);
+                const T t0 = frcp(e0) * em, t1 = frcp(e1) * em, t2 = frcp(e2) * em;
+                q0 = t0 * t0;
+                q1 = t1 * t1;
+                q2 = t2 * t2;
+            } else {
+                q0 = frcp(Q0);
+                q1 = frcp(Q1);
+                q2 = frcp(Q2);
+            }
             const T aP0 = d0 * q0, a1 = d1 * q1, aP2 = d2 * q2;
             const T aM0 = d2 * q0, aM2 = d0 * q2;
             plus = v[2] + (aP0 * Gp0 + a1 * Gp1 + aP2 * Gp2) * (frcp(aP0 + a1 + aP2) * c112);
@@ -312,22 +412,24 @@ constexpr int flux_len(int dim) { return dim + 4 + NPH; }
 
 template <int DIM, int NPH, typename T>
 __device__ __forceinline__ T sound_direct(const FaceState<DIM, NPH, T>& w) {
-    return t_sqrt(t_div(w.g * tmax(w.p + w.pi, T(1e-14)), tmax(w.rho, T(1e-14))));
+    constexpr bool F = NPH == 0;
+    return t_sqrt<T, F>(t_div<T, F>(w.g * hmax(w.p + w.pi, T(1e-14)), hmax(w.rho, T(1e-14))));
 }
 
 // Both sound speeds of a face with one division.
 template <int DIM, int NPH, typename T>
 __device__ __forceinline__ void sound_pair(const FaceState<DIM, NPH, T>& L, const FaceState<DIM, NPH, T>& R,
                                            T& cL, T& cR) {
-    const T rL = tmax(L.rho, T(1e-14)), rR = tmax(R.rho, T(1e-14));
+    constexpr bool F = NPH == 0;
+    const T rL = hmax(L.rho, T(1e-14)), rR = hmax(R.rho, T(1e-14));
     if constexpr (sizeof(T) == 8) {
-        const T inv = t_rcp(rL * rR);
-        cL = t_sqrt(L.g * tmax(L.p + L.pi, T(1e-14)) * (rR * inv));
-        cR = t_sqrt(R.g * tmax(R.p + R.pi, T(1e-14)) * (rL * inv));
+        const T inv = t_rcp<T, F>(rL * rR);
+        cL = t_sqrt<T, F>(L.g * hmax(L.p + L.pi, T(1e-14)) * (rR * inv));
+        cR = t_sqrt<T, F>(R.g * hmax(R.p + R.pi, T(1e-14)) * (rL * inv));
     } else {
         // the product of two FP32 densities can leave the normal range
-        cL = t_sqrt(t_div(L.g * tmax(L.p + L.pi, T(1e-14)), rL));
-        cR = t_sqrt(t_div(R.g * tmax(R.p + R.pi, T(1e-14)), rR));
+        cL = t_sqrt(t_div(L.g * hmax(L.p + L.pi, T(1e-14)), rL));
+        cR = t_sqrt(t_div(R.g * hmax(R.p + R.pi, T(1e-14)), rR));
     }
 }
 
@@ -371,13 +473,13 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
         if (incP) {
             T cL, cR;
             sound_pair(L, R, cL, cR);
-            sL = tmin(uL - cL, uR - cR);
-            sR = tmax(uL + cL, uR + cR);
-            sS = t_div(R.p - L.p + L.rho * uL * (sL - uL) - R.rho * uR * (sR - uR),
-                       L.rho * (sL - uL) - R.rho * (sR - uR));
+            sL = hmin(uL - cL, uR - cR);
+            sR = hmax(uL + cL, uR + cR);
+            sS = t_div<T, NPH == 0>(R.p - L.p + L.rho * uL * (sL - uL) - R.rho * uR * (sR - uR),
+                                    L.rho * (sL - uL) - R.rho * (sR - uR));
         } else {
-            sL = tmin(uL, uR);
-            sR = tmax(uL, uR);
+            sL = hmin(uL, uR);
+            sR = hmax(uL, uR);
             sS = T(0.5) * (uL + uR);
         }
         if (sL >= T(0) || sR <= T(0)) {
@@ -409,14 +511,14 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
                 // s - un and s - sS are nonzero here (sound speed > 0, sS strictly
                 // between the outer waves): one division for all three quotients
                 const T rA = W.rho * A;
-                const T inv = t_rcp(rA * B);
+                const T inv = t_rcp<T, NPH == 0>(rA * B);
                 rS = rA * (rA * inv);
                 e = E * (A * B * inv);
                 pOverRhoA = W.p * (B * inv);
             } else {
-                rS = t_div(W.rho * A, B);
-                e = t_div(E, W.rho);
-                if (incP) pOverRhoA = t_div(W.p, W.rho * A);
+                rS = t_div<T, NPH == 0>(W.rho * A, B);
+                e = t_div<T, NPH == 0>(E, W.rho);
+                if (incP) pOverRhoA = t_div<T, NPH == 0>(W.p, W.rho * A);
             }
             F[0] = W.rho * un + s * (rS - W.rho);
 #pragma unroll
@@ -448,9 +550,9 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
             if (incP) {
                 T cL, cR;
                 sound_pair(L, R, cL, cR);
-                S = tmax(aL + cL, aR + cR);
+                S = hmax(aL + cL, aR + cR);
             } else {
-                S = tmax(aL, aR);
+                S = hmax(aL, aR);
             }
             T mL = L.rho * uL, mR = R.rho * uR;
             F[0] = T(0.5) * (mL + mR) - T(0.5) * S * (R.rho - L.rho);
@@ -476,14 +578,14 @@ __device__ __forceinline__ void riemann(const FaceState<DIM, NPH, T>& L, const F
                 qL += L.u[d] * L.u[d];
                 qR += R.u[d] * R.u[d];
             }
-            qL = t_sqrt(qL);
-            qR = t_sqrt(qR);
+            qL = t_sqrt0(qL);
+            qR = t_sqrt0(qR);
             if (incP) {
                 T cL, cR;
                 sound_pair(L, R, cL, cR);
-                S = tmax(qL, qR) + tmax(cL, cR);
+                S = hmax(qL, qR) + hmax(cL, cR);
             } else {
-                S = tmax(qL, qR);
+                S = hmax(qL, qR);
             }
             F[0] = T(0.5) * (L.rho * uL + R.rho * uR) - T(0.5) * S * (R.rho - L.rho);
 #pragma unroll

@@ -221,6 +221,10 @@ int sifv_ensemble_create(int ntubes, const int* ncells, const double* nodes, con
         return set_error(SIFV_ERR_INVALID, "explicitParams.cfl and maxDt must be > 0");
     if (riemann < SIFV_LF || riemann > SIFV_HLLC) return set_error(SIFV_ERR_INVALID, "unknown Riemann solver");
     if (precision != SIFV_FP64 && precision != SIFV_MIXED) return set_error(SIFV_ERR_INVALID, "unknown precision");
+    if (precision == SIFV_MIXED && eos->kind == SIFV_EOS_STIFFENED)
+        return set_error(SIFV_ERR_UNSUPPORTED,
+                         "ensemble tubes store their FP32 state without the stiffened-gas energy offset: "
+                         "stiffened gases run in FP64");
     for (int d = 0; d < 3; ++d)
         if (cfg->bodyForceParams.a[d] != 0.0 || cfg->bodyForceParams.b[d] != 0.0 || cfg->viscousMu > 0.0 ||
             cfg->viscousNPhaseMu > 0 || cfg->surfaceTensionSigma > 0.0)

@@ -26,6 +26,11 @@ IgrPreFn<double> igr_pre_kernel_for<double>(int dim, int nph) {
 
 template <>
 IgrPreFn<float> igr_pre_kernel_for<float>(int dim, int nph) {
+    if (nph == 2) {
+        if (dim == 1) return igr_pre_kernel<1, 2, float>;
+        if (dim == 2) return igr_pre_kernel<2, 2, float>;
+        return igr_pre_kernel<3, 2, float>;
+    }
     if (nph != 0) return nullptr;
     if (dim == 1) return igr_pre_kernel<1, 0, float>;
     if (dim == 2) return igr_pre_kernel<2, 0, float>;

@@ -28,6 +28,7 @@ StageKernels<double> stage_kernels_2d(int R, int rs, int nph, bool semi, bool le
 StageKernels<double> stage_kernels_3d(int R, int rs, int nph);
 StageKernels<double> stage_kernels_3d_semi(int R, int rs, int nph);
 StageKernels<float> stage_kernels_fp32(int dim, int R, int rs, bool lean);
+StageKernels<float> stage_kernels_fp32_mp(int dim, int R, int rs);
 
 // IGR: pre-pass (rhs, diagonal, rho) per dimension / phase count, and the
 // wavefront Gauss-Seidel sweep per dimension.

@@ -27,6 +27,8 @@
 // flips applied.
 #pragma once
 
+#include <type_traits>
+
 #include "sifv_device.hpp"
 
 #ifndef SIFV_PREFETCH_ROW_D
@@ -55,11 +57,30 @@ constexpr int kOwn = 62;  // owned columns per wave in the x-marching kernels
 enum { MODE_FULL = 0, MODE_XY = 1 };
 
 // Mixture parameters (MultiPhaseParams, include/SimulationConfig.hpp:43-52).
+// eoff: the stored total energy excludes sum_k alpha_k eC_k with
+// eC_k = gamma_k pInf_k / (gamma_k - 1), the stiffened-gas part of the mixture
+// energy (FP32 storage; the conversion then reads p = (E_stored - ke) / sInv).
 struct Mix {
     int nph;
     double gamma[kMaxPhases], pInf[kMaxPhases];
     double alphaMin;
+    int eoff = 0;
+    double eC[kMaxPhases] = {};   // gamma_k pInf_k / (gamma_k - 1)
+    double gi[kMaxPhases] = {};   // 1 / (gamma_k - 1)
 };
+
+// Per-phase terms alpha/(gamma-1) and alpha gamma pInf/(gamma-1) of the mixture
+// relations, as the reference's quotients.  Two-phase water/air states sit at
+// a rounding floor set by the pInf cancellation, so their relations keep the
+// reference's operation forms (no reciprocal products or Newton quotients).
+template <typename T>
+__device__ __forceinline__ T mix_inv(const Mix& m, int ph, T a) {
+    return a / ((T)m.gamma[ph] - T(1.0));
+}
+template <typename T>
+__device__ __forceinline__ T mix_pi(const Mix& m, int ph, T a) {
+    return a * (T)m.gamma[ph] * (T)m.pInf[ph] / ((T)m.gamma[ph] - T(1.0));
+}
 
 // Wood sound speed, MixtureEOS::mixtureSoundSpeed (src/MixtureEOS.cpp:48-63).
 __device__ __forceinline__ double mix_sound(const Mix& m, double rho, double p, const double* a,
@@ -93,7 +114,8 @@ struct StageParams {
     const int* active;    // device flag; 0 skips the stage (time loop finished)
     double* dtPartial;    // per-block min of the new state's acoustic cell dt, or null
     double eps;
-    int igr, weno;
+    int igr;
+    int weno;             // 0 linear weights, 1 WENO, 2 WENO with scaled indicators (FP32, stiffened gas)
     int sigPrim;          // sigma ghosts follow the PRIM rule (no IGR sweep ran this stage)
     int unifY;            // every interior row (and plane) width is the same double
     int rows;             // rows (y) or planes (z) marched per wave
@@ -137,12 +159,21 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }
 
 // Primitive state of one cell (FP64) plus its partial densities.
-template <int DIM, int NPH>
+// C: storage type of the primitive values (double, or float in the march
+// windows of single-phase FP32 states).
+template <int DIM, int NPH, typename C = double>
 struct Cell {
-    double rho, u[DIM], p;
-    double a[NPH > 0 ? NPH : 1];
-    double ar[NPH > 0 ? NPH : 1];
+    C rho, u[DIM], p;
+    C a[NPH > 0 ? NPH : 1];
+    C ar[NPH > 0 ? NPH : 1];
 };
+
+// Storage of the march windows: FP32 for single-phase explicit states held
+// in FP32 (their reconstruction reads FP32 values anyway), FP64 otherwise
+// (two-phase states keep FP64 primitives for the alpha-rho upwinding and the
+// semi-implicit update).
+template <typename T, int NPH, bool SEMI>
+using CellT = typename std::conditional<sizeof(T) == 4 && NPH == 0 && !SEMI, float, double>::type;
 
 __device__ __forceinline__ int clampi(int c, int lo, int hi) { return c < lo ? lo : (c > hi ? hi : c); }
 
@@ -176,14 +207,69 @@ __device__ __forceinline__ void prim_of(const Eos& e, const Mix& m, const double
 #pragma unroll
         for (int ph = 0; ph < NPH; ++ph) {
             double a = U[DIM + 2 + NPH + ph];
-            double gm1 = m.gamma[ph] - 1.0;
-            sInv += a / gm1;
-            sPi += a * m.gamma[ph] * m.pInf[ph] / gm1;
+            sInv += mix_inv<double>(m, ph, a);
+            sPi += mix_pi<double>(m, ph, a);
             c.a[ph] = a;
             c.ar[ph] = U[DIM + 2 + ph];
         }
-        c.p = ((U[DIM + 1] - ke) - sPi) / sInv;
+        c.p = m.eoff ? (U[DIM + 1] - ke) / sInv : ((U[DIM + 1] - ke) - sPi) / sInv;
     }
+}
+
+// cons -> prim of one single-phase ideal-gas cell in FP32 (IdealGasEOS::toPrimitive,
+// src/IdealGasEOS.cpp:46-64, in single precision: no pInf cancellation).
+template <int DIM>
+__device__ __forceinline__ void prim_of_f32(const Eos& e, const float* U, Cell<DIM, 0, float>& c) {
+    const float rs = __builtin_fmaxf(U[0], 1e-14f);
+    const float ri = frcp(rs);
+    float ke = 0.0f;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        c.u[d] = U[1 + d] * ri;
+        ke += c.u[d] * c.u[d];
+    }
+    ke *= 0.5f * rs;
+    c.rho = U[0];
+    c.p = __builtin_fmaxf((float)(e.gamma - 1.0) * (U[DIM + 1] - ke), 1e-14f);
+}
+
+// cons -> prim of raw values of type T into a cell of storage C.  FP32 cells of
+// an ideal gas convert in FP32; a stiffened gas (pInf cancellation) and FP64
+// cells convert in FP64.
+template <int DIM, int NPH, typename T, typename C>
+__device__ __forceinline__ void convert_cell(const Eos& e, const Mix& m, const T* raw, Cell<DIM, NPH, C>& c) {
+    constexpr int NCV = DIM + 2 + 2 * NPH;
+    if constexpr (sizeof(C) == 4 && NPH == 0) {
+        if (!e.stiffened) {
+            float U[NCV];
+#pragma unroll
+            for (int q = 0; q < NCV; ++q) U[q] = (float)raw[q];
+            prim_of_f32<DIM>(e, U, c);
+            return;
+        }
+    }
+    double U[NCV];
+#pragma unroll
+    for (int q = 0; q < NCV; ++q) U[q] = (double)raw[q];
+    if constexpr (sizeof(C) == 8) {
+        prim_of<DIM, NPH>(e, m, U, c);
+    } else {
+        Cell<DIM, NPH, double> d;
+        prim_of<DIM, NPH>(e, m, U, d);
+        c.rho = (C)d.rho;
+#pragma unroll
+        for (int q = 0; q < DIM; ++q) c.u[q] = (C)d.u[q];
+        c.p = (C)d.p;
+    }
+}
+
+// Stiffened-gas part of the stored energy of one cell (Mix::eoff form).
+template <int DIM, int NPH>
+__device__ __forceinline__ double energy_offset(const Mix& m, const double* U) {
+    double o = 0.0;
+#pragma unroll
+    for (int ph = 0; ph < NPH; ++ph) o += U[DIM + 2 + NPH + ph] * m.eC[ph];
+    return o;
 }
 
 // Raw conservative values of one mapped cell plus its velocity sign flips
@@ -220,23 +306,19 @@ __device__ __forceinline__ void load_raw(const StageParams<T>& P, const T* buf, 
     rc.flip = fl;
 }
 
-template <int DIM, int NPH, typename T>
+template <int DIM, int NPH, typename T, typename C>
 __device__ __forceinline__ void raw_to_cell(const StageParams<T>& P, const RawCell<DIM, NPH, T>& rc,
-                                            Cell<DIM, NPH>& c) {
-    constexpr int NCV = DIM + 2 + 2 * NPH;
-    double U[NCV];
-#pragma unroll
-    for (int q = 0; q < NCV; ++q) U[q] = (double)rc.U[q];
-    prim_of<DIM, NPH>(P.eos, P.mix, U, c);
+                                            Cell<DIM, NPH, C>& c) {
+    convert_cell<DIM, NPH, T, C>(P.eos, P.mix, rc.U, c);
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
         if (rc.flip & (1u << d)) c.u[d] = -c.u[d];
 }
 
 // Load one cell (i, j, k) through the ghost mapping (with velocity flips).
-template <int DIM, int NPH, typename T>
+template <int DIM, int NPH, typename T, typename C>
 __device__ __forceinline__ void load_cell(const StageParams<T>& P, const T* buf, int i, int j, int k,
-                                          Cell<DIM, NPH>& c) {
+                                          Cell<DIM, NPH, C>& c) {
     constexpr int NCV = DIM + 2 + 2 * NPH;
     const Grid& g = P.g;
     bool fn[3] = {false, false, false}, ft[3] = {false, false, false};
@@ -245,10 +327,10 @@ __device__ __forceinline__ void load_cell(const StageParams<T>& P, const T* buf,
     if (DIM >= 2) sj = map_axis(clampi(j, -g.ngy, g.ny - 1 + g.ngy), g.ny, g.bc[2], g.bc[3], fn[1], ft[1]);
     if (DIM >= 3) sk = map_axis(clampi(k, -g.ngz, g.nz - 1 + g.ngz), g.nz, g.bc[4], g.bc[5], fn[2], ft[2]);
     size_t id = cell_index(g, si, sj, sk);
-    double U[NCV];
+    T U[NCV];
 #pragma unroll
-    for (int q = 0; q < NCV; ++q) U[q] = (double)buf[(size_t)q * P.total + id];
-    prim_of<DIM, NPH>(P.eos, P.mix, U, c);
+    for (int q = 0; q < NCV; ++q) U[q] = buf[(size_t)q * P.total + id];
+    convert_cell<DIM, NPH, T, C>(P.eos, P.mix, U, c);
     // velocity component d flips with the normal flag of axis d and the
     // tangential flags of the other axes (onion-peel composition)
 #pragma unroll
@@ -287,8 +369,8 @@ __device__ __forceinline__ double load_sigma(const StageParams<T>& P, int i, int
 template <int DIM, int NPH>
 constexpr int n_recon() { return DIM + 2 + NPH; }
 
-template <int DIM, int NPH, typename T>
-__device__ __forceinline__ void recon_vars(const Cell<DIM, NPH>& c, T* v) {
+template <int DIM, int NPH, typename T, typename C>
+__device__ __forceinline__ void recon_vars(const Cell<DIM, NPH, C>& c, T* v) {
     v[0] = (T)c.rho;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) v[1 + d] = (T)c.u[d];
@@ -318,10 +400,8 @@ __device__ __forceinline__ void face_state(const StageParams<T>& P, const T* v, 
 #pragma unroll
         for (int ph = 0; ph < NPH; ++ph) {
             w.a[ph] = v[DIM + 2 + ph];
-            T gk = (T)P.mix.gamma[ph];
-            T gm1 = gk - T(1.0);
-            sInv += w.a[ph] / gm1;
-            sPi += w.a[ph] * gk * (T)P.mix.pInf[ph] / gm1;
+            sInv += mix_inv<T>(P.mix, ph, w.a[ph]);
+            sPi += mix_pi<T>(P.mix, ph, w.a[ph]);
         }
         w.g = T(1.0) + T(1.0) / sInv;
         w.pi = (w.g - T(1.0)) / w.g * sPi;
@@ -335,9 +415,8 @@ __device__ __forceinline__ void face_state(const StageParams<T>& P, const T* v, 
 template <int DIM, int NPH>
 constexpr int n_fc() { return DIM + 2 + 2 * NPH + 2; }
 
-template <int DIM, int NPH, typename T>
-__device__ __forceinline__ void face_contrib(const T* F, double rhoL, double rhoR, const double* arL,
-                                             const double* arR, T* fc) {
+template <int DIM, int NPH, typename T, typename C>
+__device__ __forceinline__ void face_contrib(const T* F, C rhoL, C rhoR, const C* arL, const C* arR, T* fc) {
     fc[0] = F[0];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) fc[1 + d] = F[1 + d];
@@ -345,9 +424,10 @@ __device__ __forceinline__ void face_contrib(const T* F, double rhoL, double rho
     if constexpr (NPH > 0) {
         // upwind partial-density flux (src/ExplicitSolver.cpp:307-326)
         const bool up = F[0] >= T(0);
-        double ru = dmax_ref(up ? rhoL : rhoR, 1e-14);
+        double ru = dmax_ref((double)(up ? rhoL : rhoR), 1e-14);
 #pragma unroll
-        for (int ph = 0; ph < NPH; ++ph) fc[DIM + 2 + ph] = (T)(((up ? arL[ph] : arR[ph]) / ru) * (double)F[0]);
+        for (int ph = 0; ph < NPH; ++ph)
+            fc[DIM + 2 + ph] = (T)(((double)(up ? arL[ph] : arR[ph]) / ru) * (double)F[0]);
 #pragma unroll
         for (int ph = 0; ph < NPH; ++ph) fc[DIM + 2 + NPH + ph] = F[DIM + 4 + ph];
     }
@@ -369,15 +449,97 @@ __device__ __forceinline__ void accumulate(T* acc, const T* lo, const T* hi, T c
     acc[NF - 1] = d + c * hi[NF - 1];
 }
 
+// Body force and face stresses of one cell (src/ExplicitSolver.cpp:498-535,
+// src/SemiImplicitSolver.cpp:484-520), added to the flux part in FP64.
+template <int DIM, typename T, typename C>
+__device__ __forceinline__ void add_sources(const StageParams<T>& P, size_t id, T* acc, C rhoC, const C* uC) {
+    if (P.bodyForce) {
+        const double a[3] = {P.accel[0], P.accel[1], P.accel[2]};
+        const double rho = (double)rhoC;
+        double work = (double)uC[0] * a[0];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            acc[1 + d] = (T)((double)acc[1 + d] + rho * a[d]);
+            if (d > 0) work += (double)uC[d] * a[d];
+        }
+        acc[DIM + 1] = (T)((double)acc[DIM + 1] + rho * work);
+    }
+    if (P.visc) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) acc[1 + d] = (T)((double)acc[1 + d] + P.visc[(size_t)d * P.total + id]);
+        acc[DIM + 1] = (T)((double)acc[DIM + 1] + P.visc[(size_t)DIM * P.total + id]);
+    }
+}
+
+// Stage update of one cell of a single-phase explicit FP32 state, in FP32
+// (the stored state is FP32, so the FP64 combination would only add one
+// final rounding back to FP32); the dt of the new state converts in FP32 for
+// an ideal gas and in FP64 for a stiffened gas.
+template <int DIM, typename T, bool SRC>
+__device__ __forceinline__ double finish_cell_f32(const StageParams<T>& P, size_t id, T* acc,
+                                                  const Cell<DIM, 0, float>& cell, const double* h, double dt,
+                                                  bool wantDt, const T* Upre, const T* U0pre) {
+    constexpr int NCV = DIM + 2;
+    if (SRC) add_sources<DIM, T, float>(P, id, acc, cell.rho, cell.u);
+    float un[NCV];
+    const bool use0 = P.c2 != 0.0;
+    const float c1 = (float)P.c1, c2 = (float)P.c2, c3dt = (float)(P.c3 * dt);
+    const float c4 = (float)P.c4, c4i = (float)P.c4inv;
+    const float dtf = (float)dt;
+#pragma unroll
+    for (int q = 0; q < NCV; ++q) {
+        const float U = (float)(Upre ? Upre[q] : P.src[(size_t)q * P.total + id]);
+        const float R = (float)acc[q];
+        if (P.rk1) {
+            un[q] = fmaf(dtf, R, U);
+        } else {
+            float a = c1 * U;
+            if (use0) a = fmaf(c2, (float)(U0pre ? U0pre[q] : P.u0[(size_t)q * P.total + id]), a);
+            const float b = fmaf(c3dt, R, a);
+            // b / c4: exact product for c4 = 1, 2, 4; for c4 = 3 the product with
+            // the rounded 1/3 is biased by +3e-8 (a drift of the whole state over
+            // many steps), so the quotient gets one residual correction
+            float qt = b * c4i;
+            if (!P.c4pow2) qt = fmaf(fmaf(-c4, qt, b), c4i, qt);
+            un[q] = qt;
+        }
+        P.dst[(size_t)q * P.total + id] = (T)un[q];
+    }
+    if (!wantDt) return 1.0e300;
+    if (P.eos.stiffened) {
+        double Uq[NCV];
+#pragma unroll
+        for (int q = 0; q < NCV; ++q) Uq[q] = (double)un[q];
+        Cell<DIM, 0, double> c;
+        prim_of<DIM, 0>(P.eos, P.mix, Uq, c);
+        const double cs = sqrt_fast(div_fast(P.eos.gamma * dmax_ref(c.p + P.eos.pInf, 1e-14), dmax_ref(c.rho, 1e-14)));
+        double dc = div_fast(h[0], fabs(c.u[0]) + cs);
+#pragma unroll
+        for (int d = 1; d < DIM; ++d) dc = dmin_ref(dc, div_fast(h[d], fabs(c.u[d]) + cs));
+        return dc;
+    }
+    Cell<DIM, 0, float> c;
+    prim_of_f32<DIM>(P.eos, un, c);
+    const float cs = __builtin_amdgcn_sqrtf((float)P.eos.gamma * __builtin_fmaxf(c.p, 1e-14f) *
+                                            frcp(__builtin_fmaxf(c.rho, 1e-14f)));
+    float dc = (float)h[0] * frcp(fabsf(c.u[0]) + cs);
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) dc = __builtin_fminf(dc, (float)h[d] * frcp(fabsf(c.u[d]) + cs));
+    return (double)dc;
+}
+
 // Sources and stage update of one owned cell; returns the acoustic cell dt
 // of the new state when requested.  acc holds the complete flux part.
 // SRC = false: an instantiation for configurations without body force and
 // face stresses, which carries none of their code (P.bodyForce and P.visc
 // must then be off).
-template <int DIM, int NPH, bool SEMI, typename T, bool SRC = true>
-__device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id, T* acc, const Cell<DIM, NPH>& cell,
-                                              const double* h, double dt, bool wantDt, const T* Upre = nullptr,
-                                              const T* U0pre = nullptr) {
+template <int DIM, int NPH, bool SEMI, typename T, bool SRC = true, typename C = double>
+__device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id, T* acc,
+                                              const Cell<DIM, NPH, C>& cell, const double* h, double dt, bool wantDt,
+                                              const T* Upre = nullptr, const T* U0pre = nullptr) {
+    if constexpr (sizeof(C) == 4) {
+        return finish_cell_f32<DIM, T, SRC>(P, id, acc, cell, h, dt, wantDt, Upre, U0pre);
+    }
     constexpr int NCV = DIM + 2 + 2 * NPH;
     constexpr int QP = NCV, QD = NCV + 1;
     // sources: alpha div(u) (src/ExplicitSolver.cpp:486-496), p div(u)
@@ -385,34 +547,27 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
 #pragma unroll
     for (int ph = 0; ph < NPH; ++ph) acc[DIM + 2 + NPH + ph] = acc[DIM + 2 + NPH + ph] + (T)cell.a[ph] * acc[QD];
     if constexpr (SEMI) acc[QP] = acc[QP] + (T)cell.p * acc[QD];
-    if (SRC && P.bodyForce) {
-        // body force on the stage state (src/ExplicitSolver.cpp:498-521,
-        // src/SemiImplicitSolver.cpp:484-506)
-        const double a[3] = {P.accel[0], P.accel[1], P.accel[2]};
-        const double rho = cell.rho;
-        double work = cell.u[0] * a[0];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            acc[1 + d] = (T)((double)acc[1 + d] + rho * a[d]);
-            if (d > 0) work += cell.u[d] * a[d];
-        }
-        acc[DIM + 1] = (T)((double)acc[DIM + 1] + rho * work);
-    }
-    if (SRC && P.visc) {
-        // viscous and capillary stress (src/ExplicitSolver.cpp:523-535)
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) acc[1 + d] = (T)((double)acc[1 + d] + P.visc[(size_t)d * P.total + id]);
-        acc[DIM + 1] = (T)((double)acc[DIM + 1] + P.visc[(size_t)DIM * P.total + id]);
-    }
+    if (SRC) add_sources<DIM, T, C>(P, id, acc, cell.rho, cell.u);
     double un[NCV];
     double U0v[NCV];
+    double Uv[NCV];
     const bool use0 = P.c2 != 0.0;
 #pragma unroll
     for (int q = 0; q < NCV; ++q)
         U0v[q] = use0 ? (double)(U0pre ? U0pre[q] : P.u0[(size_t)q * P.total + id]) : 0.0;
 #pragma unroll
+    for (int q = 0; q < NCV; ++q) Uv[q] = (double)(Upre ? Upre[q] : P.src[(size_t)q * P.total + id]);
+    // stored energies relative to the volume fractions they were stored with:
+    // the update runs on the total energies (offsets added back here and the
+    // new state's offset taken off after the clamps)
+    const bool eo = NPH > 0 && !SEMI && P.mix.eoff;
+    if (eo) {
+        Uv[DIM + 1] += energy_offset<DIM, NPH>(P.mix, Uv);
+        if (use0) U0v[DIM + 1] += energy_offset<DIM, NPH>(P.mix, U0v);
+    }
+#pragma unroll
     for (int q = 0; q < NCV; ++q) {
-        double U = (double)(Upre ? Upre[q] : P.src[(size_t)q * P.total + id]);
+        double U = Uv[q];
         double Rq = (double)acc[q];
         if (SEMI && q == DIM + 1) {
             un[q] = U;  // total energy is rebuilt by the correction step
@@ -443,6 +598,12 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
         for (int ph = 0; ph < NPH; ++ph) as += un[DIM + 2 + NPH + ph];
 #pragma unroll
         for (int ph = 0; ph < NPH; ++ph) un[DIM + 2 + NPH + ph] /= as;
+        if (eo) {
+            double st[NCV];
+#pragma unroll
+            for (int q = 0; q < NCV; ++q) st[q] = (double)(T)un[q];
+            un[DIM + 1] -= energy_offset<DIM, NPH>(P.mix, st);
+        }
     }
 #pragma unroll
     for (int q = 0; q < NCV; ++q)
@@ -473,19 +634,19 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
     double cs;
     if constexpr (NPH == 0) {
         double pinf = P.eos.stiffened ? P.eos.pInf : 0.0;
-        cs = sqrt(P.eos.gamma * dmax_ref(c.p + pinf, 1e-14) / dmax_ref(c.rho, 1e-14));
+        cs = sqrt_fast(div_fast(P.eos.gamma * dmax_ref(c.p + pinf, 1e-14), dmax_ref(c.rho, 1e-14)));
     } else {
         cs = mix_sound(P.mix, c.rho, c.p, c.a, c.ar, NPH);
     }
-    double dc = h[0] / (fabs(c.u[0]) + cs);
+    double dc = t_div<double, NPH == 0>(h[0], fabs(c.u[0]) + cs);
 #pragma unroll
-    for (int d = 1; d < DIM; ++d) dc = dmin_ref(dc, h[d] / (fabs(c.u[d]) + cs));
+    for (int d = 1; d < DIM; ++d) dc = dmin_ref(dc, t_div<double, NPH == 0>(h[d], fabs(c.u[d]) + cs));
     return dc;
 }
 
-template <int DIM, int NPH, int R, typename T>
-__device__ __forceinline__ void recon_window(const StageParams<T>& P, const Cell<DIM, NPH>* win, const T* sigwin,
-                                             int first, bool useSig, T eps, bool weno,
+template <int DIM, int NPH, int R, typename T, typename C>
+__device__ __forceinline__ void recon_window(const StageParams<T>& P, const Cell<DIM, NPH, C>* win, const T* sigwin,
+                                             int first, bool useSig, T eps, int weno,
                                              FaceState<DIM, NPH, T>& minusS, FaceState<DIM, NPH, T>& plusS) {
     constexpr int NV = n_recon<DIM, NPH>();
     constexpr int S = 2 * R - 1;
@@ -586,7 +747,7 @@ stage_xy_kernel(StageParams<T> P) {
     else if (lane < 2 * E) ecol = X0 + kWave - 1 + (lane - E);
 
     const T eps = (T)P.eps;
-    const bool weno = P.weno != 0;
+    const int weno = P.weno;
     const double dt = *P.dt;
     const int cc = col < g.nx ? (col < 0 ? 0 : col) : g.nx - 1;
     const double dxc = g.dxw[cc + g.ngx];
@@ -598,7 +759,8 @@ stage_xy_kernel(StageParams<T> P) {
     const double cxU = (dyU * dzk) / ((dxc * dyU) * dzk);
     const double cyU = (dxc * dzk) / ((dxc * dyU) * dzk);
 
-    Cell<DIM, NPH> win[WS];
+    using C = CellT<T, NPH, SEMI>;
+    Cell<DIM, NPH, C> win[WS];
     T sigwin[WS];
     FaceState<DIM, NPH, T> plusPrev;
     T Glo[NF];
@@ -629,7 +791,7 @@ stage_xy_kernel(StageParams<T> P) {
 #pragma unroll
         for (int q = 0; q < NF; ++q) Ghi[q] = T(0);
         double cy = 0.0, dyr = 1.0;
-        Cell<DIM, NPH> cur;
+        Cell<DIM, NPH, C> cur;
         T sigc = T(0);
         if (YDIR) {
 #pragma unroll
@@ -688,7 +850,7 @@ stage_xy_kernel(StageParams<T> P) {
             if (useSig) sp[NV][lane + E] = sigc;
         }
         if (E > 0 && lane < 2 * E) {
-            Cell<DIM, NPH> ce;
+            Cell<DIM, NPH, C> ce;
             load_cell<DIM, NPH>(P, P.src, ecol, r, kz, ce);
             T vv[NV];
             recon_vars<DIM, NPH, T>(ce, vv);
@@ -739,8 +901,8 @@ stage_xy_kernel(StageParams<T> P) {
         riemann<RS, 0, DIM, NPH, T>(pX, rX, incP, igr, Fx);
         T FCx[NF];
         {
-            double rhoR = __shfl_down(cur.rho, 1, kWave);
-            double arR[NPH > 0 ? NPH : 1];
+            C rhoR = __shfl_down(cur.rho, 1, kWave);
+            C arR[NPH > 0 ? NPH : 1];
 #pragma unroll
             for (int ph = 0; ph < NPH; ++ph) arR[ph] = __shfl_down(cur.ar[ph], 1, kWave);
             face_contrib<DIM, NPH, T>(Fx, cur.rho, rhoR, cur.ar, arR, FCx);
@@ -761,9 +923,9 @@ stage_xy_kernel(StageParams<T> P) {
             } else {
                 const double h[3] = {dxc, dyr, dzk};
                 double dc = (YDIR && PU)
-                                ? finish_cell<DIM, NPH, SEMI, T, SRC>(P, id, acc, cur, h, dt, P.dtPartial != nullptr, preU,
-                                                                      preU0)
-                                : finish_cell<DIM, NPH, SEMI, T, SRC>(P, id, acc, cur, h, dt, P.dtPartial != nullptr);
+                                ? finish_cell<DIM, NPH, SEMI, T, SRC, C>(P, id, acc, cur, h, dt, P.dtPartial != nullptr,
+                                                                         preU, preU0)
+                                : finish_cell<DIM, NPH, SEMI, T, SRC, C>(P, id, acc, cur, h, dt, P.dtPartial != nullptr);
                 dtMin = dmin_ref(dtMin, dc);
             }
         }
@@ -810,7 +972,7 @@ stage_z_kernel(StageParams<T> P) {
     const bool incP = !SEMI;
     const bool igr = P.igr != 0;
     const T eps = (T)P.eps;
-    const bool weno = P.weno != 0;
+    const int weno = P.weno;
     const double dt = *P.dt;
     const int cc = owned ? col : g.nx - 1;
     const double dxc = g.dxw[cc + g.ngx];
@@ -818,7 +980,8 @@ stage_z_kernel(StageParams<T> P) {
     double dtMin = 1.0e300;
     const double czU = (dxc * dyj) / ((dxc * dyj) * g.dzw[g.ngz]);
 
-    Cell<DIM, NPH> win[WS];
+    using C = CellT<T, NPH, SEMI>;
+    Cell<DIM, NPH, C> win[WS];
     T sigwin[WS];
     FaceState<DIM, NPH, T> plusPrev;
     T Hlo[NF];
@@ -876,7 +1039,7 @@ stage_z_kernel(StageParams<T> P) {
             for (int q = 0; q < NF; ++q) acc[q] = PP ? accPre[q] : P.part[(size_t)q * P.total + id];
             accumulate<DIM, NPH, T>(acc, Hlo, Hhi, (T)cz, false);
             const double h[3] = {dxc, dyj, dzr};
-            double dc = finish_cell<DIM, NPH, SEMI, T>(P, id, acc, win[WROW], h, dt, P.dtPartial != nullptr);
+            double dc = finish_cell<DIM, NPH, SEMI, T, true, C>(P, id, acc, win[WROW], h, dt, P.dtPartial != nullptr);
             dtMin = dmin_ref(dtMin, dc);
         }
 #pragma unroll

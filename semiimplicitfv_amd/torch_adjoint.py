@@ -31,12 +31,19 @@ def _ext():
 
 @dataclass
 class RunSpec:
+    """One fixed-dt run shape.  The solver handle it owns records the
+    checkpoints of one forward run at a time; `_recorded` names that run
+    (a counter over forwards), so a backward whose run has since been
+    replaced (chained or repeated applies) records its own run again from the
+    U0 and gamma its forward saved."""
     mesh: object
     config: object
     nsteps: int
     checkpoint_every: int = 20
     _solver: object = None
     _gamma: float = None
+    _runs: int = 0
+    _recorded: int = -1
 
     def solver(self, gamma: float):
         if self._solver is None or self._gamma != gamma:
@@ -45,17 +52,29 @@ class RunSpec:
             eos = IdealGasEOS(gamma, 1.0, self.config)
             self._solver = ExplicitSolver(self.mesh, RusanovSolver(eos), eos, None, self.config)
             self._gamma = gamma
+            self._recorded = -1
         return self._solver
+
+    def record(self, U0, gamma: float, run_id: int):
+        """Runs the forward steps from U0 and keeps their checkpoints."""
+        ext = _ext()
+        s = self.solver(gamma)
+        ext.load_state(s._h.value, U0.detach().contiguous())
+        s.adjoint_forward(self.config, self.nsteps, self.checkpoint_every)
+        self._recorded = run_id
+        return s
 
 
 class ExplicitRun(torch.autograd.Function):
     @staticmethod
     def forward(ctx, U0, gamma, spec: RunSpec):
         ext = _ext()
-        s = spec.solver(float(gamma))
-        ext.load_state(s._h.value, U0.detach().contiguous())
-        s.adjoint_forward(spec.config, spec.nsteps, spec.checkpoint_every)
+        spec._runs += 1
+        ctx.run_id = spec._runs
+        ctx.gamma = float(gamma)
+        s = spec.record(U0, ctx.gamma, ctx.run_id)
         UN = ext.resident_state(s._h.value, U0).clone()
+        ctx.save_for_backward(U0.detach())
         ctx.spec = spec
         ctx.gamma_shape = gamma.shape
         return UN
@@ -63,6 +82,12 @@ class ExplicitRun(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gUN):
         ext = _ext()
-        s = ctx.spec.solver(ctx.spec._gamma)
+        spec = ctx.spec
+        (U0,) = ctx.saved_tensors
+        if spec._recorded != ctx.run_id or spec._gamma != ctx.gamma:
+            # another forward replaced this run's checkpoints: record it again
+            s = spec.record(U0, ctx.gamma, ctx.run_id)
+        else:
+            s = spec.solver(ctx.gamma)
         grad, dg = ext.adjoint_backward(s._h.value, gUN.contiguous())
         return grad, dg.reshape(ctx.gamma_shape).to(gUN.device), None

@@ -594,3 +594,54 @@ def ensemble_sample(k: int = 16, nmax: int = 1024, seed: int = 3):
     small = np.flatnonzero(p["ncells"] <= nmax)
     rng = np.random.default_rng(seed)
     return p, sorted(rng.choice(small, size=k, replace=False).tolist())
+
+
+def stiffened_tube(n: int, left, right, x0: float = 0.5, riemann: str = "hllc", cfl: float = 0.6,
+                   gamma: float = 4.4, pInf: float = 6.0e8):
+    """Single-phase stiffened-gas (water: gamma 4.4, pInf 6e8) Riemann problem
+    on [0,1] with n cells, (rho, u, p) = left for centroids x < x0, right
+    otherwise; outflow ends, WENO5 + SSP-RK3."""
+    cfg = SimulationConfig(dim=1, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5)
+    cfg.explicitParams.cfl = cfl
+    cfg.explicitParams.maxDt = 1.0
+    cfg.validate()
+    mesh = uniform_mesh(cfg, n, 0.0, 1.0)
+    for f in (0, 1):
+        mesh.setBoundaryCondition(f, BoundaryCondition.Outflow)
+    st = SolutionState()
+    st.allocate(mesh.totalCells(), cfg)
+    eos = StiffenedGasEOS(gamma, pInf, 1.0, cfg)
+    xc, _, _ = mesh.centroids()
+    lft = xc < x0
+    shape = (1, 1, n)
+    rho = np.where(lft, left[0], right[0]).reshape(shape)
+    u = [np.where(lft, left[1], right[1]).reshape(shape)]
+    p = np.where(lft, left[2], right[2]).reshape(shape)
+    _fill(mesh, st, eos, rho, u, p)
+    from semiimplicitfv_amd.cases import riemann_by_name
+    return cfg, mesh, st, eos, riemann_by_name(riemann, eos)
+
+
+def water_blast_2d(n: int, p_in: float = 1.0e9, p_out: float = 1.0e5, radius: float = 0.15, cfl: float = 0.6):
+    """Single-phase stiffened-gas water (gamma 4.4, pInf 6e8, rho 1000 with a
+    1 % density bump) on the unit square, a high-pressure disc at the centre
+    (pressure smoothed over two cells), slip walls, WENO5 + HLLC + SSP-RK3."""
+    cfg = SimulationConfig(dim=2, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5)
+    cfg.explicitParams.cfl = cfl
+    cfg.explicitParams.maxDt = 1.0
+    cfg.validate()
+    mesh = uniform_mesh(cfg, n, 0.0, 1.0, n, 0.0, 1.0)
+    for f in range(4):
+        mesh.setBoundaryCondition(f, BoundaryCondition.SlipWall)
+    st = SolutionState()
+    st.allocate(mesh.totalCells(), cfg)
+    eos = StiffenedGasEOS(4.4, 6.0e8, 1.0, cfg)
+    xc, yc, _ = mesh.centroids()
+    Y, X = np.meshgrid(yc, xc, indexing="ij")
+    r = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2)
+    w = 0.5 * (1.0 - np.tanh((r - radius) / (2.0 / n)))
+    p = (p_out + (p_in - p_out) * w).reshape(1, n, n)
+    rho = (1000.0 * (1.0 + 0.01 * np.cos(2 * np.pi * X) * np.sin(2 * np.pi * Y))).reshape(1, n, n)
+    u = [np.zeros((1, n, n)), np.zeros((1, n, n))]
+    _fill(mesh, st, eos, rho, u, p)
+    return cfg, mesh, st, eos, HLLCSolver(eos)

@@ -55,3 +55,21 @@ def test_unsupported_terms_are_rejected():
     cfg.surfaceTensionParams.sigma = 0.07
     rc, err = _create(cfg)
     assert rc == _abi.ERR_UNSUPPORTED and "surface tension" in err
+
+
+def test_fp32_ensemble_refuses_stiffened_gas():
+    """Ensemble tubes keep no stiffened-gas energy offset in FP32 storage, so
+    that combination is refused before any device call (the solver handles
+    carry the offset and accept it)."""
+    from semiimplicitfv_amd import StiffenedGasEOS
+
+    lib = _abi.load_library()
+    cfg = SimulationConfig(dim=1, nGhost=3, RKOrder=3, reconOrder=ReconstructionOrder.WENO5)
+    ncells = (C.c_int * 1)(16)
+    nodes = (C.c_double * 17)(*[i / 16.0 for i in range(17)])
+    cc = cfg.to_c()
+    ec = StiffenedGasEOS(4.4, 6.0e8, 1.0, cfg).to_c()
+    e = C.c_void_p()
+    rc = lib.sifv_ensemble_create(1, ncells, nodes, C.byref(cc), 2, C.byref(ec), _abi.MIXED, C.byref(e))
+    assert rc == _abi.ERR_UNSUPPORTED
+    assert "stiffened" in lib.sifv_last_error().decode()

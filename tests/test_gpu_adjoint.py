@@ -162,3 +162,63 @@ def test_torch_autograd_function(gpu):
         a, b = mesh.interior(g[q]), mesh.interior(grad_ref[q])
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), q
     assert abs(gamma.grad.item() - dgam_ref) <= 1e-12 * abs(dgam_ref)
+
+
+def test_torch_chained_and_repeated_applies(gpu):
+    """Two chained applies (U1 = run(U0), U2 = run(U1)) through one RunSpec:
+    the backward of the first run must use its own recorded run, not the
+    second's.  d/dU0 and d/dgamma equal the direct adjoint of the combined
+    2n-step run.  Then two applies on different inputs before one backward
+    through the first output only: its gradient equals the direct adjoint of
+    that input's run."""
+    import torch
+
+    from semiimplicitfv_amd.torch_adjoint import ExplicitRun, RunSpec
+
+    cfg, mesh, st, eos, rs = _case(24)
+    nsteps = 4
+    J_ref, grad_ref, dgam_ref = _gradient(cfg, mesh, st, eos, rs, 2 * nsteps, 3)
+    n = mesh.totalCells()
+    idx = np.zeros(n, dtype=bool)
+    mesh.interior(idx)[...] = True
+    interior = torch.from_numpy(idx).to("cuda")
+    dV = (10.0 / mesh.nx()) * (10.0 / mesh.ny())
+
+    def J_of(U):
+        rho, mu, mv = U[0][interior], U[1][interior], U[2][interior]
+        return (0.5 * (mu * mu + mv * mv) / rho * dV).sum()
+
+    U0 = torch.tensor(np.stack([st.field(f) for f in FIELDS]), dtype=torch.float64, device="cuda",
+                      requires_grad=True)
+    gamma = torch.tensor(1.4, dtype=torch.float64, device="cuda", requires_grad=True)
+    spec = RunSpec(mesh, cfg, nsteps, 3)
+    U1 = ExplicitRun.apply(U0, gamma, spec)
+    U2 = ExplicitRun.apply(U1, gamma, spec)
+    J = J_of(U2)
+    J.backward()
+    assert abs(J.item() - J_ref) <= 1e-13 * abs(J_ref)
+    g = U0.grad.cpu().numpy()
+    for q in range(4):
+        a, b = mesh.interior(g[q]), mesh.interior(grad_ref[q])
+        assert np.abs(a - b).max() <= 1e-11 * np.abs(b).max(), q
+    assert abs(gamma.grad.item() - dgam_ref) <= 1e-11 * abs(dgam_ref)
+
+    # repeated applies on two inputs, backward through the first only
+    J1_ref, g1_ref, dg1_ref = _gradient(cfg, mesh, st, eos, rs, nsteps, 3)
+    st_b = st.copy()
+    mesh.interior(st_b.rho)[...] *= 1.01
+    mesh.interior(st_b.rhoE)[...] *= 1.01
+    Ua = torch.tensor(np.stack([st.field(f) for f in FIELDS]), dtype=torch.float64, device="cuda",
+                      requires_grad=True)
+    Ub = torch.tensor(np.stack([st_b.field(f) for f in FIELDS]), dtype=torch.float64, device="cuda",
+                      requires_grad=True)
+    gam2 = torch.tensor(1.4, dtype=torch.float64, device="cuda", requires_grad=True)
+    Va = ExplicitRun.apply(Ua, gam2, spec)
+    Vb = ExplicitRun.apply(Ub, gam2, spec)
+    assert not torch.equal(Va, Vb)
+    J_of(Va).backward()
+    g = Ua.grad.cpu().numpy()
+    for q in range(4):
+        a, b = mesh.interior(g[q]), mesh.interior(g1_ref[q])
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), q
+    assert abs(gam2.grad.item() - dg1_ref) <= 1e-12 * abs(dg1_ref)

@@ -185,3 +185,31 @@ def test_conservation_and_uniform_state_large(gpu):
     r = mesh.interior(g.download_field(0))
     assert np.abs(r - 1.0).max() <= 1e-14
     g.close()
+
+
+def test_tiny_weno_eps_piecewise_constant(gpu, oracle_c):
+    """wenoEps = 1e-100 (the reference does not bound it from below): on flat
+    regions every beta is 0, so (eps + beta)^2 = 1e-200 and the cleared-
+    denominator products underflow; those cells take the ratio form of the
+    weights.  A density step at rest (contact, uniform pressure) keeps flat
+    regions beside non-smooth stencils.  Bar: 1e-12 against the oracle."""
+    cfg, mesh, st, eos, rs = cases.isentropic_vortex_2d(48)
+    cfg.wenoEps = 1e-100
+    cfg.validate()
+    xc, yc, _ = mesh.centroids()
+    Y, X = np.meshgrid(yc, xc, indexing="ij")
+    rho = np.where(X < 4.0, 1.0, 0.5).reshape(1, 48, 48)
+    p = np.ones((1, 48, 48))
+    u = [np.zeros((1, 48, 48)), np.zeros((1, 48, 48))]
+    cases._fill(mesh, st, eos, rho, u, p)
+    sg, sc, dts = _run_pair(oracle_c, cfg, mesh, st, eos, rs, 6)
+    for dg, dc in dts:
+        assert abs(dg - dc) <= 1e-12 * dc
+    for fid in (0, 1, 2, 4):
+        a = mesh.interior(sg.field(fid))
+        assert np.isfinite(a).all()
+    _assert_close(mesh, sg, sc, fields=(0, 4, 8))
+    for fid in (1, 2):
+        # momenta stay ~0 at rest: compare absolutely against the momentum scale
+        d = np.abs(mesh.interior(sg.field(fid)) - mesh.interior(sc.field(fid))).max()
+        assert d <= 1e-12, (fid, d)
