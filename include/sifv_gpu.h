@@ -156,6 +156,21 @@ int sifv_explicit_step(sifv_handle* h, const sifv_config* cfg, double targetDt,
 int sifv_explicit_advance(sifv_handle* h, const sifv_config* cfg, int nsteps,
                           double endTime, double* time_inout, int* steps_done);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Runs the same steps as sifv_explicit_advance with the same
+ * kernels and results, but calls no hipMalloc, hipFree or allocating library
+ * routine: the dt block partials live in one static device array of the
+ * library (65536 doubles, held per GPU until the process ends) instead of the
+ * handle's lazily allocated buffer.  Every buffer the steps need is allocated
+ * by sifv_create.  Returns SIFV_ERR_INVALID when the mesh needs more than
+ * 65536 partials per step, SIFV_ERR_UNSUPPORTED with kernel timing on.  All
+ * calls on one GPU share the array: calls that overlap on one GPU (from
+ * several host threads) race.  The primitive arrays are refreshed at the end
+ * of the call, so the next step on the handle takes its dt from them. */
+int sifv_explicit_advance_noalloc(sifv_handle* h, const sifv_config* cfg, int nsteps,
+                                  double endTime, double* time_inout, int* steps_done);
+
 /* ---- semi-implicit step ------------------------------------------------ */
 /* Replaces SemiImplicitSolver::step (src/SemiImplicitSolver.cpp:106-236)
  * with the Jacobi PressureSolver, or the PCG one after

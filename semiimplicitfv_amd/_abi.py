@@ -118,6 +118,7 @@ def _declare(lib):
         "sifv_download": (i, [vp, i, P(d), sz]),
         "sifv_explicit_step": (i, [vp, P(Config_C), d, P(d)]),
         "sifv_explicit_advance": (i, [vp, P(Config_C), i, d, P(d), P(i)]),
+        "sifv_explicit_advance_noalloc": (i, [vp, P(Config_C), i, d, P(d), P(i)]),
         "sifv_semi_implicit_step": (i, [vp, P(Config_C), d, P(d), P(i)]),
         "sifv_jacobi_solve": (i, [vp, P(d), P(d), P(d), P(d), d, d, i, P(i), P(d)]),
         "sifv_pcg_solve": (i, [vp, P(d), P(d), P(d), P(d), d, d, i, P(i)]),

@@ -1958,6 +1958,54 @@ int advance_impl(sifv_handle* h, const sifv_config* cfg, int nsteps, double endT
     return SIFV_OK;
 }
 
+// ---- allocation-free time loop ---------------------------------------------
+// The dt block partials of sifv_explicit_advance_noalloc live in this array:
+// one copy per GPU for the life of the process, shared by every call on that
+// GPU (overlapping calls on one GPU race on it).  Nothing has to start at
+// zero: every partial the dt finalize reads was written earlier in the same
+// step.  Host code uses only the per-device address from hipGetSymbolAddress.
+constexpr int kScratchPartials = 65536;  // 512 KiB of FP64 block minima
+static __device__ __align__(256) double scratch[kScratchPartials];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of `scratch` on the handle's device (looked up once per device,
+// with that device current).
+int scratch_for(const sifv_handle* h, double** out) {
+    if (h->device < 0 || h->device >= kMaxDevices) return fail(SIFV_ERR_INVALID, "device id out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[h->device]) {
+        int cur = 0;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != h->device) HIPCHK(hipSetDevice(h->device));
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != h->device) HIPCHK(hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return fail(SIFV_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[h->device] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[h->device];
+    return SIFV_OK;
+}
+
+// Largest number of dt partials one step of this handle writes.
+int partials_needed(const sifv_handle* h) {
+    int need = elementwise_blocks(h);  // dt from the primitive arrays
+    int nstrip, nseg, rows;
+    stage_shape(h, nstrip, nseg, rows);
+    int stage = nstrip * nseg;
+    if (h->dim == 3) {
+        int zs, zn, zr;
+        zpass_shape(h, zs, zn, zr);
+        stage = zs * zn * h->ny;
+    }
+    const bool stress = h->cfg.viscousMu > 0.0 || has_surface_tension(h->cfg);
+    const size_t ncell = (size_t)h->nx * h->ny * h->nz;
+    const int nbs = stress ? (int)std::min<size_t>((ncell + kRedThreads - 1) / kRedThreads, 2048) : 0;
+    return std::max(need, stage + nbs);
+}
 }  // namespace
 
 int sifv_explicit_advance(sifv_handle* h, const sifv_config* cfg, int nsteps, double endTime,
@@ -1966,6 +2014,44 @@ int sifv_explicit_advance(sifv_handle* h, const sifv_config* cfg, int nsteps, do
     if (h->cfg.semiImplicit) return fail(SIFV_ERR_INVALID, "handle was created for the semi-implicit solver");
     if (nsteps < 0) return fail(SIFV_ERR_INVALID, "nsteps < 0");
     return advance_impl(h, cfg, nsteps, endTime, time_inout, steps_done);
+}
+
+int sifv_explicit_advance_noalloc(sifv_handle* h, const sifv_config* cfg, int nsteps, double endTime,
+                                  double* time_inout, int* steps_done) {
+    if (!h || !time_inout) return fail(SIFV_ERR_INVALID, "null argument");
+    if (h->cfg.semiImplicit) return fail(SIFV_ERR_INVALID, "handle was created for the semi-implicit solver");
+    if (nsteps < 0) return fail(SIFV_ERR_INVALID, "nsteps < 0");
+    if (h->timing.on) return fail(SIFV_ERR_UNSUPPORTED, "kernel timing creates events: switch it off for this entry point");
+    const int need = partials_needed(h);
+    if (need > kScratchPartials)
+        return fail(SIFV_ERR_INVALID, "mesh needs " + std::to_string(need) + " dt partials per step; the allocation-free "
+                                      "entry point holds at most " + std::to_string(kScratchPartials));
+    double* sp = nullptr;
+    int rc = scratch_for(h, &sp);
+    if (rc) return rc;
+    double* ownP = h->partial;
+    const int ownCap = h->npartialCap;
+    h->noalloc = true;
+    // the next step's dt partials of an earlier call sit in the handle's own
+    // buffer: start from the primitive arrays instead (same cell minima)
+    if (h->dtNextValid) {
+        rc = refresh_prim(h);
+        h->dtNextValid = false;
+    }
+    h->partial = sp;
+    h->npartialCap = kScratchPartials;
+    h->npartialValid = 0;
+    if (!rc) rc = advance_impl(h, cfg, nsteps, endTime, time_inout, steps_done);
+    // leave no reference to the shared array behind: the next call on this
+    // handle takes its dt from the refreshed primitive arrays
+    if (!rc) rc = refresh_prim(h);
+    if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(SIFV_ERR_HIP, "stream synchronize");
+    h->dtNextValid = false;
+    h->partial = ownP;
+    h->npartialCap = ownCap;
+    h->npartialValid = 0;
+    h->noalloc = false;
+    return rc;
 }
 
 int sifv_fill_ghosts(sifv_handle* h, int varset) {
