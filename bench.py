@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--variants", default="mixed,vortex1024,ensemble,adjoint,bubble,graphs,igr",
                     help="comma list of the side workloads to run with an FP64 main run")
     ap.add_argument("--bubble-n", type=int, default=512, help="cells per side of the 3D bubble workload")
+    ap.add_argument("--bubble-only", action="store_true",
+                    help="only run the 3D bubble at rest (FP64, Jacobi) for --steps steps and print its line "
+                         "(profiling the 3D two-phase stage kernels)")
     ap.add_argument("--rows", type=int, default=0, help="rows marched per wave (0 = built-in choice)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration) and exit")
@@ -284,7 +287,12 @@ def bench_adjoint(n: int = 512, nsteps: int = 400, every: int = 20, ndir: int = 
 
     rng = np.random.default_rng(cases.ADJOINT_SEED + 1)
     base = [st.field(f).copy() for f in F]
-    errs = []
+    errs, abs_errs, dots, errs4 = [], [], [], []
+    h4 = 1e-4
+
+    def J_at(d, dg, t):
+        return J_of([a + t * b for a, b in zip(base, d)], 1.4 + t * dg)
+
     for _ in range(ndir):
         d = []
         for a in base:
@@ -292,19 +300,30 @@ def bench_adjoint(n: int = 512, nsteps: int = 400, every: int = 20, ndir: int = 
             I(x)[...] = rng.standard_normal(I(x).shape) * 1e-2 * np.abs(I(a)).max()
             d.append(x)
         dg = 0.05 * rng.standard_normal()
-        fd = (J_of([a + h * b for a, b in zip(base, d)], 1.4 + h * dg) -
-              J_of([a - h * b for a, b in zip(base, d)], 1.4 - h * dg)) / (2 * h)
+        fd = (J_at(d, dg, h) - J_at(d, dg, -h)) / (2 * h)
+        # fourth-order central difference at a larger step: truncation O(h^4),
+        # rounding noise of J over 2h 100x smaller than at h = 1e-6
+        fd4 = (8.0 * (J_at(d, dg, h4) - J_at(d, dg, -h4)) - (J_at(d, dg, 2 * h4) - J_at(d, dg, -2 * h4))) / (12 * h4)
         dot = sum(float((I(gq) * I(dq)).sum()) for gq, dq in zip(grad, d)) + dgam * dg
         errs.append(abs(dot - fd) / abs(fd))
-    e = np.array(errs)
+        abs_errs.append(abs(dot - fd))
+        errs4.append(abs(dot - fd4) / abs(fd4))
+        dots.append(abs(dot))
+    e, ea, e4 = np.array(errs), np.array(abs_errs), np.array(errs4)
     return {"workload": f"{n}^2 perturbed vortex, WENO5+Rusanov+SSP-RK3 FP64, {nsteps} steps, fixed dt "
                         f"{cfg.explicitParams.constDt:.4e}, checkpoint every {every}",
             "J": J, "dJ_dgamma": dgam, "forward_s": t_fwd, "backward_s": t_bwd,
             "fd_directions": ndir, "fd_h": h, "fd_rel_err_median": float(np.median(e)),
             "fd_rel_err_p90": float(np.percentile(e, 90)), "fd_rel_err_max": float(e.max()),
             "fd_within_1e-6": int((e <= 1e-6).sum()),
-            "note": "the discrete map has kinks (Rusanov wave-speed max), so central differences of single "
-                    "directions can cross one; the adjoint itself matches exact tangents to 1e-9 in the tests"}
+            "fd_abs_err_max": float(ea.max()), "median_abs_directional_derivative": float(np.median(dots)),
+            "min_abs_directional_derivative": float(np.min(dots)),
+            "fd4_h": h4, "fd4_rel_err_median": float(np.median(e4)), "fd4_rel_err_max": float(e4.max()),
+            "fd4_within_1e-6": int((e4 <= 1e-6).sum()),
+            "note": ("central differences at h = 1e-6 carry the rounding noise of J (~1e-14 on J ~ 100) over "
+                     "2h, ~3e-8 absolute: every direction's |adjoint - FD| sits at that floor, and the relative "
+                     "misses are the directions whose derivative nearly cancels (min |dJ.d| / median). The "
+                     "fourth-order difference at h = 1e-4 removes that floor (fd4_*).")}
 
 
 def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred: bool = False,
@@ -503,6 +522,11 @@ def main():
     a = parse()
     if a.copy_probe:
         copy_probe(a.copy_probe)
+        return
+    if a.bubble_only:
+        r = bench_bubble(a.bubble_n, a.steps)
+        r.pop("_pres", None)
+        print(json.dumps(r))
         return
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
