@@ -47,8 +47,10 @@ def parse():
                     help="comma list of the side workloads to run with an FP64 main run")
     ap.add_argument("--bubble-n", type=int, default=512, help="cells per side of the 3D bubble workload")
     ap.add_argument("--bubble-only", action="store_true",
-                    help="only run the 3D bubble at rest (FP64, Jacobi) for --steps steps and print its line "
-                         "(profiling the 3D two-phase stage kernels)")
+                    help="only run the 3D bubble (FP64) for --steps steps and print its line (profiling the 3D "
+                         "two-phase stage kernels and the pressure solvers)")
+    ap.add_argument("--bubble-solver", default="jacobi", choices=["jacobi", "pcg", "mgpcg"])
+    ap.add_argument("--bubble-stirred", action="store_true", help="the stirred bubble (stiff pressure solves)")
     ap.add_argument("--rows", type=int, default=0, help="rows marched per wave (0 = built-in choice)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration) and exit")
@@ -524,7 +526,7 @@ def main():
         copy_probe(a.copy_probe)
         return
     if a.bubble_only:
-        r = bench_bubble(a.bubble_n, a.steps)
+        r = bench_bubble(a.bubble_n, a.steps, stirred=a.bubble_stirred, solver=a.bubble_solver)
         r.pop("_pres", None)
         print(json.dumps(r))
         return

@@ -129,6 +129,48 @@ __device__ __forceinline__ double pcg_row(const PcgArgs& A, const double* v, dou
     return acc;
 }
 
+// Traversal of the interior by bricks of 64 columns x 4 rows x kBrickPlanes
+// planes for the stencil kernels (2D/3D; blocks of 256 threads = 64 x 4): a
+// thread marches its column (i, j) up the brick's planes, so the z
+// neighbours of a cell were read by the same thread one plane earlier and
+// its y neighbours by the sibling waves of the block, both still in L1/L2.
+// Blocks resident on one XCD (block ids are dealt round-robin over the 8
+// XCDs) take a contiguous range of bricks, y fastest, so that neighbouring
+// bricks share that XCD's L2.  The block count is the elementwise kernels'
+// (their per-block partial sums keep one layout).
+#ifndef SIFV_PCG_BRICKS
+#define SIFV_PCG_BRICKS 1
+#endif
+constexpr int kBrickPlanes = 16;
+
+template <int DIM, typename F>
+__device__ __forceinline__ void pcg_bricks(const Grid& g, F&& body) {
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int bx = (g.nx + 63) / 64, by = (g.ny + 3) / 4;
+    const int bz = DIM == 3 ? (g.nz + kBrickPlanes - 1) / kBrickPlanes : 1;
+    const long nbr = (long)bx * by * bz;
+    const int G = (int)gridDim.x;
+    long lo = 0, hi = nbr, t0 = blockIdx.x, step = G;
+    if (G >= 8) {
+        // every XCD holds at least one block: it sweeps its own brick range
+        const int x = (int)blockIdx.x % 8;
+        lo = (long)x * nbr / 8;
+        hi = (long)(x + 1) * nbr / 8;
+        t0 = lo + (int)blockIdx.x / 8;
+        step = (G - x + 7) / 8;
+    }
+    for (long t = t0; t < hi; t += step) {
+        const int cy = (int)(t % by);
+        const long rest = t / by;
+        const int cx = (int)(rest % bx), cz = (int)(rest / bx);
+        const int i = cx * 64 + tx, j = cy * 4 + ty;
+        if (i >= g.nx || j >= g.ny) continue;
+        const int k0 = DIM == 3 ? cz * kBrickPlanes : 0;
+        const int k1 = DIM == 3 ? min(g.nz, k0 + kBrickPlanes) : 1;
+        for (int k = k0; k < k1; ++k) body(i, j, k);
+    }
+}
+
 // Face weights W (low face of each cell; high face of the last cell of a
 // line in the slot after it), mass term s = V/c and diagonal d, from rho and
 // the widths with each row's own operand order.
@@ -257,12 +299,17 @@ __global__ void __launch_bounds__(256) pcg_matvec_kernel(PcgArgs A) {
     __shared__ double sh[256 / kWave];
     const Grid& g = A.g;
     double pq = 0.0;
-    int i, j, k;
-    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
+    auto cell = [&](int i, int j, int k) {
         const size_t id = cell_index(g, i, j, k);
         const double qv = pcg_row<DIM>(A, A.p, A.s[id], i, j, k, id);
         A.q[id] = qv;
         pq += A.p[id] * qv;
+    };
+    if (DIM >= 2 && SIFV_PCG_BRICKS) {
+        pcg_bricks<DIM>(g, cell);
+    } else {
+        int i, j, k;
+        for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) cell(i, j, k);
     }
     pcg_partials(A, sh, 1, pq, 0.0, 0.0);
 }
@@ -538,8 +585,7 @@ __global__ void __launch_bounds__(256) mg_post_fine_kernel(MgArgs M) {
     const Grid& g = A.g;
     constexpr double om = mg_omega(DIM);
     double rz = 0.0;
-    int i, j, k;
-    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
+    auto cell = [&](int i, int j, int k) {
         const size_t id = cell_index(g, i, j, k);
         const double a = pcg_row<DIM>(A, A.q, A.s[id], i, j, k, id);
         const double r = A.r[id];
@@ -547,6 +593,12 @@ __global__ void __launch_bounds__(256) mg_post_fine_kernel(MgArgs M) {
         A.z[id] = z;
         rz += r * z;
         if (M.init) A.p[id] = z;
+    };
+    if (DIM >= 2 && SIFV_PCG_BRICKS) {
+        pcg_bricks<DIM>(g, cell);
+    } else {
+        int i, j, k;
+        for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) cell(i, j, k);
     }
     const double t = wave_sum(rz);
     const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
