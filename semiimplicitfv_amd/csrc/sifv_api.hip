@@ -192,6 +192,7 @@ struct sifv_handle {
     // IGR: right-hand side, diagonal, padded rho; wavefront tiles and flags
     double *igrRhs = nullptr, *igrDiag = nullptr, *igrRho = nullptr;
     int* igrOrder = nullptr;
+    double* igrWrap = nullptr;  // wrap side slots of the sweep: x, y, z parts
     int* igrFlags = nullptr;   // [ntiles] tile epochs, [ntiles] error flag
     int igrNtx = 0, igrNty = 0, igrTj = 1, igrTiles = 0;
     int igrEpoch = 0;
@@ -1368,10 +1369,15 @@ int setup_igr(sifv_handle* h) {
     HIPCHK(hipMemset(h->igrRhs, 0, nb));
     HIPCHK(hipMemset(h->igrDiag, 0, nb));
     HIPCHK(hipMemset(h->igrRho, 0, nb));
+    {
+        const size_t nw = 2 * ((size_t)h->ny * h->nz + (size_t)h->nx * h->nz + (size_t)h->nx * h->ny);
+        HIPCHK(hipMalloc(&h->igrWrap, sizeof(double) * nw));
+        HIPCHK(hipMemset(h->igrWrap, 0, sizeof(double) * nw));
+    }
     HIPCHK(hipMalloc(&h->igrOrder, sizeof(int) * h->igrTiles));
     HIPCHK(hipMemcpy(h->igrOrder, order.data(), sizeof(int) * h->igrTiles, hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&h->igrFlags, sizeof(int) * (h->igrTiles + 1)));
-    HIPCHK(hipMemset(h->igrFlags, 0, sizeof(int) * (h->igrTiles + 1)));
+    HIPCHK(hipMalloc(&h->igrFlags, sizeof(int) * (h->igrTiles + 2)));
+    HIPCHK(hipMemset(h->igrFlags, 0, sizeof(int) * (h->igrTiles + 2)));
     h->igrEpoch = 0;
     // every sweep block must be co-resident (blocks wait on tiles of other blocks)
     int perCu = 0, dev = 0, cus = 0;
@@ -1435,18 +1441,25 @@ int run_igr(sifv_handle* h, int src, const int* active) {
     S.order = h->igrOrder;
     S.flags = h->igrFlags;
     S.err = h->igrFlags + h->igrTiles;
+    S.next = h->igrFlags + h->igrTiles + 1;
+    S.wrapX = h->igrWrap;
+    S.wrapY = S.wrapX + 2 * (size_t)h->ny * h->nz;
+    S.wrapZ = S.wrapY + 2 * (size_t)h->nx * h->nz;
     S.active = active;
     IgrSweepFn sweep = igr_sweep_kernel_for(h->dim);
     const int nblk = h->igrBlocks;
-    for (int it = 0; it < iters; ++it) {
-        S.epoch = ++h->igrEpoch;
-        void* sargs[] = {&S};
-        rc = timed_launch(h, reinterpret_cast<const void*>(sweep), dim3(nblk), dim3(kWave), sargs);
-        if (rc) return rc;
-        rc = fill_dirs(h, f, comp, 1, true);
-        if (rc) return rc;
-    }
-    return SIFV_OK;
+    // all sweeps in one launch; the kernel keeps the 1-deep ghosts as the
+    // fill after every sweep would, the full fill follows the last sweep
+    S.epoch = h->igrEpoch;
+    S.nsweeps = iters;
+    h->igrEpoch += iters;
+#if SIFV_IGR_DYN
+    HIPCHK(hipMemsetAsync(S.next, 0, sizeof(int), h->stream));
+#endif
+    void* sargs[] = {&S};
+    rc = timed_launch(h, reinterpret_cast<const void*>(sweep), dim3(nblk), dim3(kWave), sargs);
+    if (rc) return rc;
+    return fill_dirs(h, f, comp, 1, true);
 }
 
 int run_igr_any(sifv_handle* h, int src, const int* active) {
@@ -1789,6 +1802,7 @@ int sifv_destroy(sifv_handle* h) {
     hipFree(h->igrDiag);
     hipFree(h->igrRho);
     hipFree(h->igrOrder);
+    hipFree(h->igrWrap);
     hipFree(h->igrFlags);
     for (int q = 0; q < 6; ++q) hipFree(h->jac[q]);
     hipFree(h->resid);

@@ -9,7 +9,8 @@
 //                  velocity gradients, the diagonal (1/rho + sum of face
 //                  terms) and rho itself, all from the stage's conservative
 //                  buffer through the ghost mapping.
-// igr_sweep_kernel one Gauss-Seidel sweep in exactly the reference's order.
+// igr_sweep_kernel the Gauss-Seidel sweeps of one solve in exactly the
+//                  reference's order.
 //                  Cell (i,j,k) needs the new values of (i-1,j,k), (i,j-1,k),
 //                  (i,j,k-1) and the old values of the +1 neighbours, so the
 //                  sweep is a wavefront.  The domain is cut into tiles of 64
@@ -23,13 +24,25 @@
 //                  diagonal) -- before it waits for its upstream tiles -- and,
 //                  after the wait, the new values of the left column, the row
 //                  below and the plane below, so the walk's dependent chain
-//                  touches LDS only.  Tiles start when their three upstream
-//                  tiles have published the current epoch in a flag array;
-//                  every block handles tiles in topological order and the grid
-//                  never exceeds the co-resident block count, so the lowest
-//                  unfinished tile can always proceed.  Waits are bounded: a
-//                  wait that exceeds its budget raises an error flag and every
-//                  wave exits.
+//                  touches LDS only.  All the sweeps of a solve run in one
+//                  launch, pipelined: work item (s, tile) starts when the
+//                  tile's three upstream tiles have finished sweep s, the
+//                  tile itself and its three downstream tiles sweep s-1, and,
+//                  on a periodic axis, the far-end tile of its line sweep s-1
+//                  (each tile publishes its finished sweep count in a flag
+//                  array).  Boundary tiles keep the 1-deep scalar ghosts as the
+//                  reference's fill after every sweep leaves them: a zero-
+//                  gradient ghost is written by the tile that owns its source
+//                  cell as it computes it; on a periodic axis the high tile
+//                  writes the low ghost as it computes it and the low tile
+//                  writes its new values into the parity slot of a side array
+//                  that the high tile reads in the next sweep.  Every block
+//                  takes work items in (sweep, topological tile) order and the
+//                  grid never exceeds the
+//                  co-resident block count; every dependency is on an earlier
+//                  item, so the lowest unfinished item can always proceed.
+//                  Waits are bounded: a wait that exceeds its budget raises an
+//                  error flag and every wave exits.
 #pragma once
 
 #include "sifv_stage.hpp"
@@ -103,16 +116,34 @@ struct IgrSweepArgs {
     int ntx, nty, tj;     // tiles: ntx x nty x nz
     int ntiles;
     const int* order;     // tile ids in topological (anti-diagonal) order
-    int* flags;           // per tile: last finished epoch
-    int epoch;
+    int* flags;           // per tile: sweeps finished (cumulative over launches)
+    int epoch;            // flag value of every tile when the launch starts
+    int nsweeps;          // sweeps run by this launch
     int* err;             // set when a wait exceeds its budget
+    int* next;            // work-item counter, zero at launch (SIFV_IGR_DYN)
+    // wrapping high faces: the first cells of each line as the previous sweep
+    // left them, two slots by sweep parity ([2][nz][ny], [2][nz][nx], [2][ny][nx])
+    double* wrapX;
+    double* wrapY;
+    double* wrapZ;
     const int* active;    // device time-loop flag (null: always run)
 };
 
-__device__ __forceinline__ bool igr_wait(const IgrSweepArgs& A, int t) {
+// Flags are polled with relaxed loads and each group of waits ends in one
+// acquire fence: an acquire at agent scope invalidates the XCD's L2, which an
+// acquiring poll loop would do on every iteration (measured 1.8x slower).
+// Blocks take work items by a fixed stride of the grid (0) or from a counter
+// in item order (1; 3 % slower on the 2D vortex at 1024^2 and 2048^2).
+#ifndef SIFV_IGR_DYN
+#define SIFV_IGR_DYN 0
+#endif
+
+__device__ __forceinline__ void igr_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
+__device__ __forceinline__ bool igr_wait(const IgrSweepArgs& A, int t, int epoch) {
     if (t < 0) return true;
     int spins = 0;
-    while (__hip_atomic_load(&A.flags[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < A.epoch) {
+    while (__hip_atomic_load(&A.flags[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
         if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
         if (++spins > (1 << 25)) {
             __hip_atomic_store(A.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -149,8 +180,27 @@ __global__ void __launch_bounds__(64) igr_sweep_kernel(IgrSweepArgs A) {
     const int lane = threadIdx.x;
     const size_t sy = (size_t)g.nxT, sz = (size_t)g.nxT * g.nyT;
     const double alpha = A.alpha;
-    for (int q = blockIdx.x; q < A.ntiles; q += gridDim.x) {
-        const int t = A.order[q];
+    const int ntz = DIM >= 3 ? g.nz : 1;
+    const int plane = A.ntx * A.nty;
+    // per face: does the scalar fill wrap (ghost from the far end of the line)?
+    const bool loX = g.sbc[0] == BK_WRAP, hiX = g.sbc[1] == BK_WRAP;
+    const bool loY = DIM >= 2 && g.sbc[2] == BK_WRAP, hiY = DIM >= 2 && g.sbc[3] == BK_WRAP;
+    const bool loZ = DIM >= 3 && g.sbc[4] == BK_WRAP, hiZ = DIM >= 3 && g.sbc[5] == BK_WRAP;
+    const int nitems = A.nsweeps * A.ntiles;
+#if SIFV_IGR_DYN
+    for (;;) {
+        // every lane takes part in the grab (lane 0 adds 1, the others 0) and
+        // in the flag store below: no lane-dependent branch may separate one
+        // item from the next, or the compiler may run the lanes' loops apart
+        // and read the item from a lane that did not take it
+        const int q = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_fetch_add(A.next, lane == 0 ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (q >= nitems) break;
+#else
+    for (int q = blockIdx.x; q < nitems; q += gridDim.x) {
+#endif
+        const int sw = q / A.ntiles;
+        const int t = A.order[q - sw * A.ntiles];
         const int I = t % A.ntx;
         const int rest = t / A.ntx;
         const int J = rest % A.nty;
@@ -164,6 +214,27 @@ __global__ void __launch_bounds__(64) igr_sweep_kernel(IgrSweepArgs A) {
         const double dx = g.dxw[ci + g.ngx];
         const double d2x = 1.0 / (dx * dx);
         const double d2z = DIM >= 3 ? 1.0 / (g.dzw[K + g.ngz] * g.dzw[K + g.ngz]) : 0.0;
+        const int now = A.epoch + sw + 1, prev = A.epoch + sw;
+        // sweep sw > 0 follows the tile's own sweep sw-1 (another block may
+        // have run it), reads the downstream tiles' values of sweep sw-1, and a
+        // low tile on a wrapping axis replaces the high ghost the far-end tile
+        // read in sweep sw-1 (sweep 0 is ordered by the launch itself)
+        bool ok = true;
+        if (sw > 0) {
+            ok = igr_wait(A, t, prev);
+            ok = ok && igr_wait(A, I < A.ntx - 1 ? t + 1 : -1, prev);
+            ok = ok && igr_wait(A, DIM >= 2 && J < A.nty - 1 ? t + A.ntx : -1, prev);
+            ok = ok && igr_wait(A, DIM >= 3 && K < ntz - 1 ? t + plane : -1, prev);
+            ok = ok && igr_wait(A, (loX || hiX) && I == 0 && A.ntx > 1 ? t + A.ntx - 1 : -1, prev);
+            ok = ok && igr_wait(A, (loY || hiY) && J == 0 && A.nty > 1 ? t + (A.nty - 1) * A.ntx : -1, prev);
+            ok = ok && igr_wait(A, (loZ || hiZ) && K == 0 && ntz > 1 ? t + (ntz - 1) * plane : -1, prev);
+            if (!ok) return;
+            igr_acquire();
+        }
+        // wrapping high faces read the first cells of the line as the previous
+        // sweep left them: from the ghost the fill before the launch wrote, and
+        // later from the side slot the low tile filled in sweep sw-1
+        const int par = (A.epoch + sw) & 1, npar = par ^ 1;
         // operands that this sweep's new values do not change: the +x / +y
         // neighbours are in this tile or in tiles that wait for it
 #pragma unroll 4
@@ -176,14 +247,18 @@ __global__ void __launch_bounds__(64) igr_sweep_kernel(IgrSweepArgs A) {
 #else
                 sFl[jj][lane] = 0.5 * (ri + A.rho[id - 1]);
 #endif
-                sTx[jj][lane] = A.sg[id + 1] / (0.5 * (ri + A.rho[id + 1]));
+                double sr = A.sg[id + 1];
+                if (hiX && sw > 0 && i == g.nx - 1) sr = A.wrapX[((size_t)par * ntz + K) * g.ny + j0 + jj];
+                sTx[jj][lane] = sr / (0.5 * (ri + A.rho[id + 1]));
                 if (DIM >= 2) {
 #if SIFV_IGR_RCP
                     sFly[jj][lane] = 1.0 / (0.5 * (ri + A.rho[id - sy]));
 #else
                     sFly[jj][lane] = 0.5 * (ri + A.rho[id - sy]);
 #endif
-                    sTy[jj][lane] = A.sg[id + sy] / (0.5 * (ri + A.rho[id + sy]));
+                    double st = A.sg[id + sy];
+                    if (hiY && sw > 0 && j0 + jj == g.ny - 1) st = A.wrapY[((size_t)par * ntz + K) * g.nx + i];
+                    sTy[jj][lane] = st / (0.5 * (ri + A.rho[id + sy]));
                 }
                 sRhs[jj][lane] = A.rhs[id];
 #if SIFV_IGR_RCP
@@ -197,10 +272,11 @@ __global__ void __launch_bounds__(64) igr_sweep_kernel(IgrSweepArgs A) {
             const double hy = g.dyw[j0 + lane + g.ngy];
             sD2y[lane] = 1.0 / (hy * hy);
         }
-        bool ok = igr_wait(A, I > 0 ? t - 1 : -1);
-        ok = ok && igr_wait(A, J > 0 ? t - A.ntx : -1);
-        ok = ok && igr_wait(A, K > 0 ? t - A.ntx * A.nty : -1);
+        ok = igr_wait(A, I > 0 ? t - 1 : -1, now);
+        ok = ok && igr_wait(A, DIM >= 2 && J > 0 ? t - A.ntx : -1, now);
+        ok = ok && igr_wait(A, DIM >= 3 && K > 0 ? t - plane : -1, now);
         if (!ok) return;
+        igr_acquire();
         // new values published by the upstream tiles (ghosts at domain edges)
         if (lane < nrows) sLeft[lane] = A.sg[cell_index(g, i0 - 1, j0 + lane, K)];
         if (DIM >= 2 && colOk) sBot[lane] = A.sg[cell_index(g, i, j0 - 1, K)];
@@ -211,7 +287,9 @@ __global__ void __launch_bounds__(64) igr_sweep_kernel(IgrSweepArgs A) {
                     const size_t id = cell_index(g, i, j0 + jj, K);
                     const double ri = A.rho[id];
                     const double fl = 0.5 * (ri + A.rho[id - sz]), fr = 0.5 * (ri + A.rho[id + sz]);
-                    sQz[jj][lane] = A.sg[id - sz] / fl + A.sg[id + sz] / fr;
+                    double su = A.sg[id + sz];
+                    if (hiZ && sw > 0 && K == g.nz - 1) su = A.wrapZ[((size_t)par * g.ny + j0 + jj) * g.nx + i];
+                    sQz[jj][lane] = A.sg[id - sz] / fl + su / fr;
                 }
             }
         }
@@ -244,13 +322,46 @@ __global__ void __launch_bounds__(64) igr_sweep_kernel(IgrSweepArgs A) {
 #else
                 last = (sRhs[jj][lane] + off) / sDiag[jj][lane];
 #endif
-                A.sg[cell_index(g, i, j0 + jj, K)] = last;
+                const size_t id = cell_index(g, i, j0 + jj, K);
+                A.sg[id] = last;
+                // ghosts of the boundary cells as the scalar fill after this
+                // sweep leaves them: zero-gradient ghosts copy the cell, the
+                // low ghost of a periodic axis the last cell of the line
+                if (i == 0) {
+                    if (!loX) A.sg[id - 1] = last;
+                    if (hiX) A.wrapX[((size_t)npar * ntz + K) * g.ny + j0 + jj] = last;
+                }
+                if (i == g.nx - 1) {
+                    if (loX) A.sg[id - (size_t)g.nx] = last;
+                    if (!hiX) A.sg[id + 1] = last;
+                }
+                if (DIM >= 2) {
+                    const int j = j0 + jj;
+                    if (j == 0) {
+                        if (!loY) A.sg[id - sy] = last;
+                        if (hiY) A.wrapY[((size_t)npar * ntz + K) * g.nx + i] = last;
+                    }
+                    if (j == g.ny - 1) {
+                        if (loY) A.sg[id - (size_t)g.ny * sy] = last;
+                        if (!hiY) A.sg[id + sy] = last;
+                    }
+                }
+                if (DIM >= 3) {
+                    if (K == 0) {
+                        if (!loZ) A.sg[id - sz] = last;
+                        if (hiZ) A.wrapZ[((size_t)npar * g.ny + j0 + jj) * g.nx + i] = last;
+                    }
+                    if (K == g.nz - 1) {
+                        if (loZ) A.sg[id - (size_t)g.nz * sz] = last;
+                        if (!hiZ) A.sg[id + sz] = last;
+                    }
+                }
             }
         }
-        // publish the tile: data before flag
+        // publish the tile: data before flag (every lane stores the same value)
         __threadfence();
         __syncthreads();
-        if (lane == 0) __hip_atomic_store(&A.flags[t], A.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&A.flags[t], now, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
