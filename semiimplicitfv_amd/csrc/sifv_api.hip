@@ -193,6 +193,10 @@ struct sifv_handle {
     double *igrRhs = nullptr, *igrDiag = nullptr, *igrRho = nullptr;
     int* igrOrder = nullptr;
     double* igrWrap = nullptr;  // wrap side slots of the sweep: x, y, z parts
+    // 2D strip-streaming sweep (used when igrStrips > 0)
+    int igrStrips = 0, igrPeriod = 0;
+    double *igrRec = nullptr, *igrEdge = nullptr;
+    int* igrProg = nullptr;
     int* igrFlags = nullptr;   // [ntiles] tile epochs, [ntiles] error flag
     int igrNtx = 0, igrNty = 0, igrTj = 1, igrTiles = 0;
     int igrEpoch = 0;
@@ -1387,6 +1391,27 @@ int setup_igr(sifv_handle* h) {
         &perCu, reinterpret_cast<const void*>(igr_sweep_kernel_for(h->dim)), kWave, 0));
     if (perCu < 1 || cus < 1) return fail(SIFV_ERR_HIP, "IGR sweep kernel cannot be resident");
     h->igrBlocks = std::min(h->igrTiles, perCu * cus);
+    // 2D: one wave per 64-column strip when every strip can be resident and
+    // the rows outnumber a strip's lane skew plus two progress chunks (the
+    // wrapping faces of each axis must match: both wrap or neither)
+    h->igrStrips = 0;
+    const int W = (h->nx + kWave - 1) / kWave;
+    const bool wrapPair = (h->sbc[0] == BK_WRAP) == (h->sbc[1] == BK_WRAP) &&
+                          (h->sbc[2] == BK_WRAP) == (h->sbc[3] == BK_WRAP);
+    if (h->dim == 2 && h->ny >= 2 * kStripChunk + kWave + kStripChunk + 16 && wrapPair) {
+        int perCuS = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &perCuS, reinterpret_cast<const void*>(igr_strip_kernels().sweep), kWave, 0));
+        if (W <= perCuS * cus) {
+            const bool perX = h->sbc[0] == BK_WRAP;
+            const int gap = perX ? std::max(0, W * (kWave + kStripChunk) + kWave + kStripChunk - h->ny) : 0;
+            h->igrStrips = W;
+            h->igrPeriod = h->ny + gap;
+            HIPCHK(hipMalloc(&h->igrRec, sizeof(double) * (size_t)W * h->ny * kStripSlot));
+            HIPCHK(hipMalloc(&h->igrEdge, sizeof(double) * 4 * (size_t)W * h->ny));
+            HIPCHK(hipMalloc(&h->igrProg, sizeof(int) * W));
+        }
+    }
     return SIFV_OK;
 }
 
@@ -1427,6 +1452,48 @@ int run_igr(sifv_handle* h, int src, const int* active) {
     f[0] = h->sigma;
     rc = fill_dirs(h, f, comp, 1, true);
     if (rc) return rc;
+    if (h->igrStrips > 0) {
+        const IgrStripKernels K = igr_strip_kernels();
+        IgrStripPackArgs PA;
+        PA.g = h->grid();
+        PA.sigma = h->sigma;
+        PA.rho = h->igrRho;
+        PA.rhs = h->igrRhs;
+        PA.diag = h->igrDiag;
+        PA.alpha = A.alpha;
+        PA.rec = h->igrRec;
+        PA.edgeL = h->igrEdge;
+        PA.edgeR = h->igrEdge + 2 * (size_t)h->igrStrips * h->ny;
+        PA.W = h->igrStrips;
+        void* pargs2[] = {&PA};
+        rc = timed_launch(h, reinterpret_cast<const void*>(K.pack), dim3(elementwise_blocks(h)), dim3(256), pargs2);
+        if (rc) return rc;
+        IgrStripArgs SA;
+        SA.nx = h->nx;
+        SA.ny = h->ny;
+        SA.rec = h->igrRec;
+        SA.edgeL = PA.edgeL;
+        SA.edgeR = PA.edgeR;
+        SA.prog = h->igrProg;
+        SA.err = h->igrFlags + h->igrTiles;
+        SA.active = active;
+        SA.W = h->igrStrips;
+        SA.P = h->igrPeriod;
+        SA.nsweeps = iters;
+        SA.perX = h->sbc[0] == BK_WRAP;
+        SA.perY = h->sbc[2] == BK_WRAP;
+        HIPCHK(hipMemsetAsync(h->igrProg, 0, sizeof(int) * h->igrStrips, h->stream));
+        void* sargs2[] = {&SA};
+        rc = timed_launch(h, reinterpret_cast<const void*>(K.sweep), dim3(h->igrStrips), dim3(kWave), sargs2);
+        if (rc) return rc;
+        Grid g = h->grid();
+        const double* recc = h->igrRec;
+        double* sig = h->sigma;
+        void* uargs[] = {&g, &recc, &sig};
+        rc = timed_launch(h, reinterpret_cast<const void*>(K.unpack), dim3(elementwise_blocks(h)), dim3(256), uargs);
+        if (rc) return rc;
+        return fill_dirs(h, f, comp, 1, true);
+    }
     IgrSweepArgs S;
     S.g = h->grid();
     S.sg = h->sigma;
@@ -1803,6 +1870,9 @@ int sifv_destroy(sifv_handle* h) {
     hipFree(h->igrRho);
     hipFree(h->igrOrder);
     hipFree(h->igrWrap);
+    hipFree(h->igrRec);
+    hipFree(h->igrEdge);
+    hipFree(h->igrProg);
     hipFree(h->igrFlags);
     for (int q = 0; q < 6; ++q) hipFree(h->jac[q]);
     hipFree(h->resid);

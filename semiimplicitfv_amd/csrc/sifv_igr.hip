@@ -43,4 +43,8 @@ IgrSweepFn igr_sweep_kernel_for(int dim) {
     return igr_sweep_kernel<3>;
 }
 
+IgrStripKernels igr_strip_kernels() {
+    return {igr_strip_pack_kernel<2>, igr_strip_kernel<2>, igr_strip_unpack_kernel<2>};
+}
+
 }  // namespace sifv

@@ -365,4 +365,277 @@ __global__ void __launch_bounds__(64) igr_sweep_kernel(IgrSweepArgs A) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Strip-streaming sweep (2D).  A wave owns a strip of 64 columns and walks
+// every row of every sweep of the solve: at schedule step s lane l updates
+// row r = s - l, where r runs over the sweeps with period P = ny + gap (rows
+// ny..P-1 of a period are idle).  The left neighbour's new value is lane
+// l-1's result of the previous step (DPP shift), the lower one the lane's
+// own previous result, the +x neighbour's old value is what lane l+1 holds
+// as its +y operand (DPP shift), so no tile waits for another tile to finish.
+// Per cell the pack kernel stores the five coefficients of the update
+//     new = c0 + cR*right + cT*top + cB*bottom + cL*left,
+// c0 = rhs/diag, cX = alpha/h^2 / rho_face / diag, and sigma, in a cyclic
+// skew: cell (l, j) of a strip sits in slot (j + l) mod ny, so at any step
+// all lanes read one contiguous slot of 64 x 6 doubles (each lane's six
+// values adjacent: three 16-byte loads per lane and step).  A wave may have
+// 63 vector memory operations outstanding, stores included: two groups of
+// kStripGroup steps are loaded ahead (48 loads) and the edge values a chunk
+// produces are gathered in registers and written once per chunk.  Lane 0's left
+// values and the last lane's right values cross strips through edge arrays
+// (by sweep parity), read per chunk of kStripChunk steps after the strip
+// has polled its neighbours' progress (rows completed by their last lane),
+// which every strip publishes at the end of each chunk.  On a wrapping x
+// axis sweep q of strip 0 needs sweep q-1 of the last strip on the same
+// row, while a strip's last lane trails its lane 0 by 63 rows and each strip
+// trails its left neighbour by up to 64 + kStripChunk steps: the gap between
+// sweeps absorbs that skew (gap = W (64 + C) + 64 + C - ny when positive).
+// Ghost semantics are those of the per-sweep scalar fill: zero-gradient
+// faces read the cell's own previous-sweep value, wrapping y faces the
+// previous result (bottom) and the saved previous-sweep row-0 value (top).
+constexpr int kStripGroup = 8;     // steps per prefetch group
+constexpr int kStripBufs = 3;      // groups held in registers: two in flight while one is used
+constexpr int kStripChunk = kStripGroup * kStripBufs;   // steps per progress hand-off
+constexpr int kStripSlot = 6 * kWave;   // doubles per record slot: [lane][c0, cL, cR, cB, cT, sigma]
+
+struct IgrStripPackArgs {
+    Grid g;
+    const double* sigma;   // padded
+    const double* rho;     // padded, ghosts filled
+    const double* rhs;
+    const double* diag;
+    double alpha;
+    double* rec;           // [W][ny][6][64]
+    double* edgeL;         // [2][W][ny]
+    double* edgeR;         // [2][W][ny]
+    int W;
+};
+
+template <int DIM>
+__global__ void __launch_bounds__(256) igr_strip_pack_kernel(IgrStripPackArgs A) {
+    const Grid& g = A.g;
+    const size_t sy = (size_t)g.nxT;
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, 1); it_.next(i, j, k);) {
+        const size_t id = cell_index(g, i, j, 0);
+        const double ri = A.rho[id];
+        const double rd = 1.0 / A.diag[id];
+        const double hx = g.dxw[i + g.ngx], hy = g.dyw[j + g.ngy];
+        const double ax = A.alpha / (hx * hx), ay = A.alpha / (hy * hy);
+        const int w = i / kWave, l = i - w * kWave;
+        int t = j + l;
+        if (t >= g.ny) t -= g.ny;
+        double* r = A.rec + ((size_t)w * g.ny + t) * kStripSlot + 6 * l;
+        r[0] = A.rhs[id] * rd;
+        r[1] = ax / (0.5 * (ri + A.rho[id - 1])) * rd;
+        r[2] = ax / (0.5 * (ri + A.rho[id + 1])) * rd;
+        r[3] = ay / (0.5 * (ri + A.rho[id - sy])) * rd;
+        r[4] = ay / (0.5 * (ri + A.rho[id + sy])) * rd;
+        const double sg = A.sigma[id];
+        r[5] = sg;
+        // edge values of "sweep -1" (parity slot 1): the initial iterate
+        const int L = min(kWave - 1, g.nx - 1 - w * kWave);
+        if (l == 0) A.edgeL[((size_t)A.W + w) * g.ny + j] = sg;
+        if (l == L) A.edgeR[((size_t)A.W + w) * g.ny + j] = sg;
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) igr_strip_unpack_kernel(Grid g, const double* rec, double* sigma) {
+    int i, j, k;
+    for (InteriorIter it_(g.nx, g.ny, 1); it_.next(i, j, k);) {
+        const int w = i / kWave, l = i - w * kWave;
+        int t = j + l;
+        if (t >= g.ny) t -= g.ny;
+        sigma[cell_index(g, i, j, 0)] = rec[((size_t)w * g.ny + t) * kStripSlot + 6 * l + 5];
+    }
+}
+
+struct IgrStripArgs {
+    int nx, ny;
+    double* rec;
+    double* edgeL;
+    double* edgeR;
+    int* prog;             // [W] schedule rows completed by the strip's last lane; zero at launch
+    int* err;
+    const int* active;
+    int W, P, nsweeps;
+    int perX, perY;
+};
+
+__device__ __forceinline__ double dpp_shr1(double v) {   // lane l <- lane l-1
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double dpp_shl1(double v) {   // lane l <- lane l+1
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double read_lane(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Largest schedule row in [a, b] that is a real row (not a gap row, inside
+// the launch's [0, Rt)), or -1.
+__device__ __forceinline__ int strip_last_row(int a, int b, int P, int ny, int Rt) {
+    b = min(b, Rt - 1);
+    if (b < a || b < 0) return -1;
+    const int jb = b % P;
+    const int r = jb < ny ? b : b - jb + ny - 1;
+    return r >= a && r >= 0 ? r : -1;
+}
+
+__device__ __forceinline__ bool strip_wait(const IgrStripArgs& A, int u, int need) {
+    if (need <= 0) return true;
+    int spins = 0;
+    while (__hip_atomic_load(&A.prog[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+        if (++spins > (1 << 25)) {
+            __hip_atomic_store(A.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(64) igr_strip_kernel(IgrStripArgs A) {
+    if (A.active && *A.active == 0) return;
+    constexpr int U = kStripGroup, C = kStripChunk;
+    const int w = blockIdx.x, lane = threadIdx.x;
+    const int ny = A.ny, P = A.P, W = A.W;
+    const int L = min(kWave - 1, A.nx - 1 - w * kWave);
+    const bool act = lane <= L;
+    const int Rt = A.nsweeps * P;
+    const int nsteps = Rt + kWave - 1;
+    double* rec = A.rec + (size_t)w * ny * kStripSlot + 6 * lane;
+    const bool leftWall = w == 0 && !A.perX, rightWall = w == W - 1 && !A.perX;
+    const int leftSrc = w > 0 ? w - 1 : W - 1, rightSrc = w < W - 1 ? w + 1 : 0;
+    // lane state: row 0 of the current and of the previous sweep, last result
+    double row0 = rec[(size_t)lane * kStripSlot + 5];           // slot of (lane, 0) = lane
+    int tl = ny - 1 + lane;
+    if (tl >= ny) tl -= ny;
+    double last = A.perY ? rec[(size_t)tl * kStripSlot + 5] : row0;
+    double row0old = row0;
+    // schedule position of the lane at the next compute step and at the
+    // next prefetch step: row in the period and sweep
+    int jc = lane == 0 ? 0 : P - lane, qc = lane == 0 ? 0 : -1;
+    int jp = jc;
+    double buf[kStripBufs][6][U];
+    auto prefetch = [&](double (&bf)[6][U]) {
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            int t = jp < ny ? jp + lane : lane;
+            if (t >= ny) t -= ny;
+            const double2* r = reinterpret_cast<const double2*>(rec + (size_t)t * kStripSlot);
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const double2 v = r[f];
+                bf[2 * f][k] = v.x;
+                bf[2 * f + 1][k] = v.y;
+            }
+            if (++jp == P) jp = 0;
+        }
+    };
+    // lane k: lane 0's left value / the last lane's right value of chunk step
+    // k (in), and lane 0's / the last lane's result of chunk step k (out)
+    double lchunk = 0.0, rchunk = 0.0, eL = 0.0, eR = 0.0;
+    auto step = [&](int s, int kk, const double (&cur)[6][U], int k, double sgT) {
+        const int r = s - lane;
+        const bool valid = act && r >= 0 && r < Rt && jc < ny;
+        const double sendR = jc + 1 == P ? row0 : sgT;
+        const double rightS = dpp_shl1(sendR);
+        const double leftS = dpp_shr1(last);
+        const double cellOld = cur[5][k];
+        const double top = jc < ny - 1 ? sgT : (A.perY ? row0old : cellOld);
+        const double bot = jc > 0 || A.perY ? last : row0;
+        const double lv = read_lane(lchunk, kk), rv = read_lane(rchunk, kk);
+        const double right = lane < L ? rightS : (rightWall ? cellOld : rv);
+        const double left = lane > 0 ? leftS : (leftWall ? cellOld : lv);
+        const double base = cur[0][k] + cur[2][k] * right + cur[4][k] * top;
+        const double nw = base + cur[3][k] * bot + cur[1][k] * left;
+        if (valid) {
+            if (jc == 0) {
+                row0old = row0;
+                row0 = nw;
+            }
+            int t = jc + lane;
+            if (t >= ny) t -= ny;
+            rec[(size_t)t * kStripSlot + 5] = nw;
+            last = nw;
+        }
+        const double n0 = read_lane(nw, 0), nL = read_lane(nw, L);
+        if (lane == kk) {
+            eL = n0;
+            eR = nL;
+        }
+        if (++jc == P) {
+            jc = 0;
+            ++qc;
+        }
+    };
+    auto group = [&](int g0, int s0, const double (&cur)[6][U], const double (&nxt)[6][U]) {
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            if (g0 + k < nsteps) step(g0 + k, g0 + k - s0, cur, k, k < U - 1 ? cur[5][k + 1] : nxt[5][0]);
+    };
+    prefetch(buf[0]);
+    prefetch(buf[1]);
+    for (int s0 = 0; s0 < nsteps; s0 += C) {
+        // progress the chunk needs from the neighbouring strips
+        bool ok = true;
+        if (!leftWall) {
+            const int r = strip_last_row(s0, s0 + C - 1, P, ny, Rt);
+            if (r >= 0) ok = strip_wait(A, leftSrc, w > 0 ? r + 1 : r - P + 1);
+        }
+        if (ok && !rightWall) {
+            const int r = strip_last_row(s0 - L, s0 + C - 1 - L, P, ny, Rt);
+            if (r >= 0) ok = strip_wait(A, rightSrc, r - P + 1);
+        }
+        if (!ok) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // every lane loads (index 0 where it has nothing to load): no branch on
+        // the lane index may open the chunk, or the compiler may thread it to
+        // the one that closes it and run the lanes' loops apart
+        {
+            int r = s0 + lane;
+            bool v = lane < C && !leftWall && r < Rt && r % P < ny;
+            int q = v ? r / P : 0, j = v ? r % P : 0;
+            const double a = A.edgeR[v ? ((size_t)((w > 0 ? q : q - 1) & 1) * W + leftSrc) * ny + j : 0];
+            lchunk = v ? a : 0.0;
+            r = s0 + lane - L;
+            v = lane < C && !rightWall && r >= 0 && r < Rt && r % P < ny;
+            q = v ? r / P : 0;
+            j = v ? r % P : 0;
+            const double b = A.edgeL[v ? ((size_t)((q - 1) & 1) * W + rightSrc) * ny + j : 0];
+            rchunk = v ? b : 0.0;
+        }
+        prefetch(buf[2]);
+        group(s0, s0, buf[0], buf[1]);
+        prefetch(buf[0]);
+        group(s0 + U, s0, buf[1], buf[2]);
+        prefetch(buf[1]);
+        group(s0 + 2 * U, s0, buf[2], buf[0]);
+        // this chunk's edge values, then the progress (every lane stores the same value)
+        {
+            int r = s0 + lane;
+            if (lane < C && r < Rt && r % P < ny) A.edgeL[((size_t)((r / P) & 1) * W + w) * ny + r % P] = eL;
+            r = s0 + lane - L;
+            if (lane < C && r >= 0 && r < Rt && r % P < ny)
+                A.edgeR[((size_t)((r / P) & 1) * W + w) * ny + r % P] = eR;
+        }
+        const int done = min(max(min(s0 + C, nsteps) - L, 0), Rt);
+        __threadfence();
+        __hip_atomic_store(&A.prog[w], done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 }  // namespace sifv

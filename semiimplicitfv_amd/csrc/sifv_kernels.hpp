@@ -42,6 +42,14 @@ IgrPreFn<double> igr_pre_kernel_for<double>(int dim, int nph);
 template <>
 IgrPreFn<float> igr_pre_kernel_for<float>(int dim, int nph);
 IgrSweepFn igr_sweep_kernel_for(int dim);
+// 2D strip-streaming sweep: pack (coefficients + sigma into skewed records),
+// the sweeps, unpack (sigma back into the padded array).
+struct IgrStripKernels {
+    void (*pack)(IgrStripPackArgs);
+    void (*sweep)(IgrStripArgs);
+    void (*unpack)(Grid, const double*, double*);
+};
+IgrStripKernels igr_strip_kernels();
 
 // Viscous stress right-hand side per dimension / phase count.
 template <typename T>

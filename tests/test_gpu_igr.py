@@ -91,17 +91,24 @@ def _wavefront_case(dim, n, periods, warm, iters, seed):
     return cfg, mesh, st, eos
 
 
-# Meshes with many sweep tiles (64 columns x up to 16 rows), all sweeps of a
-# solve in one launch: wrapping and wall faces per axis, and (1024^2, and the
-# 3D case) more tiles than co-resident blocks, so one tile's consecutive sweeps
-# run on different blocks.
+# All sweeps of a solve run in one launch.  Tile wavefront (1D, 3D, and 2D
+# meshes with fewer than 160 rows): many tiles of 64 columns x up to 16 rows,
+# wrapping and wall faces per axis, and (3D) more tiles than co-resident
+# blocks, so one tile's consecutive sweeps run on different blocks.  2D strip
+# streaming (one wave per 64-column strip, >= 160 rows): wall and wrapping
+# faces per axis, partial last strips (down to one column), a single strip
+# wrapping onto itself, and 1024^2 with 16 strips.
 WAVEFRONT = {
-    "w2_per_xy": (2, (300, 170), (1, 1, 0), 30, 5, 2),
-    "w2_per_y": (2, (200, 133), (0, 1, 0), 25, 4, 2),
+    "w2_per_y_tiles": (2, (200, 133), (0, 1, 0), 25, 4, 2),
     "w1_per_x": (1, (3000,), (1, 0, 0), 60, 5, 2),
     "w3_per_xyz": (3, (70, 40, 12), (1, 1, 1), 20, 3, 2),
     "w3_walls_many_tiles": (3, (130, 64, 48), (0, 0, 0), 12, 3, 1),
-    "w2_per_xy_1024": (2, (1024, 1024), (1, 1, 0), 12, 3, 1),
+    "s2_per_xy": (2, (300, 170), (1, 1, 0), 30, 5, 2),
+    "s2_walls": (2, (200, 170), (0, 0, 0), 20, 4, 2),
+    "s2_per_y": (2, (130, 200), (0, 1, 0), 15, 3, 2),
+    "s2_per_x_one_column_strip": (2, (257, 300), (1, 0, 0), 12, 3, 2),
+    "s2_per_x_single_strip": (2, (50, 180), (1, 0, 0), 15, 3, 2),
+    "s2_per_xy_1024": (2, (1024, 1024), (1, 1, 0), 12, 3, 1),
 }
 
 
