@@ -1401,14 +1401,15 @@ int setup_igr(sifv_handle* h) {
     if (h->dim == 2 && h->ny >= 2 * kStripChunk + kWave + kStripChunk + 16 && wrapPair) {
         int perCuS = 0;
         HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &perCuS, reinterpret_cast<const void*>(igr_strip_kernels().sweep), kWave, 0));
+            &perCuS, reinterpret_cast<const void*>(igr_strip_kernels().sweep), 2 * kWave, 0));
         if (W <= perCuS * cus) {
             const bool perX = h->sbc[0] == BK_WRAP;
             const int gap = perX ? std::max(0, W * (kWave + kStripChunk) + kWave + kStripChunk - h->ny) : 0;
             h->igrStrips = W;
             h->igrPeriod = h->ny + gap;
             HIPCHK(hipMalloc(&h->igrRec, sizeof(double) * (size_t)W * h->ny * kStripSlot));
-            HIPCHK(hipMalloc(&h->igrEdge, sizeof(double) * 4 * (size_t)W * h->ny));
+            // edge values [2][W][ny] per side, then a sink for lanes without a value
+            HIPCHK(hipMalloc(&h->igrEdge, sizeof(double) * (4 * (size_t)W * h->ny + kWave)));
             HIPCHK(hipMalloc(&h->igrProg, sizeof(int) * W));
         }
     }
@@ -1484,7 +1485,7 @@ int run_igr(sifv_handle* h, int src, const int* active) {
         SA.perY = h->sbc[2] == BK_WRAP;
         HIPCHK(hipMemsetAsync(h->igrProg, 0, sizeof(int) * h->igrStrips, h->stream));
         void* sargs2[] = {&SA};
-        rc = timed_launch(h, reinterpret_cast<const void*>(K.sweep), dim3(h->igrStrips), dim3(kWave), sargs2);
+        rc = timed_launch(h, reinterpret_cast<const void*>(K.sweep), dim3(h->igrStrips), dim3(2 * kWave), sargs2);
         if (rc) return rc;
         Grid g = h->grid();
         const double* recc = h->igrRec;

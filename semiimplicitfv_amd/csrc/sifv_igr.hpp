@@ -492,33 +492,136 @@ __device__ __forceinline__ int strip_last_row(int a, int b, int P, int ny, int R
     return r >= a && r >= 0 ? r : -1;
 }
 
-__device__ __forceinline__ bool strip_wait(const IgrStripArgs& A, int u, int need) {
-    if (need <= 0) return true;
-    int spins = 0;
-    while (__hip_atomic_load(&A.prog[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-        if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
-        if (++spins > (1 << 25)) {
-            __hip_atomic_store(A.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(2);
-    }
-    return true;
-}
+// Two waves per strip.  Wave 0 computes; wave 1 carries the strip's traffic
+// with its neighbours through LDS: it publishes each chunk wave 0 finishes
+// (edge values to the edge arrays, release, progress flag) and loads the
+// neighbours' edge values of upcoming chunks as soon as their progress
+// allows (acquire), up to kStripRing chunks ahead.  Wave 0 thus never waits
+// on a device-scope fence, which would drain its prefetch pipeline.  Wave 1
+// never blocks on one duty: each pass of its loop publishes what it can and
+// loads what it can.  Neither wave branches on its lane index at a loop
+// boundary (stores of lanes without a value go to a sink element).
+constexpr int kStripRing = 8;
 
 template <int DIM>
-__global__ void __launch_bounds__(64) igr_strip_kernel(IgrStripArgs A) {
+__global__ void __launch_bounds__(128) igr_strip_kernel(IgrStripArgs A) {
     if (A.active && *A.active == 0) return;
-    constexpr int U = kStripGroup, C = kStripChunk;
-    const int w = blockIdx.x, lane = threadIdx.x;
+    constexpr int U = kStripGroup, C = kStripChunk, NR = kStripRing;
+    __shared__ double sIn[NR][2][kWave];    // neighbours' values for wave 0: [slot][left/right][chunk step]
+    __shared__ double sOut[NR][C][kWave];   // wave 0's results: [slot][chunk step][lane]
+    __shared__ int sFlag[4];                // chunks done by wave 0, loaded for it, published; abort
+    const int w = blockIdx.x, lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);   // wave-uniform
     const int ny = A.ny, P = A.P, W = A.W;
     const int L = min(kWave - 1, A.nx - 1 - w * kWave);
-    const bool act = lane <= L;
     const int Rt = A.nsweeps * P;
     const int nsteps = Rt + kWave - 1;
-    double* rec = A.rec + (size_t)w * ny * kStripSlot + 6 * lane;
+    const int nchunks = (nsteps + C - 1) / C;
     const bool leftWall = w == 0 && !A.perX, rightWall = w == W - 1 && !A.perX;
     const int leftSrc = w > 0 ? w - 1 : W - 1, rightSrc = w < W - 1 ? w + 1 : 0;
+    if (threadIdx.x < 4) sFlag[threadIdx.x] = 0;
+    __syncthreads();
+    auto lds_load = [](int* p) __attribute__((always_inline)) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP); };
+    auto lds_store = [](int* p, int v) __attribute__((always_inline)) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); };
+    // wave 1 watches the device error flag and raises the block's abort flag,
+    // which wave 0 polls in LDS (a device-scope load would make wave 0 wait
+    // for its whole prefetch pipeline)
+    auto aborted = [&]() __attribute__((always_inline)) { return __hip_atomic_load(&sFlag[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0; };
+    if (wave == 1) {
+        double* sink = A.edgeL + 4 * (size_t)W * ny;
+        int nextPub = 0, nextIn = 0, knownL = 0, knownR = 0, idle = 0;
+        while (nextPub < nchunks) {
+            bool moved = false;
+            // publish every chunk wave 0 has finished: edge values, then one
+            // release and the progress of the last of them
+            const int aDone = lds_load(&sFlag[0]);
+            if (aDone > nextPub) {
+                for (int c = nextPub; c < aDone; ++c) {
+                    const int s0 = c * C, slot = c % NR;
+                    int r = s0 + lane;
+                    bool v = lane < C && r < Rt && r % P < ny;
+                    double* dl = v ? A.edgeL + ((size_t)((r / P) & 1) * W + w) * ny + r % P : sink;
+                    r = s0 + lane - L;
+                    v = lane < C && r >= 0 && r < Rt && r % P < ny;
+                    double* dr = v ? A.edgeR + ((size_t)((r / P) & 1) * W + w) * ny + r % P : sink;
+                    *dl = sOut[slot][min(lane, C - 1)][0];
+                    *dr = sOut[slot][min(lane, C - 1)][L];
+                }
+                const int done = min(max(min(aDone * C, nsteps) - L, 0), Rt);
+                __threadfence();
+                __hip_atomic_store(&A.prog[w], done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                lds_store(&sFlag[2], aDone);
+                nextPub = aDone;
+                moved = true;
+            }
+            // load the inputs of every chunk whose slot is free and whose rows
+            // the neighbours have published: one acquire for all of them
+            const int upto = min(nchunks, aDone + NR);
+            if (nextIn < upto) {
+                int nlL = knownL, nlR = knownR;
+                auto need = [&](int c, int& nL, int& nR) __attribute__((always_inline)) {
+                    const int s0 = c * C;
+                    nL = 0;
+                    nR = 0;
+                    if (!leftWall) {
+                        const int r = strip_last_row(s0, s0 + C - 1, P, ny, Rt);
+                        if (r >= 0) nL = w > 0 ? r + 1 : r - P + 1;
+                    }
+                    if (!rightWall) {
+                        const int r = strip_last_row(s0 - L, s0 + C - 1 - L, P, ny, Rt);
+                        if (r >= 0) nR = r - P + 1;
+                    }
+                };
+                int nL, nR;
+                need(nextIn, nL, nR);
+                if (nL > nlL) nlL = __hip_atomic_load(&A.prog[leftSrc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (nR > nlR) nlR = __hip_atomic_load(&A.prog[rightSrc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                knownL = nlL;
+                knownR = nlR;
+                int last = nextIn;
+                while (last < upto) {
+                    need(last, nL, nR);
+                    if (nL > knownL || nR > knownR) break;
+                    ++last;
+                }
+                if (last > nextIn) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    for (int c = nextIn; c < last; ++c) {
+                        const int s0 = c * C, slot = c % NR;
+                        int r = s0 + lane;
+                        bool v = lane < C && !leftWall && r < Rt && r % P < ny;
+                        int q = v ? r / P : 0, j = v ? r % P : 0;
+                        const double a = A.edgeR[v ? ((size_t)((w > 0 ? q : q - 1) & 1) * W + leftSrc) * ny + j : 0];
+                        r = s0 + lane - L;
+                        const bool vr = lane < C && !rightWall && r >= 0 && r < Rt && r % P < ny;
+                        q = vr ? r / P : 0;
+                        j = vr ? r % P : 0;
+                        const double b = A.edgeL[vr ? ((size_t)((q - 1) & 1) * W + rightSrc) * ny + j : 0];
+                        sIn[slot][0][lane] = v ? a : 0.0;
+                        sIn[slot][1][lane] = vr ? b : 0.0;
+                    }
+                    lds_store(&sFlag[1], last);
+                    nextIn = last;
+                    moved = true;
+                }
+            }
+            if (moved) {
+                idle = 0;
+            } else {
+                const bool budget = ++idle > (1 << 25);
+                if (budget) __hip_atomic_store(A.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (budget || __hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || aborted()) {
+                    __hip_atomic_store(&sFlag[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        return;
+    }
+    // wave 0: the sweeps
+    const bool act = lane <= L;
+    double* rec = A.rec + (size_t)w * ny * kStripSlot + 6 * lane;
     // lane state: row 0 of the current and of the previous sweep, last result
     double row0 = rec[(size_t)lane * kStripSlot + 5];           // slot of (lane, 0) = lane
     int tl = ny - 1 + lane;
@@ -527,10 +630,10 @@ __global__ void __launch_bounds__(64) igr_strip_kernel(IgrStripArgs A) {
     double row0old = row0;
     // schedule position of the lane at the next compute step and at the
     // next prefetch step: row in the period and sweep
-    int jc = lane == 0 ? 0 : P - lane, qc = lane == 0 ? 0 : -1;
+    int jc = lane == 0 ? 0 : P - lane;
     int jp = jc;
     double buf[kStripBufs][6][U];
-    auto prefetch = [&](double (&bf)[6][U]) {
+    auto prefetch = [&](double (&bf)[6][U]) __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < U; ++k) {
             int t = jp < ny ? jp + lane : lane;
@@ -545,96 +648,102 @@ __global__ void __launch_bounds__(64) igr_strip_kernel(IgrStripArgs A) {
             if (++jp == P) jp = 0;
         }
     };
-    // lane k: lane 0's left value / the last lane's right value of chunk step
-    // k (in), and lane 0's / the last lane's result of chunk step k (out)
-    double lchunk = 0.0, rchunk = 0.0, eL = 0.0, eR = 0.0;
-    auto step = [&](int s, int kk, const double (&cur)[6][U], int k, double sgT) {
-        const int r = s - lane;
-        const bool valid = act && r >= 0 && r < Rt && jc < ny;
-        const double sendR = jc + 1 == P ? row0 : sgT;
-        const double rightS = dpp_shl1(sendR);
-        const double leftS = dpp_shr1(last);
-        const double cellOld = cur[5][k];
-        const double top = jc < ny - 1 ? sgT : (A.perY ? row0old : cellOld);
-        const double bot = jc > 0 || A.perY ? last : row0;
-        const double lv = read_lane(lchunk, kk), rv = read_lane(rchunk, kk);
-        const double right = lane < L ? rightS : (rightWall ? cellOld : rv);
-        const double left = lane > 0 ? leftS : (leftWall ? cellOld : lv);
-        const double base = cur[0][k] + cur[2][k] * right + cur[4][k] * top;
-        const double nw = base + cur[3][k] * bot + cur[1][k] * left;
-        if (valid) {
-            if (jc == 0) {
-                row0old = row0;
-                row0 = nw;
+    // lane k holds lane 0's left value and the last lane's right value of
+    // chunk step k (read once per chunk from LDS, picked per step by readlane:
+    // an LDS read per step would wait behind the previous step's LDS write)
+    double lchunk = 0.0, rchunk = 0.0;
+    // a group in which every lane's row is an interior row (1 .. ny-2) of one
+    // sweep: all lanes update, none meets a boundary row, and all lanes write
+    // the same record slot (s - qP mod ny); lanes past the mesh edge write
+    // their padding records
+    auto group = [&](int g0, int s0, int slot, const double (&cur)[6][U], const double (&nxt)[6][U]) __attribute__((always_inline)) {
+        const int lo = g0 - (kWave - 1), hi = g0 + U - 1;
+        const int q = lo >= 0 ? lo / P : -1;
+        if (lo >= 0 && hi < Rt && lo - q * P >= 1 && hi - q * P <= ny - 2) {
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                const int kk = g0 + k - s0;
+                const double sgT = k < U - 1 ? cur[5][k + 1] : nxt[5][0];
+                const double cellOld = cur[5][k];
+                const double rightS = dpp_shl1(sgT);
+                const double leftS = dpp_shr1(last);
+                const double lv = read_lane(lchunk, kk), rv = read_lane(rchunk, kk);
+                const double right = lane < L ? rightS : (rightWall ? cellOld : rv);
+                const double left = lane > 0 ? leftS : (leftWall ? cellOld : lv);
+                const double base = cur[0][k] + cur[2][k] * right + cur[4][k] * sgT;
+                const double nw = base + cur[3][k] * last + cur[1][k] * left;
+                int t = g0 + k - q * P;
+                if (t >= ny) t -= ny;
+                rec[(size_t)t * kStripSlot + 5] = nw;
+                sOut[slot][kk][lane] = nw;
+                last = nw;
             }
-            int t = jc + lane;
-            if (t >= ny) t -= ny;
-            rec[(size_t)t * kStripSlot + 5] = nw;
-            last = nw;
-        }
-        const double n0 = read_lane(nw, 0), nL = read_lane(nw, L);
-        if (lane == kk) {
-            eL = n0;
-            eR = nL;
-        }
-        if (++jc == P) {
-            jc = 0;
-            ++qc;
+            jc += U;
+        } else {
+            // general steps: any row position (first or last row of a sweep,
+            // gap rows, rows outside the launch)
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                const int s = g0 + k, kk = s - s0;
+                if (s >= nsteps) break;
+                const double sgT = k < U - 1 ? cur[5][k + 1] : nxt[5][0];
+                const int r = s - lane;
+                const bool valid = act && r >= 0 && r < Rt && jc < ny;
+                const double sendR = jc + 1 == P ? row0 : sgT;
+                const double rightS = dpp_shl1(sendR);
+                const double leftS = dpp_shr1(last);
+                const double cellOld = cur[5][k];
+                const double top = jc < ny - 1 ? sgT : (A.perY ? row0old : cellOld);
+                const double bot = jc > 0 || A.perY ? last : row0;
+                const double lv = read_lane(lchunk, kk), rv = read_lane(rchunk, kk);
+                const double right = lane < L ? rightS : (rightWall ? cellOld : rv);
+                const double left = lane > 0 ? leftS : (leftWall ? cellOld : lv);
+                const double base = cur[0][k] + cur[2][k] * right + cur[4][k] * top;
+                const double nw = base + cur[3][k] * bot + cur[1][k] * left;
+                if (valid) {
+                    if (jc == 0) {
+                        row0old = row0;
+                        row0 = nw;
+                    }
+                    int t = jc + lane;
+                    if (t >= ny) t -= ny;
+                    rec[(size_t)t * kStripSlot + 5] = nw;
+                    last = nw;
+                }
+                sOut[slot][kk][lane] = nw;
+                if (++jc == P) jc = 0;
+            }
         }
     };
-    auto group = [&](int g0, int s0, const double (&cur)[6][U], const double (&nxt)[6][U]) {
-#pragma unroll
-        for (int k = 0; k < U; ++k)
-            if (g0 + k < nsteps) step(g0 + k, g0 + k - s0, cur, k, k < U - 1 ? cur[5][k + 1] : nxt[5][0]);
+    auto wait_flag = [&](int* f, int v) __attribute__((always_inline)) {
+        int spins = 0;
+        while (lds_load(f) < v) {
+            if (aborted()) return false;
+            if (++spins > (1 << 27)) {
+                __hip_atomic_store(A.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&sFlag[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        return true;
     };
     prefetch(buf[0]);
     prefetch(buf[1]);
-    for (int s0 = 0; s0 < nsteps; s0 += C) {
-        // progress the chunk needs from the neighbouring strips
-        bool ok = true;
-        if (!leftWall) {
-            const int r = strip_last_row(s0, s0 + C - 1, P, ny, Rt);
-            if (r >= 0) ok = strip_wait(A, leftSrc, w > 0 ? r + 1 : r - P + 1);
-        }
-        if (ok && !rightWall) {
-            const int r = strip_last_row(s0 - L, s0 + C - 1 - L, P, ny, Rt);
-            if (r >= 0) ok = strip_wait(A, rightSrc, r - P + 1);
-        }
-        if (!ok) return;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        // every lane loads (index 0 where it has nothing to load): no branch on
-        // the lane index may open the chunk, or the compiler may thread it to
-        // the one that closes it and run the lanes' loops apart
-        {
-            int r = s0 + lane;
-            bool v = lane < C && !leftWall && r < Rt && r % P < ny;
-            int q = v ? r / P : 0, j = v ? r % P : 0;
-            const double a = A.edgeR[v ? ((size_t)((w > 0 ? q : q - 1) & 1) * W + leftSrc) * ny + j : 0];
-            lchunk = v ? a : 0.0;
-            r = s0 + lane - L;
-            v = lane < C && !rightWall && r >= 0 && r < Rt && r % P < ny;
-            q = v ? r / P : 0;
-            j = v ? r % P : 0;
-            const double b = A.edgeL[v ? ((size_t)((q - 1) & 1) * W + rightSrc) * ny + j : 0];
-            rchunk = v ? b : 0.0;
-        }
+    for (int c = 0; c < nchunks; ++c) {
+        const int s0 = c * C, slot = c % NR;
+        // inputs of the chunk loaded, and the output slot published by wave 1
+        if (!wait_flag(&sFlag[1], c + 1)) return;
+        if (!wait_flag(&sFlag[2], c + 1 - NR)) return;
+        lchunk = sIn[slot][0][lane];
+        rchunk = sIn[slot][1][lane];
         prefetch(buf[2]);
-        group(s0, s0, buf[0], buf[1]);
+        group(s0, s0, slot, buf[0], buf[1]);
         prefetch(buf[0]);
-        group(s0 + U, s0, buf[1], buf[2]);
+        group(s0 + U, s0, slot, buf[1], buf[2]);
         prefetch(buf[1]);
-        group(s0 + 2 * U, s0, buf[2], buf[0]);
-        // this chunk's edge values, then the progress (every lane stores the same value)
-        {
-            int r = s0 + lane;
-            if (lane < C && r < Rt && r % P < ny) A.edgeL[((size_t)((r / P) & 1) * W + w) * ny + r % P] = eL;
-            r = s0 + lane - L;
-            if (lane < C && r >= 0 && r < Rt && r % P < ny)
-                A.edgeR[((size_t)((r / P) & 1) * W + w) * ny + r % P] = eR;
-        }
-        const int done = min(max(min(s0 + C, nsteps) - L, 0), Rt);
-        __threadfence();
-        __hip_atomic_store(&A.prog[w], done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        group(s0 + 2 * U, s0, slot, buf[2], buf[0]);
+        lds_store(&sFlag[0], c + 1);
     }
 }
 
