@@ -51,6 +51,9 @@ def parse():
                          "two-phase stage kernels and the pressure solvers)")
     ap.add_argument("--bubble-solver", default="jacobi", choices=["jacobi", "pcg", "mgpcg"])
     ap.add_argument("--bubble-stirred", action="store_true", help="the stirred bubble (stiff pressure solves)")
+    ap.add_argument("--igr-only", action="store_true",
+                    help="only run the IGR side line (2D vortex, periodic and outflow faces, 1024^2 and 2048^2) and "
+                         "print it (profiling the IGR kernels)")
     ap.add_argument("--rows", type=int, default=0, help="rows marched per wave (0 = built-in choice)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration) and exit")
@@ -159,14 +162,17 @@ def bench_igr(sizes=(1024, 2048), steps: int = 20):
     """2D vortex with IGR (LF + WENO5 + SSP-RK3, IGRIters 5 exact
     Gauss-Seidel sweeps per stage, alphaCoeff 1) against the same run
     without IGR: wall time per step after a warm-up step (whose 50 warm-start
-    sweeps are excluded)."""
+    sweeps are excluded).  Rows: the periodic vortex ("vortex{n}") and the same
+    state between outflow faces ("outflow{n}"); a periodic x axis makes the
+    strip-streaming sweep wait once per sweep for the strips' skew around the
+    axis."""
     from semiimplicitfv_amd import ExplicitSolver, cases
 
     out = {}
-    for n in sizes:
+    for n, periodic in [(n, p) for n in sizes for p in (True, False)]:
         row = {}
         for igr in (False, True):
-            cfg, mesh, st, eos, rs = cases.isentropic_vortex_2d(n, riemann="lf")
+            cfg, mesh, st, eos, rs = cases.isentropic_vortex_2d(n, riemann="lf", periodic=periodic)
             if igr:
                 cfg.useIGR = True
                 cfg.igrParams.alphaCoeff = 1.0
@@ -187,7 +193,7 @@ def bench_igr(sizes=(1024, 2048), steps: int = 20):
         row["igr_sweeps_per_step"] = 15
         row["ms_per_igr_sweep"] = (row["igr_ms_per_step"] - row["plain_ms_per_step"]) / 15
         row["cell_sweeps_per_s"] = n * n / (row["ms_per_igr_sweep"] * 1e-3)
-        out[f"vortex{n}"] = row
+        out[f"{'vortex' if periodic else 'outflow'}{n}"] = row
     return out
 
 
@@ -529,6 +535,9 @@ def main():
         r = bench_bubble(a.bubble_n, a.steps, stirred=a.bubble_stirred, solver=a.bubble_solver)
         r.pop("_pres", None)
         print(json.dumps(r))
+        return
+    if a.igr_only:
+        print(json.dumps(bench_igr(steps=min(a.steps, 20))))
         return
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",

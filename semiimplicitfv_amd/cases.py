@@ -56,15 +56,17 @@ def sod_1d(n: int = 1024, riemann: str = "hllc", recon=ReconstructionOrder.WENO5
 
 
 def isentropic_vortex_2d(n: int = 1024, riemann: str = "hllc", recon=ReconstructionOrder.WENO5, cfl: float = 0.6,
-                         beta: float = 5.0, L: float = 10.0, center=(5.0, 5.0), gamma: float = 1.4):
+                         beta: float = 5.0, L: float = 10.0, center=(5.0, 5.0), gamma: float = 1.4,
+                         periodic: bool = True):
     """Periodic isentropic vortex on [0,L]^2 with freestream rho=p=u=v=1:
     T = 1 - (g-1) beta^2/(8 g pi^2) e^{1-r^2}, rho = T^{1/(g-1)},
-    p = T^{g/(g-1)}, (u,v) = (1,1) + beta/(2 pi) e^{(1-r^2)/2} (-dy, dx)."""
+    p = T^{g/(g-1)}, (u,v) = (1,1) + beta/(2 pi) e^{(1-r^2)/2} (-dy, dx).
+    periodic=False gives the same initial state between outflow faces."""
     cfg = SimulationConfig(dim=2, nGhost=3, RKOrder=3, reconOrder=recon)
     cfg.explicitParams.cfl = cfl
     cfg.explicitParams.maxDt = 10.0
     cfg.validate()
-    mesh = uniform_mesh(cfg, n, 0.0, L, n, 0.0, L, periods=(1, 1, 0))
+    mesh = uniform_mesh(cfg, n, 0.0, L, n, 0.0, L, periods=(1, 1, 0) if periodic else (0, 0, 0))
     st = SolutionState()
     st.allocate(mesh.totalCells(), cfg)
     eos = IdealGasEOS(gamma, 1.0, cfg)

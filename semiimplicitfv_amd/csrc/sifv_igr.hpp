@@ -411,28 +411,32 @@ struct IgrStripPackArgs {
     int W;
 };
 
+// Threads run in record order (strip, slot, lane): each wave writes one
+// contiguous 3 KB slot, and reads the slot's diagonal of cells (row = slot -
+// lane), whose lines the neighbouring slots share in L2.
 template <int DIM>
 __global__ void __launch_bounds__(256) igr_strip_pack_kernel(IgrStripPackArgs A) {
     const Grid& g = A.g;
     const size_t sy = (size_t)g.nxT;
-    int i, j, k;
-    for (InteriorIter it_(g.nx, g.ny, 1); it_.next(i, j, k);) {
+    const size_t total = (size_t)A.W * g.ny * kWave;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int l = (int)(e % kWave);
+        const size_t ws = e / kWave;
+        const int t = (int)(ws % g.ny), w = (int)(ws / g.ny);
+        const int i = w * kWave + l;
+        if (i >= g.nx) continue;
+        int j = t - l;
+        if (j < 0) j += g.ny;
         const size_t id = cell_index(g, i, j, 0);
         const double ri = A.rho[id];
         const double rd = 1.0 / A.diag[id];
         const double hx = g.dxw[i + g.ngx], hy = g.dyw[j + g.ngy];
         const double ax = A.alpha / (hx * hx), ay = A.alpha / (hy * hy);
-        const int w = i / kWave, l = i - w * kWave;
-        int t = j + l;
-        if (t >= g.ny) t -= g.ny;
-        double* r = A.rec + ((size_t)w * g.ny + t) * kStripSlot + 6 * l;
-        r[0] = A.rhs[id] * rd;
-        r[1] = ax / (0.5 * (ri + A.rho[id - 1])) * rd;
-        r[2] = ax / (0.5 * (ri + A.rho[id + 1])) * rd;
-        r[3] = ay / (0.5 * (ri + A.rho[id - sy])) * rd;
-        r[4] = ay / (0.5 * (ri + A.rho[id + sy])) * rd;
+        double2* r = reinterpret_cast<double2*>(A.rec + ws * kStripSlot + 6 * l);
         const double sg = A.sigma[id];
-        r[5] = sg;
+        r[0] = make_double2(A.rhs[id] * rd, ax / (0.5 * (ri + A.rho[id - 1])) * rd);
+        r[1] = make_double2(ax / (0.5 * (ri + A.rho[id + 1])) * rd, ay / (0.5 * (ri + A.rho[id - sy])) * rd);
+        r[2] = make_double2(ay / (0.5 * (ri + A.rho[id + sy])) * rd, sg);
         // edge values of "sweep -1" (parity slot 1): the initial iterate
         const int L = min(kWave - 1, g.nx - 1 - w * kWave);
         if (l == 0) A.edgeL[((size_t)A.W + w) * g.ny + j] = sg;
