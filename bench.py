@@ -51,6 +51,8 @@ def parse():
                          "two-phase stage kernels and the pressure solvers)")
     ap.add_argument("--bubble-solver", default="jacobi", choices=["jacobi", "pcg", "mgpcg"])
     ap.add_argument("--bubble-stirred", action="store_true", help="the stirred bubble (stiff pressure solves)")
+    ap.add_argument("--bubble-graphs", action="store_true",
+                    help="replay the Jacobi sweep chunks from captured HIP graphs")
     ap.add_argument("--igr-only", action="store_true",
                     help="only run the IGR side line (2D vortex, periodic and outflow faces, 1024^2 and 2048^2) and "
                          "print it (profiling the IGR kernels)")
@@ -335,7 +337,7 @@ def bench_adjoint(n: int = 512, nsteps: int = 400, every: int = 20, ndir: int = 
 
 
 def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred: bool = False,
-                 solver: str = "jacobi"):
+                 solver: str = "jacobi", graphs: bool = False):
     """3D water/air bubble (unit cube, n^3 cells, air sphere r = 0.2, alpha
     tanh over 2 cells, alphaMin 1e-8, slip walls, p = 1e5 at rest), semi-
     implicit SSP-RK3 with the Jacobi pressure solve (tol 1e-6, <= 1000
@@ -348,7 +350,9 @@ def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred:
     1000-sweep cap.  solver "pcg": the conjugate-gradient pressure solve
     (stop on ||rhs - A p||_2 <= 1e-6 ||rhs||_2, <= 1000 iterations) instead of
     Jacobi (stop on max |p_new - p| < 1e-6); solver "mgpcg": the same with the
-    multigrid V-cycle preconditioner."""
+    multigrid V-cycle preconditioner.  graphs: Jacobi chunks replayed from
+    captured HIP graphs (captured in the warm-up step) instead of enqueued
+    sweep by sweep."""
     import numpy as np
 
     from semiimplicitfv_amd import (BoundaryCondition, HLLCSolver, JacobiPressureSolver, MGPCGPressureSolver,
@@ -372,6 +376,7 @@ def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred:
     t0 = time.perf_counter()
     ps = {"pcg": PCGPressureSolver, "mgpcg": MGPCGPressureSolver}.get(solver, JacobiPressureSolver)()
     g = SemiImplicitSolver(mesh, HLLCSolver(eos), ps, eos, None, cfg, precision=precision)
+    g.set_graphs(graphs)
     xc, yc, zc = mesh.centroids()
     r = np.sqrt((xc[None, None, :] - 0.5) ** 2 + (yc[None, :, None] - 0.5) ** 2 + (zc[:, None, None] - 0.5) ** 2)
     amin = cfg.multiPhaseParams.alphaMin
@@ -532,7 +537,8 @@ def main():
         copy_probe(a.copy_probe)
         return
     if a.bubble_only:
-        r = bench_bubble(a.bubble_n, a.steps, stirred=a.bubble_stirred, solver=a.bubble_solver)
+        r = bench_bubble(a.bubble_n, a.steps, stirred=a.bubble_stirred, solver=a.bubble_solver,
+                         graphs=a.bubble_graphs)
         r.pop("_pres", None)
         print(json.dumps(r))
         return

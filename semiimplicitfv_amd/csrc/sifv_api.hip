@@ -174,6 +174,16 @@ struct sifv_handle {
     // launches, which the device-resident time loop already keeps back to back);
     // rebuilt when a launch argument changes
     bool useGraphs = false;
+    // captured chunks of Jacobi sweeps (useGraphs): one per chunk of the
+    // sweep schedule and density buffer, replayed while the arguments repeat
+    struct JacGraph {
+        int start, end;
+        const double* rho;
+        const double* dt;
+        double tol;
+        hipGraphExec_t exec;
+    };
+    std::vector<JacGraph> jacGraphs;
     hipGraph_t stepGraph = nullptr;
     hipGraphExec_t stepExec = nullptr;
     double graphEnd = 0.0;
@@ -787,6 +797,11 @@ int elementwise_blocks(const sifv_handle* h) {
     return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192));
 }
 
+void drop_jac_graphs(sifv_handle* h) {
+    for (auto& g : h->jacGraphs) hipGraphExecDestroy(g.exec);
+    h->jacGraphs.clear();
+}
+
 int ensure_jacobi(sifv_handle* h, int maxIter) {
     for (int q = 0; q < 6; ++q) {
         if (h->jac[q]) continue;
@@ -795,6 +810,7 @@ int ensure_jacobi(sifv_handle* h, int maxIter) {
     }
     if (maxIter > h->residCap) {
         HIPCHK(hipStreamSynchronize(h->stream));
+        drop_jac_graphs(h);
         hipFree(h->resid);
         if (h->hostResid) hipHostFree(h->hostResid);
         h->resid = nullptr;
@@ -852,8 +868,37 @@ int run_jacobi(sifv_handle* h, const double* rho, const double* dt, double tol, 
     const int nb = elementwise_blocks(h);
     int it = 0, seen = 0, chunk = 4;
     *n = maxIter;
+    // with graphs on, each chunk of tiled sweeps is captured once per density
+    // buffer and replayed (a converged sweep's successors return at once)
+    const bool graphs = tiled && h->useGraphs && !h->timing.on && !h->noalloc;
     while (it < maxIter) {
         const int end = std::min(maxIter, it + chunk);
+        if (graphs) {
+            hipGraphExec_t exec = nullptr;
+            for (auto& g : h->jacGraphs)
+                if (g.start == it && g.end == end && g.rho == rho && g.dt == dt && g.tol == tol) exec = g.exec;
+            if (!exec) {
+                HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+                hipError_t e = hipSuccess;
+                for (int k = it; k < end && e == hipSuccess; ++k) {
+                    Jt.iter = k;
+                    Jt.pin = h->jac[k % 2];
+                    Jt.pout = h->jac[(k + 1) % 2];
+                    void* cargs[] = {&Jt};
+                    e = hipLaunchKernel(reinterpret_cast<const void*>(jacobi_tile_kernel<double, TILE_SWEEP64>),
+                                        jacobi_tile_grid(h), dim3(64, kTileRows), cargs, 0, h->stream);
+                }
+                hipGraph_t gr = nullptr;
+                const hipError_t ec = hipStreamEndCapture(h->stream, &gr);
+                if (e == hipSuccess) e = ec;
+                if (e == hipSuccess) e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+                if (gr) hipGraphDestroy(gr);
+                HIPCHK(e);
+                h->jacGraphs.push_back({it, end, rho, dt, tol, exec});
+            }
+            HIPCHK(hipGraphLaunch(exec, h->stream));
+            it = end;
+        }
         for (; it < end; ++it) {
             int rc;
             if (tiled) {
@@ -1835,6 +1880,7 @@ int sifv_destroy(sifv_handle* h) {
     if (!h) return SIFV_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
     drop_step_graph(h);
+    drop_jac_graphs(h);
     for (auto& pr : h->timing.pending) {
         hipEventDestroy(pr.first);
         hipEventDestroy(pr.second);
@@ -2252,6 +2298,7 @@ int sifv_set_graphs(sifv_handle* h, int on) {
     if (!h->useGraphs) {
         HIPCHK(hipStreamSynchronize(h->stream));
         drop_step_graph(h);
+        drop_jac_graphs(h);
     }
     return SIFV_OK;
 }
