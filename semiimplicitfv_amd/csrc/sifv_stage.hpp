@@ -760,6 +760,31 @@ stage_xy_kernel(StageParams<T> P) {
     const double cyU = (dxc * dzk) / ((dxc * dyU) * dzk);
 
     using C = CellT<T, NPH, SEMI>;
+    // interior fast path: when every column the wave reads (its 64 lanes and
+    // the E halo columns either side) lies inside the mesh, a load of an
+    // interior row needs no ghost mapping, clamp or velocity flip
+    const bool xin = X0 - 1 - E >= 0 && X0 + kWave - 1 + E < g.nx;
+    auto loadRaw = [&](int c, int row, RawCell<DIM, NPH, T>& rc) __attribute__((always_inline)) {
+        if (xin && row >= 0 && row < g.ny) {
+            const size_t id = cell_index(g, c, row, kz);
+#pragma unroll
+            for (int q = 0; q < DIM + 2 + 2 * NPH; ++q) rc.U[q] = P.src[(size_t)q * P.total + id];
+            rc.flip = 0;
+        } else {
+            load_raw<DIM, NPH, T>(P, P.src, c, row, kz, rc);
+        }
+    };
+    auto loadCell = [&](int c, int row, Cell<DIM, NPH, C>& out) __attribute__((always_inline)) {
+        if (xin && row >= 0 && row < g.ny) {
+            const size_t id = cell_index(g, c, row, kz);
+            T U[DIM + 2 + 2 * NPH];
+#pragma unroll
+            for (int q = 0; q < DIM + 2 + 2 * NPH; ++q) U[q] = P.src[(size_t)q * P.total + id];
+            convert_cell<DIM, NPH, T, C>(P.eos, P.mix, U, out);
+        } else {
+            load_cell<DIM, NPH>(P, P.src, c, row, kz, out);
+        }
+    };
     Cell<DIM, NPH, C> win[WS];
     T sigwin[WS];
     FaceState<DIM, NPH, T> plusPrev;
@@ -783,7 +808,7 @@ stage_xy_kernel(StageParams<T> P) {
     constexpr bool PF = MarchTuning<T>::kPrefetchRow;
     constexpr bool PU = MarchTuning<T>::kPreloadUpdate && MODE == MODE_FULL;
     RawCell<DIM, NPH, T> nextRow;
-    if (YDIR && PF) load_raw<DIM, NPH, T>(P, P.src, col, rStart + R, kz, nextRow);
+    if (YDIR && PF) loadRaw(col, rStart + R, nextRow);
     T preU[DIM + 2 + 2 * NPH], preU0[DIM + 2 + 2 * NPH];
 
     for (int r = rStart; r < Y1; ++r) {
@@ -801,9 +826,9 @@ stage_xy_kernel(StageParams<T> P) {
             }
             if constexpr (PF) {
                 raw_to_cell<DIM, NPH, T>(P, nextRow, win[WS - 1]);
-                if (r + 1 < Y1) load_raw<DIM, NPH, T>(P, P.src, col, r + 1 + R, kz, nextRow);
+                if (r + 1 < Y1) loadRaw(col, r + 1 + R, nextRow);
             } else {
-                load_cell<DIM, NPH>(P, P.src, col, r + R, kz, win[WS - 1]);
+                loadCell(col, r + R, win[WS - 1]);
             }
             sigwin[WS - 1] = useSig ? (T)load_sigma<DIM>(P, col, r + R, kz) : T(0);
             // y reconstruction of row r+1
@@ -851,7 +876,7 @@ stage_xy_kernel(StageParams<T> P) {
         }
         if (E > 0 && lane < 2 * E) {
             Cell<DIM, NPH, C> ce;
-            load_cell<DIM, NPH>(P, P.src, ecol, r, kz, ce);
+            loadCell(ecol, r, ce);
             T vv[NV];
             recon_vars<DIM, NPH, T>(ce, vv);
             const int pos = lane < E ? lane : kWave + lane;
