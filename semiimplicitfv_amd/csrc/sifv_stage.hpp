@@ -1013,16 +1013,30 @@ stage_z_kernel(StageParams<T> P) {
 #pragma unroll
     for (int q = 0; q < NF; ++q) Hlo[q] = T(0);
     const int rStart = Z0 - 2;
-#pragma unroll
-    for (int w = 1; w < WS; ++w) {
-        const int kk = rStart - 1 + WLO + w;
-        load_cell<DIM, NPH>(P, P.src, cc, j, kk, win[w]);
-        sigwin[w] = useSig ? (T)load_sigma<DIM>(P, cc, j, kk) : T(0);
-    }
     // raw values of the plane entering the window next iteration, and the
     // x+y part of the current cell, are loaded one step ahead of their use
     constexpr bool PF = SIFV_PREFETCH_Z;
     constexpr bool PP = SIFV_PRELOAD_PART_Z;
+    // the conservative values of each plane as they entered the window, kept
+    // in LDS until the plane reaches the window centre R iterations later and
+    // the update reads them (instead of a second read from HBM)
+    constexpr int NCV = DIM + 2 + 2 * NPH;
+    constexpr int RING = R + 1;
+    __shared__ T sU[RING][NCV][kWave];
+#pragma unroll
+    for (int w = 1; w < WS; ++w) {
+        const int kk = rStart - 1 + WLO + w;
+        if (w == WS - 1) {
+            RawCell<DIM, NPH, T> rc;
+            load_raw<DIM, NPH, T>(P, P.src, cc, j, kk, rc);
+#pragma unroll
+            for (int q = 0; q < NCV; ++q) sU[(kk + RING) % RING][q][lane] = rc.U[q];
+            raw_to_cell<DIM, NPH, T>(P, rc, win[w]);
+        } else {
+            load_cell<DIM, NPH>(P, P.src, cc, j, kk, win[w]);
+        }
+        sigwin[w] = useSig ? (T)load_sigma<DIM>(P, cc, j, kk) : T(0);
+    }
     RawCell<DIM, NPH, T> nextRow;
     if (PF) load_raw<DIM, NPH, T>(P, P.src, cc, j, rStart + R, nextRow);
     for (int r = rStart; r < Z1; ++r) {
@@ -1032,10 +1046,16 @@ stage_z_kernel(StageParams<T> P) {
             sigwin[w] = sigwin[w + 1];
         }
         if constexpr (PF) {
+#pragma unroll
+            for (int q = 0; q < NCV; ++q) sU[(r + R + RING) % RING][q][lane] = nextRow.U[q];
             raw_to_cell<DIM, NPH, T>(P, nextRow, win[WS - 1]);
             if (r + 1 < Z1) load_raw<DIM, NPH, T>(P, P.src, cc, j, r + 1 + R, nextRow);
         } else {
-            load_cell<DIM, NPH>(P, P.src, cc, j, r + R, win[WS - 1]);
+            RawCell<DIM, NPH, T> rc;
+            load_raw<DIM, NPH, T>(P, P.src, cc, j, r + R, rc);
+#pragma unroll
+            for (int q = 0; q < NCV; ++q) sU[(r + R + RING) % RING][q][lane] = rc.U[q];
+            raw_to_cell<DIM, NPH, T>(P, rc, win[WS - 1]);
         }
         sigwin[WS - 1] = useSig ? (T)load_sigma<DIM>(P, cc, j, r + R) : T(0);
         T accPre[NF];
@@ -1064,7 +1084,11 @@ stage_z_kernel(StageParams<T> P) {
             for (int q = 0; q < NF; ++q) acc[q] = PP ? accPre[q] : P.part[(size_t)q * P.total + id];
             accumulate<DIM, NPH, T>(acc, Hlo, Hhi, (T)cz, false);
             const double h[3] = {dxc, dyj, dzr};
-            double dc = finish_cell<DIM, NPH, SEMI, T, true, C>(P, id, acc, win[WROW], h, dt, P.dtPartial != nullptr);
+            T Upre[NCV];
+#pragma unroll
+            for (int q = 0; q < NCV; ++q) Upre[q] = sU[(r + RING) % RING][q][lane];
+            double dc = finish_cell<DIM, NPH, SEMI, T, true, C>(P, id, acc, win[WROW], h, dt, P.dtPartial != nullptr,
+                                                                Upre);
             dtMin = dmin_ref(dtMin, dc);
         }
 #pragma unroll
