@@ -944,7 +944,8 @@ stage_xy_kernel(StageParams<T> P) {
             const size_t id = cell_index(g, col, r, kz);
             if (MODE == MODE_XY) {
 #pragma unroll
-                for (int q = 0; q < NF; ++q) P.part[(size_t)q * P.total + id] = acc[q];
+                for (int q = 0; q < NF; ++q)
+                    if (!(SEMI && q == DIM + 1)) P.part[(size_t)q * P.total + id] = acc[q];   // SEMI: energy unused
             } else {
                 const double h[3] = {dxc, dyr, dzk};
                 double dc = (YDIR && PU)
@@ -1062,7 +1063,8 @@ stage_z_kernel(StageParams<T> P) {
         if (PP && r >= Z0 && owned) {
             const size_t pid = cell_index(g, col, j, r);
 #pragma unroll
-            for (int q = 0; q < NF; ++q) accPre[q] = P.part[(size_t)q * P.total + pid];
+            for (int q = 0; q < NF; ++q)
+                accPre[q] = (SEMI && q == DIM + 1) ? T(0) : P.part[(size_t)q * P.total + pid];
         }
         FaceState<DIM, NPH, T> mZ, pZ;
         recon_window<DIM, NPH, R, T>(P, win, sigwin, WST, useSig, eps, weno, mZ, pZ);
@@ -1081,7 +1083,10 @@ stage_z_kernel(StageParams<T> P) {
             const size_t id = cell_index(g, col, j, r);
             T acc[NF];
 #pragma unroll
-            for (int q = 0; q < NF; ++q) acc[q] = PP ? accPre[q] : P.part[(size_t)q * P.total + id];
+            // the semi-implicit update rebuilds the total energy in the correction
+            // step: the x+y pass does not store its energy slot
+            for (int q = 0; q < NF; ++q)
+                acc[q] = PP ? accPre[q] : ((SEMI && q == DIM + 1) ? T(0) : P.part[(size_t)q * P.total + id]);
             accumulate<DIM, NPH, T>(acc, Hlo, Hhi, (T)cz, false);
             const double h[3] = {dxc, dyj, dzr};
             T Upre[NCV];
