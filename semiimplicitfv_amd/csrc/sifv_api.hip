@@ -157,6 +157,7 @@ struct sifv_handle {
     double* vel[3] = {nullptr, nullptr, nullptr};
     double* pres = nullptr;
     double* sigma = nullptr;
+    bool sigmaUploaded = false;   // sigma may be nonzero without IGR (a caller uploaded it)
     double* partial = nullptr;
     int npartialCap = 0;
     int npartialValid = 0;
@@ -1377,7 +1378,7 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
         K.total = h->total;
         K.cons = h->consField<double>(dst[s], 0);
         K.p = pfinal;
-        K.sigma = h->sigma;
+        K.sigma = h->cfg.useIGR || h->sigmaUploaded ? h->sigma : nullptr;   // otherwise zero
         for (int d = 0; d < 3; ++d) K.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
         K.pres = h->pres;
         K.dt = h->scal;
@@ -1998,6 +1999,7 @@ int sifv_upload(sifv_handle* h, int field, const double* host, size_t n) {
     } else {
         double* p = prim_ptr(h, field);
         if (!p) return fail(SIFV_ERR_INVALID, "field not present for this configuration");
+        if (field == SIFV_F_SIGMA) h->sigmaUploaded = true;
         HIPCHK(hipMemcpy(p, host, n * sizeof(double), hipMemcpyHostToDevice));
     }
     h->dtNextValid = false;

@@ -202,17 +202,34 @@ __global__ void __launch_bounds__(256) semi_correct_kernel(CorrectArgs A) {
     const double dt = *A.dt;
     const int dim = A.dim;
     int i, j, k;
+    // sigma is zero without IGR (A.sigma null): p + sigma is p then
+    const bool sig = A.sigma != nullptr;
+    const size_t st[3] = {1, (size_t)g.nxT, (size_t)g.nxT * g.nyT};
     for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
-        const double pc = A.p[id] + A.sigma[id];
+        const double pc = sig ? A.p[id] + A.sigma[id] : A.p[id];
         const double rho = A.cons[id];
         const double rs = dmax_ref(rho, 1e-14);
         const double h[3] = {g.dxw[i + g.ngx], g.dyw[j + g.ngy], g.dzw[k + g.ngz]};
+        // neighbours of an interior cell are read directly, the others through
+        // the scalar ghost rule
+        const bool inner = i > 0 && i < g.nx - 1 && (dim < 2 || (j > 0 && j < g.ny - 1)) &&
+                           (dim < 3 || (k > 0 && k < g.nz - 1));
         double u[3];
         for (int d = 0; d < dim; ++d) {
             const int di = d == 0, dj = d == 1, dk = d == 2;
-            const double pm = scalar_at(g, A.p, i - di, j - dj, k - dk) + scalar_at(g, A.sigma, i - di, j - dj, k - dk);
-            const double pp = scalar_at(g, A.p, i + di, j + dj, k + dk) + scalar_at(g, A.sigma, i + di, j + dj, k + dk);
+            double pm, pp;
+            if (inner) {
+                pm = sig ? A.p[id - st[d]] + A.sigma[id - st[d]] : A.p[id - st[d]];
+                pp = sig ? A.p[id + st[d]] + A.sigma[id + st[d]] : A.p[id + st[d]];
+            } else {
+                pm = scalar_at(g, A.p, i - di, j - dj, k - dk);
+                pp = scalar_at(g, A.p, i + di, j + dj, k + dk);
+                if (sig) {
+                    pm += scalar_at(g, A.sigma, i - di, j - dj, k - dk);
+                    pp += scalar_at(g, A.sigma, i + di, j + dj, k + dk);
+                }
+            }
             const double pl = 0.5 * (pm + pc);
             const double pr = 0.5 * (pc + pp);
             double* mom = A.cons + (size_t)(1 + d) * A.total;

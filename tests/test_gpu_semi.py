@@ -238,3 +238,29 @@ def test_mixed_precision_jacobi(gpu, case):
     for fid in a.field_ids():
         l1, li = rel_errs(mesh.interior(b.field(fid)), mesh.interior(a.field(fid)))
         assert l1 <= 1e-8, (fid, l1, li, ia, ib)
+
+
+@pytest.mark.parametrize("dim,n", [(2, (40, 36)), (3, (16, 14, 12))])
+def test_semi_correction_without_sigma_upload(gpu, dim, n):
+    """Without IGR, a handle whose sigma was never uploaded leaves sigma out of
+    the pressure correction (it is zero); its steps equal, bit for bit, those of
+    a handle that uploaded sigma = 0 and reads it."""
+    cfg, mesh, st, eos = make_semi_case(dim, n, bcs=[BoundaryCondition.SlipWall] * 6, seed=11 + dim, tol=1e-10,
+                                        maxit=300)
+    assert not np.any(st.field(_abi.F_SIGMA))
+    runs = []
+    for with_sigma in (True, False):
+        g = SemiImplicitSolver(mesh, HLLCSolver(eos), JacobiPressureSolver(), eos, None, cfg)
+        for fid in st.field_ids():
+            if with_sigma or fid != _abi.F_SIGMA:
+                g.upload_field(fid, st.field(fid).copy())
+        out = []
+        for k in range(3):
+            cfg.step = k
+            out.append((g.step_resident(cfg), g.lastPressureIterations()))
+        fields = {f: g.download_field(f).copy() for f in (_abi.F_RHO, _abi.F_RHOU, _abi.F_RHOE, _abi.F_PRES)}
+        g.close()
+        runs.append((out, fields))
+    assert runs[0][0] == runs[1][0]
+    for f in runs[0][1]:
+        assert np.array_equal(runs[0][1][f], runs[1][1][f]), f
