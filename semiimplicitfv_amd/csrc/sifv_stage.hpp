@@ -415,6 +415,15 @@ __device__ __forceinline__ void face_state(const StageParams<T>& P, const T* v, 
 template <int DIM, int NPH>
 constexpr int n_fc() { return DIM + 2 + 2 * NPH + 2; }
 
+// Slots of the 3D x+y partial right-hand side the z pass's update reads: the
+// semi-implicit update rebuilds the total energy in its correction step, and
+// a multiphase update rebuilds the density as the sum of the partial
+// densities, so neither slot is stored (nor its arithmetic kept alive).
+template <int DIM, int NPH, bool SEMI>
+__device__ __forceinline__ constexpr bool part_slot_used(int q) {
+    return !((SEMI && q == DIM + 1) || (NPH > 0 && q == 0));
+}
+
 template <int DIM, int NPH, typename T, typename C>
 __device__ __forceinline__ void face_contrib(const T* F, C rhoL, C rhoR, const C* arL, const C* arR, T* fc) {
     fc[0] = F[0];
@@ -945,7 +954,7 @@ stage_xy_kernel(StageParams<T> P) {
             if (MODE == MODE_XY) {
 #pragma unroll
                 for (int q = 0; q < NF; ++q)
-                    if (!(SEMI && q == DIM + 1)) P.part[(size_t)q * P.total + id] = acc[q];   // SEMI: energy unused
+                    if (part_slot_used<DIM, NPH, SEMI>(q)) P.part[(size_t)q * P.total + id] = acc[q];
             } else {
                 const double h[3] = {dxc, dyr, dzk};
                 double dc = (YDIR && PU)
@@ -1064,7 +1073,7 @@ stage_z_kernel(StageParams<T> P) {
             const size_t pid = cell_index(g, col, j, r);
 #pragma unroll
             for (int q = 0; q < NF; ++q)
-                accPre[q] = (SEMI && q == DIM + 1) ? T(0) : P.part[(size_t)q * P.total + pid];
+                accPre[q] = part_slot_used<DIM, NPH, SEMI>(q) ? P.part[(size_t)q * P.total + pid] : T(0);
         }
         FaceState<DIM, NPH, T> mZ, pZ;
         recon_window<DIM, NPH, R, T>(P, win, sigwin, WST, useSig, eps, weno, mZ, pZ);
@@ -1083,10 +1092,8 @@ stage_z_kernel(StageParams<T> P) {
             const size_t id = cell_index(g, col, j, r);
             T acc[NF];
 #pragma unroll
-            // the semi-implicit update rebuilds the total energy in the correction
-            // step: the x+y pass does not store its energy slot
             for (int q = 0; q < NF; ++q)
-                acc[q] = PP ? accPre[q] : ((SEMI && q == DIM + 1) ? T(0) : P.part[(size_t)q * P.total + id]);
+                acc[q] = PP ? accPre[q] : (part_slot_used<DIM, NPH, SEMI>(q) ? P.part[(size_t)q * P.total + id] : T(0));
             accumulate<DIM, NPH, T>(acc, Hlo, Hhi, (T)cz, false);
             const double h[3] = {dxc, dyj, dzr};
             T Upre[NCV];
