@@ -434,6 +434,9 @@ int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const d
     P.pres = h->pres;
     for (int d = 0; d < 3; ++d) P.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
     P.pAdv = h->jac[4];
+    // semi-implicit stages also leave rho c^2 of the updated state for the
+    // pressure system (semi_setup_kernel reads it then)
+    P.rhoc2 = semi ? h->jac[2] : nullptr;
     P.bodyForce = has_body_force(h->cfg) ? 1 : 0;
     P.accel = h->scal + 4;
     P.visc = has_face_stress(h->cfg) ? h->visc : nullptr;
@@ -1344,6 +1347,7 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
         }
         S.pAdv = h->jac[4];
         S.rhoc2 = h->jac[2];
+        S.rc2Given = 1;
         S.rhs = h->jac[3];
         S.p0 = h->jac[0];
         S.resid = h->resid;

@@ -1,5 +1,6 @@
 // sifv_semi.hpp - kernels of the semi-implicit pressure step:
-//   semi_setup_kernel    rho c^2, div(u*), rhs = p_adv - rho c^2 dt div(u*),
+//   semi_setup_kernel    rho c^2 (unless the stage update wrote it), div(u*),
+//                        rhs = p_adv - rho c^2 dt div(u*),
 //                        initial guess (SemiImplicitSolver::solvePressure,
 //                        src/SemiImplicitSolver.cpp:533-575, computeDivergence :680-717)
 //   jacobi_kernel        one Jacobi sweep of (I + dt^2 rho c^2 L) p = rhs with the
@@ -30,9 +31,10 @@ struct SemiArgs {
     const double* vel[3];     // star velocity (ghosts filled) / corrected velocity (out)
     double* velOut[3];
     const double* pAdv;
-    double* rhoc2;
+    double* rhoc2;            // written here unless the stage update wrote it (rc2Given)
     double* rhs;
     double* p0;               // Jacobi initial guess
+    int rc2Given;
     double* resid;            // per-iteration max |pNew - p| (zeroed here)
     int maxIter;
     const double* dt;
@@ -54,21 +56,27 @@ __global__ void __launch_bounds__(256) semi_setup_kernel(SemiArgs A) {
     int i, j, k;
     for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
-        const double rho = A.rho[id];
         const double p = A.pres[id];
-        double c;
-        if (A.nph > 0) {
-            double a[kMaxPhases], ar[kMaxPhases];
-            for (int ph = 0; ph < A.nph; ++ph) {
-                a[ph] = A.al[ph][id];
-                ar[ph] = A.ar[ph][id];
-            }
-            c = mix_sound(A.mix, rho, p, a, ar, A.nph);
+        double rc2;
+        if (A.rc2Given) {
+            rc2 = A.rhoc2[id];
         } else {
-            double pinf = A.eos.stiffened ? A.eos.pInf : 0.0;
-            c = sqrt(A.eos.gamma * dmax_ref(p + pinf, 1e-14) / dmax_ref(rho, 1e-14));
+            const double rho = A.rho[id];
+            double c;
+            if (A.nph > 0) {
+                double a[kMaxPhases], ar[kMaxPhases];
+                for (int ph = 0; ph < A.nph; ++ph) {
+                    a[ph] = A.al[ph][id];
+                    ar[ph] = A.ar[ph][id];
+                }
+                c = mix_sound(A.mix, rho, p, a, ar, A.nph);
+            } else {
+                double pinf = A.eos.stiffened ? A.eos.pInf : 0.0;
+                c = sqrt(A.eos.gamma * dmax_ref(p + pinf, 1e-14) / dmax_ref(rho, 1e-14));
+            }
+            rc2 = rho * c * c;
+            A.rhoc2[id] = rc2;
         }
-        const double rc2 = rho * c * c;
         // central divergence of the star velocity (ghosts filled by the PRIM rule)
         double div = 0.0;
         const size_t sx = 1, sy = (size_t)g.nxT, sz = (size_t)g.nxT * g.nyT;
@@ -80,7 +88,6 @@ __global__ void __launch_bounds__(256) semi_setup_kernel(SemiArgs A) {
             double fr = 0.5 * (v[id] + v[id + st[d]]);
             div += (fr - fl) / h[d];
         }
-        A.rhoc2[id] = rc2;
         A.rhs[id] = A.pAdv[id] - rc2 * dt * div;
         A.p0[id] = p;
     }

@@ -124,6 +124,7 @@ struct StageParams {
     double* pres;         // semi-implicit: stage-start pressure (written)
     double* vel[3];       // semi-implicit: star velocity (written)
     double* pAdv;         // semi-implicit: advected pressure (written)
+    double* rhoc2;        // semi-implicit: rho c^2 of the updated state (written; null: not)
     int bodyForce = 0;              // hasBodyForce
     const double* accel = nullptr;  // device-resident body-force acceleration of the step
     const double* visc = nullptr;   // viscous momentum/energy right-hand side (DIM+1 fields) or null
@@ -632,6 +633,25 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
         double rs = dmax_ref(un[0], 1e-14);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) P.vel[d][id] = un[1 + d] / rs;
+        if (P.rhoc2) {
+            // rho c^2 of the updated state at the stage-start pressure, as the
+            // pressure system's set-up forms it (semi_setup_kernel)
+            const double rho = (double)(T)un[0];
+            double c;
+            if constexpr (NPH > 0) {
+                double a[kMaxPhases], ar[kMaxPhases];
+#pragma unroll
+                for (int ph = 0; ph < NPH; ++ph) {
+                    a[ph] = (double)(T)un[DIM + 2 + NPH + ph];
+                    ar[ph] = (double)(T)un[DIM + 2 + ph];
+                }
+                c = mix_sound(P.mix, rho, cell.p, a, ar, NPH);
+            } else {
+                const double pinf = P.eos.stiffened ? P.eos.pInf : 0.0;
+                c = sqrt(P.eos.gamma * dmax_ref(cell.p + pinf, 1e-14) / dmax_ref(rho, 1e-14));
+            }
+            P.rhoc2[id] = rho * c * c;
+        }
         return 1.0e300;
     }
     if (!wantDt) return 1.0e300;
