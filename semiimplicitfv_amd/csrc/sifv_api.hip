@@ -1231,8 +1231,8 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
                             cgrid(M.C.cells));
             }
         }
+        // fine level: prolongation folded into the post-smooth
         M.C = h->mg[0];
-        if (!r) r = mlaunch(reinterpret_cast<const void*>(mg_prolong_fine_kernel), nb);
         if (!r) r = mlaunch(by_dim(dim, mg_post_fine_kernel<1>, mg_post_fine_kernel<2>, mg_post_fine_kernel<3>), nb);
         return r;
     };

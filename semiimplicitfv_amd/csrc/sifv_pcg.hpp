@@ -129,6 +129,38 @@ __device__ __forceinline__ double pcg_row(const PcgArgs& A, const double* v, dou
     return acc;
 }
 
+// Coordinate along dir of the neighbour pcg_nb returns.
+__device__ __forceinline__ int pcg_nbc(const Grid& g, int dir, int side, int c) {
+    const int n = dir == 0 ? g.nx : dir == 1 ? g.ny : g.nz;
+    if (side == 0) return c > 0 ? c - 1 : (g.sbc[2 * dir] == BK_WRAP ? n - 1 : c);
+    return c < n - 1 ? c + 1 : (g.sbc[2 * dir + 1] == BK_WRAP ? 0 : c);
+}
+
+// pcg_row with the vector given by val(index, i, j, k) at the cell and its
+// neighbours instead of an array (a vector the kernel forms on the fly from
+// the arrays it is made of); same operand order as pcg_row.  vi: the value
+// at the cell.
+template <int DIM, typename V>
+__device__ __forceinline__ double pcg_row_f(const PcgArgs& A, V&& val, double si, int i, int j, int k, size_t id,
+                                            double& vi) {
+    const Grid& g = A.g;
+    vi = val(id, i, j, k);
+    double acc = si * vi;
+#pragma unroll
+    for (int dir = 0; dir < DIM; ++dir) {
+        const size_t stride = dir == 0 ? 1 : dir == 1 ? (size_t)g.nxT : (size_t)g.nxT * g.nyT;
+        const double wl = A.W[dir][id], wh = A.W[dir][id + stride];
+        const int c = dir == 0 ? i : dir == 1 ? j : k;
+        const int cl = pcg_nbc(g, dir, 0, c), ch = pcg_nbc(g, dir, 1, c);
+        const size_t nl = pcg_nb(g, dir, 0, i, j, k, id, stride), nh = pcg_nb(g, dir, 1, i, j, k, id, stride);
+        const double vl = dir == 0 ? val(nl, cl, j, k) : dir == 1 ? val(nl, i, cl, k) : val(nl, i, j, cl);
+        const double vh = dir == 0 ? val(nh, ch, j, k) : dir == 1 ? val(nh, i, ch, k) : val(nh, i, j, ch);
+        acc += wl * (vi - vl);
+        acc += wh * (vi - vh);
+    }
+    return acc;
+}
+
 // Traversal of the interior by bricks of 64 columns x 4 rows x kBrickPlanes
 // planes for the stencil kernels (2D/3D; blocks of 256 threads = 64 x 4): a
 // thread marches its column (i, j) up the brick's planes, so the z
@@ -415,8 +447,8 @@ __global__ void __launch_bounds__(256) pcg_dir_kernel(PcgArgs A) {
 // is a fixed SPD operator and conjugate gradients stay valid.
 // Fine-level work per V-cycle and cell: pre (q = om r / d, folded into
 // pcg_update_kernel inside the iteration), restrict (read q with neighbours,
-// W, s, r: 48 B in 3D), prolong (q: 16 B), post (read q W s r d, write z, with
-// the r.z partial sums: 64 B).
+// W, s, r: 48 B in 3D), prolong and post in one pass (read q W s r d and the
+// parent's correction, write z, with the r.z partial sums: 64 B).
 constexpr int kMgMaxLevels = 24;
 constexpr int kMgCoarseSweeps = 16;
 
@@ -562,34 +594,31 @@ __global__ void __launch_bounds__(256) mg_restrict_fine_kernel(MgArgs M) {
     }
 }
 
-// q += t(parent)
-__global__ void __launch_bounds__(256) mg_prolong_fine_kernel(MgArgs M) {
-    if (mg_skip(M)) return;
-    const PcgArgs& A = M.A;
-    const Grid& g = A.g;
-    const MgLevel& C = M.C;
-    int i, j, k;
-    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
-        const size_t id = cell_index(g, i, j, k);
-        A.q[id] += C.t[mg_idx(C, i >> 1, j >> 1, k >> 1)];
-    }
-}
-
-// fine post-smooth: z = q + om (r - S q) / d, with the partial sums of r.z
-// into slot M.slot (and p = z when M.init)
+// fine prolongation and post-smooth: with q' = q + t(parent) (the coarse
+// correction added to the pre-smoothed q, formed for the cell and its
+// neighbours where the product needs it, so q' is never stored),
+// z = q' + om (r - S q') / d, with the partial sums of r.z into slot M.slot
+// (and p = z when M.init)
 template <int DIM>
 __global__ void __launch_bounds__(256) mg_post_fine_kernel(MgArgs M) {
     if (mg_skip(M)) return;
     __shared__ double sh[256 / kWave];
     const PcgArgs& A = M.A;
     const Grid& g = A.g;
+    const MgLevel& C = M.C;
     constexpr double om = mg_omega(DIM);
+    const double* __restrict__ q = A.q;
+    const double* __restrict__ tc = C.t;
+    auto qp = [&](size_t id, int i, int j, int k) __attribute__((always_inline)) {
+        return q[id] + tc[mg_idx(C, i >> 1, j >> 1, k >> 1)];
+    };
     double rz = 0.0;
     auto cell = [&](int i, int j, int k) {
         const size_t id = cell_index(g, i, j, k);
-        const double a = pcg_row<DIM>(A, A.q, A.s[id], i, j, k, id);
+        double qi;
+        const double a = pcg_row_f<DIM>(A, qp, A.s[id], i, j, k, id, qi);
         const double r = A.r[id];
-        const double z = A.q[id] + om * (r - a) / A.d[id];
+        const double z = qi + om * (r - a) / A.d[id];
         A.z[id] = z;
         rz += r * z;
         if (M.init) A.p[id] = z;
