@@ -50,6 +50,8 @@ def parse():
                     help="only run the 3D bubble (FP64) for --steps steps and print its line (profiling the 3D "
                          "two-phase stage kernels and the pressure solvers)")
     ap.add_argument("--bubble-solver", default="jacobi", choices=["jacobi", "pcg", "mgpcg"])
+    ap.add_argument("--bubble-precision", default="fp64", choices=["fp64", "mixed"],
+                    help="mixed: FP32 Jacobi sweeps + FP64 refinement, or the FP32 V-cycle of the multigrid PCG")
     ap.add_argument("--bubble-stirred", action="store_true", help="the stirred bubble (stiff pressure solves)")
     ap.add_argument("--bubble-graphs", action="store_true",
                     help="replay the Jacobi sweep chunks from captured HIP graphs")
@@ -344,7 +346,9 @@ def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred:
     sweeps), WENO5 + HLLC, FP64.  Fields are built one at a time to bound host
     memory.  One step = 3 advective stages + 3 pressure solves + corrections.
     precision "mixed": the same FP64 state with the pressure solve run as FP32
-    Jacobi sweeps refined by FP64 residual passes.  stirred: the velocity
+    Jacobi sweeps refined by FP64 residual passes (solver "jacobi") or with
+    the multigrid V-cycle in FP32 inside FP64 conjugate gradients (solver
+    "mgpcg").  stirred: the velocity
     field of the 12^3 bubble fixture (u = 2 sin(pi x) cos(pi y), v = -2 cos(pi x)
     sin(pi y), w = 0.5 sin(pi z)), which makes every pressure solve run to its
     1000-sweep cap.  solver "pcg": the conjugate-gradient pressure solve
@@ -426,8 +430,13 @@ def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred:
     g.close()
     solve = {"pcg": "PCG (relative residual 1e-6)",
              "mgpcg": "multigrid-preconditioned PCG (relative residual 1e-6)"}.get(solver, "Jacobi (tol 1e-6)")
-    return {"workload": f"3D water/air bubble {n}^3, semi-implicit SSP-RK3 + {solve}, WENO5+HLLC, FP64 "
-                        + ("state, FP32 Jacobi sweeps + FP64 refinement" if precision == "mixed" else "throughout"),
+    if precision != "mixed":
+        prec = "throughout"
+    elif solver == "mgpcg":
+        prec = "state and conjugate gradients, FP32 V-cycle"
+    else:
+        prec = "state, FP32 Jacobi sweeps + FP64 refinement"
+    return {"workload": f"3D water/air bubble {n}^3, semi-implicit SSP-RK3 + {solve}, WENO5+HLLC, FP64 " + prec,
             "value": n ** 3 * steps / el, "unit": "cell-updates/s", "ms_per_step": el / steps * 1e3,
             "pressure_iterations_last_stage": iters, "host_setup_s": setup_s, "finite": finite, "_pres": pres}
 
@@ -537,8 +546,8 @@ def main():
         copy_probe(a.copy_probe)
         return
     if a.bubble_only:
-        r = bench_bubble(a.bubble_n, a.steps, stirred=a.bubble_stirred, solver=a.bubble_solver,
-                         graphs=a.bubble_graphs)
+        r = bench_bubble(a.bubble_n, a.steps, precision=a.bubble_precision, stirred=a.bubble_stirred,
+                         solver=a.bubble_solver, graphs=a.bubble_graphs)
         r.pop("_pres", None)
         print(json.dumps(r))
         return
@@ -620,7 +629,11 @@ def main():
             r32 = bench_bubble(n, steps, precision="mixed", stirred=stirred)
             rcg = bench_bubble(n, steps, precision="fp64", stirred=stirred, solver="pcg")
             rmg = bench_bubble(n, steps, precision="fp64", stirred=stirred, solver="mgpcg")
+            rmg32 = bench_bubble(n, steps, precision="mixed", stirred=stirred, solver="mgpcg")
             p64, p32, pcg, pmg = r64.pop("_pres"), r32.pop("_pres"), rcg.pop("_pres"), rmg.pop("_pres")
+            pmg32 = rmg32.pop("_pres")
+            rmg32["pres_rel_L1_vs_mgpcg"] = float(np.abs(pmg32 - pmg).sum() / np.abs(pmg).sum())
+            rmg32["ratio_vs_mgpcg"] = rmg32["value"] / rmg["value"]
             rmg["pres_rel_L1_vs_jacobi"] = float(np.abs(pmg - p64).sum() / np.abs(p64).sum())
             rmg["pres_rel_L1_vs_pcg"] = float(np.abs(pmg - pcg).sum() / np.abs(pcg).sum())
             rmg["ratio_vs_jacobi"] = rmg["value"] / r64["value"]
@@ -631,7 +644,7 @@ def main():
             if stirred:
                 r32["note"] = ("both solves stop at the 1000-sweep cap unconverged, so pres_rel_L1_vs_fp64 "
                                "compares two different partial iterates, not FP32 rounding")
-            out[tag] = {"fp64": r64, "fp32_sweeps": r32, "pcg": rcg, "mgpcg": rmg}
+            out[tag] = {"fp64": r64, "fp32_sweeps": r32, "pcg": rcg, "mgpcg": rmg, "mgpcg_fp32_vcycle": rmg32}
         return out
 
     for name, fn in (("ensemble", bench_ensemble), ("adjoint", bench_adjoint), ("bubble", bubble_pair),

@@ -62,7 +62,8 @@ enum { SIFV_EOS_IDEAL = 0, SIFV_EOS_STIFFENED = 1 };
  * stiffened-gas constant sum_k alpha_k gamma_k pInf_k/(gamma_k-1), so the FP32
  * state keeps the pressure part; an ideal gas converts and updates in FP32.
  * With the semi-implicit solver SIFV_MIXED keeps the FP64 state and runs the
- * Jacobi sweeps in FP32 with FP64 residual refinement. */
+ * Jacobi sweeps in FP32 with FP64 residual refinement, or the multigrid
+ * V-cycle of the MG-PCG pressure solve in FP32 (sifv_mgpcg_solve). */
 enum { SIFV_FP64 = 0, SIFV_MIXED = 1 };
 
 /* Field ids for upload/download (SolutionState members,
@@ -214,15 +215,20 @@ int sifv_pcg_solve(sifv_handle* h, const double* rho, const double* rhoc2,
  * (weighted-Jacobi relaxation, one pre- and one post-sweep, 2:1 coarsening of
  * every axis down to at most 4 cells per axis, piecewise-constant transfers,
  * rediscretised coarse operators) instead of the diagonal.  Same arguments,
- * stopping rule, iteration count and refusals as sifv_pcg_solve. */
+ * stopping rule, iteration count and refusals as sifv_pcg_solve.  On a
+ * SIFV_MIXED semi-implicit handle the V-cycle runs in FP32 (operator rounded
+ * to FP32 once per solve) inside FP64 conjugate gradients, whose residual
+ * and stopping test stay FP64; the iteration count may then differ from the
+ * FP64 form's by a few. */
 int sifv_mgpcg_solve(sifv_handle* h, const double* rho, const double* rhoc2,
                      const double* rhs, double* pressure, double dt,
                      double tolerance, int maxIter, int* iters);
 
 /* Chooses the PressureSolver of sifv_semi_implicit_step (the reference's
  * SemiImplicitSolver constructor argument, include/SemiImplicitSolver.hpp:24-31):
- * SIFV_PS_JACOBI (default), SIFV_PS_PCG or SIFV_PS_MGPCG (the last two on
- * FP64 handles only). */
+ * SIFV_PS_JACOBI (default), SIFV_PS_PCG (FP64 handles only) or
+ * SIFV_PS_MGPCG (on a SIFV_MIXED handle with the FP32 V-cycle of
+ * sifv_mgpcg_solve). */
 int sifv_set_pressure_solver(sifv_handle* h, int kind);
 
 /* ---- ghost fill and time step ------------------------------------------ */

@@ -226,6 +226,10 @@ struct sifv_handle {
     int mgLevels = 0;         // coarse levels of the multigrid preconditioner (0: not built)
     MgLevel mg[kMgMaxLevels] = {};
     double* mgBuf = nullptr;  // storage of every coarse level
+    int mgLevels32 = 0;       // the same levels in FP32 (mixed-precision V-cycle)
+    MgLevelT<float> mg32[kMgMaxLevels] = {};
+    float* mgBuf32 = nullptr;
+    float* pcg32[6] = {};     // FP32 fine level of that V-cycle: W x3, s, d, pre-smoothed q
     double* jacPn = nullptr;  // FP64 Jacobi update of the refined iterate
     float* jac32 = nullptr;   // rho, co, r, d0, d1 (FP32)
     // discrete adjoint: checkpoints of the forward run and work arrays
@@ -1069,13 +1073,14 @@ int run_jacobi_mixed(sifv_handle* h, const double* rho, const double* dt, double
 // Coarse levels of the multigrid preconditioner (sifv_pcg.hpp): every axis
 // halved (rounding up) while the finest has no coarse level yet or the
 // coarsest still has an axis longer than 4 cells; 8 compact arrays per level.
-int ensure_mg(sifv_handle* h) {
-    if (h->mgLevels) return SIFV_OK;
+template <typename R>
+int ensure_mg_levels(sifv_handle* h, MgLevelT<R>* out, R** bufOut, int* levelsOut) {
+    if (*levelsOut) return SIFV_OK;
     static_assert(kMgCoarsestCells == 64, "the coarsest level holds at most 4 x 4 x 4 cells");
     int n[3] = {h->nx, h->ny, h->nz};
     int L = 0;
     size_t total = 0;
-    MgLevel lv[kMgMaxLevels] = {};
+    MgLevelT<R> lv[kMgMaxLevels] = {};
     while (L < kMgMaxLevels) {
         const int mx = std::max(n[0], std::max(n[1], n[2]));
         if (L > 0 && mx <= 4) break;
@@ -1086,10 +1091,10 @@ int ensure_mg(sifv_handle* h) {
         ++L;
     }
     if (lv[L - 1].cells > kMgCoarsestCells) return fail(SIFV_ERR_HIP, "multigrid coarsest level too large");
-    double* buf = nullptr;
-    HIPCHK(hipMalloc(&buf, sizeof(double) * total));
-    HIPCHK(hipMemset(buf, 0, sizeof(double) * total));
-    double* q = buf;
+    R* buf = nullptr;
+    HIPCHK(hipMalloc(&buf, sizeof(R) * total));
+    HIPCHK(hipMemset(buf, 0, sizeof(R) * total));
+    R* q = buf;
     for (int l = 0; l < L; ++l) {
         const size_t c = lv[l].cells;
         lv[l].s = q;
@@ -1107,10 +1112,15 @@ int ensure_mg(sifv_handle* h) {
         lv[l].t = q;
         q += c;
     }
-    h->mgBuf = buf;
-    for (int l = 0; l < L; ++l) h->mg[l] = lv[l];
-    h->mgLevels = L;
+    *bufOut = buf;
+    for (int l = 0; l < L; ++l) out[l] = lv[l];
+    *levelsOut = L;
     return SIFV_OK;
+}
+
+int ensure_mg(sifv_handle* h, bool fp32) {
+    return fp32 ? ensure_mg_levels<float>(h, h->mg32, &h->mgBuf32, &h->mgLevels32)
+                : ensure_mg_levels<double>(h, h->mg, &h->mgBuf, &h->mgLevels);
 }
 
 template <typename F1, typename F2, typename F3>
@@ -1118,6 +1128,111 @@ const void* by_dim(int dim, F1 f1, F2 f2, F3 f3) {
     return dim == 1 ? reinterpret_cast<const void*>(f1)
            : dim == 2 ? reinterpret_cast<const void*>(f2) : reinterpret_cast<const void*>(f3);
 }
+
+// Multigrid V-cycle of run_pcg over the coarse levels lv[0..L-1] in R
+// (double: the FP64 preconditioner; float: the FP32 one of the mixed form):
+// build() forms the coarse operators, run() applies z = V(r) on the fine level,
+// its kernels skipped once *M.gate is set (null gate during the set-up).
+template <typename R>
+struct MgCycle {
+    sifv_handle* h;
+    MgArgsT<R> M;
+    const MgLevelT<R>* lv;
+    int L, nb, dim;
+
+    int launch(const void* fn, int grid) {
+        void* a[] = {&M};
+        return timed_launch(h, fn, dim3(grid), dim3(256), a);
+    }
+    static int cgrid(int cells) { return std::max(1, std::min((cells + 255) / 256, 8192)); }
+
+    int build() {
+        int rc = SIFV_OK;
+        for (int l = 0; l < L && !rc; ++l) {
+            M.C = lv[l];
+            if (l == 0) {
+                rc = launch(by_dim(dim, mg_build_kernel<1, true, R>, mg_build_kernel<2, true, R>,
+                                   mg_build_kernel<3, true, R>),
+                            cgrid(M.C.cells));
+            } else {
+                M.F = lv[l - 1];
+                rc = launch(by_dim(dim, mg_build_kernel<1, false, R>, mg_build_kernel<2, false, R>,
+                                   mg_build_kernel<3, false, R>),
+                            cgrid(M.C.cells));
+            }
+            if (!rc)
+                rc = launch(by_dim(dim, mg_diag_kernel<1, R>, mg_diag_kernel<2, R>, mg_diag_kernel<3, R>),
+                            cgrid(M.C.cells));
+        }
+        return rc;
+    }
+
+    // pre: run the fine pre-smooth (inside the iteration pcg_update_kernel
+    // has already written it)
+    int run(bool pre) {
+        int r = pre ? launch(by_dim(dim, mg_pre_fine_kernel<1, R>, mg_pre_fine_kernel<2, R>, mg_pre_fine_kernel<3, R>),
+                             nb)
+                    : SIFV_OK;
+        M.C = lv[0];
+        if (!r)
+            r = launch(by_dim(dim, mg_restrict_fine_kernel<1, R>, mg_restrict_fine_kernel<2, R>,
+                              mg_restrict_fine_kernel<3, R>),
+                       cgrid(M.C.cells));
+        // levels m..L-1 (at most SIFV_MG_TAIL_CELLS cells each) run in one
+        // workgroup, the coarsest included
+        int m = L - 1;
+        while (m > 0 && lv[m - 1].cells <= SIFV_MG_TAIL_CELLS && L - (m - 1) <= kMgTailMaxLevels) --m;
+        for (int l = 0; l < m && !r; ++l) {
+            M.C = lv[l];
+            r = launch(by_dim(dim, mg_pre_kernel<1, R>, mg_pre_kernel<2, R>, mg_pre_kernel<3, R>), cgrid(M.C.cells));
+            if (!r) {
+                M.F = lv[l];
+                M.C = lv[l + 1];
+                r = launch(by_dim(dim, mg_restrict_kernel<1, R>, mg_restrict_kernel<2, R>, mg_restrict_kernel<3, R>),
+                           cgrid(M.C.cells));
+            }
+        }
+        if (!r) {
+            MgTailArgsT<R> T;
+            T.gate = M.gate;
+            T.nlev = L - m;
+            for (int q = 0; q < T.nlev; ++q) T.lv[q] = lv[m + q];
+            void* targs[] = {&T};
+            r = timed_launch(h, by_dim(dim, mg_tail_kernel<1, R>, mg_tail_kernel<2, R>, mg_tail_kernel<3, R>),
+                             dim3(1), dim3(kMgTailThreads), targs);
+        }
+        for (int l = m - 1; l >= 0 && !r; --l) {
+            M.F = lv[l];
+            M.C = lv[l + 1];
+            r = launch(reinterpret_cast<const void*>(mg_prolong_kernel<R>), cgrid(M.F.cells));
+            if (!r) {
+                M.C = lv[l];
+                M.src = M.C.x;
+                M.dst = M.C.t;
+                r = launch(by_dim(dim, mg_smooth_kernel<1, R>, mg_smooth_kernel<2, R>, mg_smooth_kernel<3, R>),
+                           cgrid(M.C.cells));
+            }
+        }
+        // fine level: prolongation folded into the post-smooth
+        M.C = lv[0];
+        if (!r)
+            r = launch(by_dim(dim, mg_post_fine_kernel<1, R>, mg_post_fine_kernel<2, R>, mg_post_fine_kernel<3, R>),
+                       nb);
+        return r;
+    }
+
+    // set-up V-cycle (z and p of the initial residual, r.z partials in slot
+    // 2), then the gate and slot of the iteration
+    int first() {
+        M.slot = 2;
+        M.init = 1;
+        const int rc = run(true);
+        M.gate = M.A.st;
+        M.slot = 1;
+        M.init = 0;
+        return rc;
+    }
+};
 
 int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int maxIter, int* n, bool mg) {
     if (!h->pcgSym)
@@ -1180,85 +1295,43 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
     const void* matvec = h->dim == 1 ? reinterpret_cast<const void*>(pcg_matvec_kernel<1>)
                          : h->dim == 2 ? reinterpret_cast<const void*>(pcg_matvec_kernel<2>)
                                        : reinterpret_cast<const void*>(pcg_matvec_kernel<3>);
-    // multigrid V-cycle: z = V(r) on the fine level, kernels skipped once
-    // *gate is set (null gate during the set-up)
-    MgArgs M;
-    M.A = A;
-    M.gate = nullptr;
-    M.src = nullptr;
-    M.dst = nullptr;
-    M.slot = 0;
-    M.init = 0;
-    void* margs[] = {&M};
-    const int dim = h->dim;
-    auto cgrid = [](int cells) { return std::max(1, std::min((cells + 255) / 256, 8192)); };
-    auto mlaunch = [&](const void* fn, int grid) { return timed_launch(h, fn, dim3(grid), blk, margs); };
-    // pre: run the fine pre-smooth (inside the iteration pcg_update_kernel
-    // has already written it)
-    auto vcycle = [&](bool pre) -> int {
-        const int L = h->mgLevels;
-        int r = pre ? mlaunch(by_dim(dim, mg_pre_fine_kernel<1>, mg_pre_fine_kernel<2>, mg_pre_fine_kernel<3>), nb)
-                    : SIFV_OK;
-        M.C = h->mg[0];
-        if (!r)
-            r = mlaunch(by_dim(dim, mg_restrict_fine_kernel<1>, mg_restrict_fine_kernel<2>,
-                               mg_restrict_fine_kernel<3>),
-                        cgrid(M.C.cells));
-        for (int l = 0; l + 1 < L && !r; ++l) {
-            M.C = h->mg[l];
-            r = mlaunch(by_dim(dim, mg_pre_kernel<1>, mg_pre_kernel<2>, mg_pre_kernel<3>), cgrid(M.C.cells));
-            if (!r) {
-                M.F = h->mg[l];
-                M.C = h->mg[l + 1];
-                r = mlaunch(by_dim(dim, mg_restrict_kernel<1>, mg_restrict_kernel<2>, mg_restrict_kernel<3>),
-                            cgrid(M.C.cells));
+    // multigrid preconditioner: FP64, or FP32 levels in the mixed form
+    const bool mg32 = mg && h->jacMixed;
+    MgCycle<double> V64;
+    MgCycle<float> V32;
+    int rc = SIFV_OK;
+    if (mg32) {
+        for (int q = 0; q < 6 && !rc; ++q)
+            if (!h->pcg32[q] && (q >= 3 || q < h->dim)) {
+                HIPCHK(hipMalloc(&h->pcg32[q], sizeof(float) * h->total));
+                HIPCHK(hipMemset(h->pcg32[q], 0, sizeof(float) * h->total));
             }
-        }
-        // coarsest: pre-smooth and all sweeps in one workgroup, answer in t
-        M.C = h->mg[L - 1];
-        if (!r)
-            r = timed_launch(h, by_dim(dim, mg_coarsest_kernel<1>, mg_coarsest_kernel<2>, mg_coarsest_kernel<3>),
-                             dim3(1), dim3(kMgCoarsestCells), margs);
-        for (int l = L - 2; l >= 0 && !r; --l) {
-            M.F = h->mg[l];
-            M.C = h->mg[l + 1];
-            r = mlaunch(reinterpret_cast<const void*>(mg_prolong_kernel), cgrid(M.F.cells));
-            if (!r) {
-                M.C = h->mg[l];
-                M.src = M.C.x;
-                M.dst = M.C.t;
-                r = mlaunch(by_dim(dim, mg_smooth_kernel<1>, mg_smooth_kernel<2>, mg_smooth_kernel<3>),
-                            cgrid(M.C.cells));
-            }
-        }
-        // fine level: prolongation folded into the post-smooth
-        M.C = h->mg[0];
-        if (!r) r = mlaunch(by_dim(dim, mg_post_fine_kernel<1>, mg_post_fine_kernel<2>, mg_post_fine_kernel<3>), nb);
-        return r;
-    };
-    int rc = launch(by_dim(h->dim, pcg_weights_kernel<1>, pcg_weights_kernel<2>, pcg_weights_kernel<3>), nb);
+        A.q32 = h->pcg32[5];
+        rc = ensure_mg(h, true);
+        MgFine32 F32;
+        for (int d = 0; d < 3; ++d) F32.W[d] = h->pcg32[d];
+        F32.s = h->pcg32[3];
+        F32.d = h->pcg32[4];
+        F32.q = h->pcg32[5];
+        V32 = MgCycle<float>{h, MgArgsT<float>{}, h->mg32, h->mgLevels32, nb, h->dim};
+        V32.M.f32 = F32;
+    } else if (mg) {
+        rc = ensure_mg(h, false);
+        V64 = MgCycle<double>{h, MgArgs{}, h->mg, h->mgLevels, nb, h->dim};
+    }
+    V64.M.A = A;
+    V32.M.A = A;
+    if (!rc) rc = launch(by_dim(h->dim, pcg_weights_kernel<1>, pcg_weights_kernel<2>, pcg_weights_kernel<3>), nb);
     if (!rc) rc = launch(setup, nb);
-    if (!rc && mg) {
-        rc = ensure_mg(h);
-        for (int l = 0; l < h->mgLevels && !rc; ++l) {
-            M.C = h->mg[l];
-            if (l == 0) {
-                rc = mlaunch(by_dim(dim, mg_build_kernel<1, true>, mg_build_kernel<2, true>, mg_build_kernel<3, true>),
-                             cgrid(M.C.cells));
-            } else {
-                M.F = h->mg[l - 1];
-                rc = mlaunch(
-                    by_dim(dim, mg_build_kernel<1, false>, mg_build_kernel<2, false>, mg_build_kernel<3, false>),
-                    cgrid(M.C.cells));
-            }
-            if (!rc) rc = mlaunch(by_dim(dim, mg_diag_kernel<1>, mg_diag_kernel<2>, mg_diag_kernel<3>), cgrid(M.C.cells));
-        }
-        M.slot = 2;
-        M.init = 1;
-        if (!rc) rc = vcycle(true);
-        M.gate = A.st;
-        M.slot = 1;
-        M.init = 0;
+    if (!rc && mg32) {
+        MgFine32 F32 = V32.M.f32;
+        void* cargs[] = {&A, &F32, (void*)&h->dim, (void*)&h->total};
+        rc = timed_launch(h, reinterpret_cast<const void*>(mg_fine32_kernel), dim3(nb), blk, cargs);
+        if (!rc) rc = V32.build();
+        if (!rc) rc = V32.first();
+    } else if (!rc && mg) {
+        rc = V64.build();
+        if (!rc) rc = V64.first();
     }
     if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_setup_fold_kernel), 1);
     if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_zero_kernel), nb);
@@ -1278,7 +1351,8 @@ int run_pcg(sifv_handle* h, const double* rho, const double* dt, double tol, int
             rc = launch(matvec, nb);
             if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_alpha_kernel), 1);
             if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_update_kernel), nb);
-            if (!rc && mg) rc = vcycle(false);
+            if (!rc && mg32) rc = V32.run(false);
+            else if (!rc && mg) rc = V64.run(false);
             if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_beta_kernel), 1);
             if (!rc) rc = launch(reinterpret_cast<const void*>(pcg_dir_kernel), nb);
             if (rc) return rc;
@@ -1907,6 +1981,8 @@ int sifv_destroy(sifv_handle* h) {
     for (int q = 0; q < 7; ++q) hipFree(h->pcgv[q]);
     for (int d = 0; d < 3; ++d) hipFree(h->pcgW[d]);
     hipFree(h->mgBuf);
+    hipFree(h->mgBuf32);
+    for (int q = 0; q < 6; ++q) hipFree(h->pcg32[q]);
     hipFree(h->pcgSc);
     hipFree(h->pcgSt);
     if (h->pcgStHost) hipHostFree(h->pcgStHost);
@@ -2577,8 +2653,9 @@ int sifv_set_pressure_solver(sifv_handle* h, int kind) {
     if (kind != SIFV_PS_JACOBI && !h->pcgSym)
         return fail(SIFV_ERR_UNSUPPORTED,
                     "PCG needs a symmetric system: a periodic axis with stretched wrap widths (use Jacobi)");
-    if (kind != SIFV_PS_JACOBI && h->jacMixed)
-        return fail(SIFV_ERR_UNSUPPORTED, "the PCG pressure solve runs in FP64 (no mixed-precision form)");
+    if (kind == SIFV_PS_PCG && h->jacMixed)
+        return fail(SIFV_ERR_UNSUPPORTED,
+                    "the diagonal PCG pressure solve runs in FP64 (no mixed-precision form; use SIFV_PS_MGPCG)");
     h->psolver = kind;
     return SIFV_OK;
 }

@@ -58,6 +58,7 @@ struct PcgArgs {
     double tol2;           // tol^2
     int maxIter;
     double mgOmega = 0.0;  // > 0: multigrid form, the update writes the V-cycle's pre-smooth q = om r / d
+    float* q32 = nullptr;  // multigrid form with an FP32 V-cycle: the pre-smooth goes here, rounded
 };
 
 // Interior index of the neighbour across the low (side 0) / high (side 1)
@@ -138,27 +139,32 @@ __device__ __forceinline__ int pcg_nbc(const Grid& g, int dir, int side, int c) 
 
 // pcg_row with the vector given by val(index, i, j, k) at the cell and its
 // neighbours instead of an array (a vector the kernel forms on the fly from
-// the arrays it is made of); same operand order as pcg_row.  vi: the value
-// at the cell.
-template <int DIM, typename V>
-__device__ __forceinline__ double pcg_row_f(const PcgArgs& A, V&& val, double si, int i, int j, int k, size_t id,
-                                            double& vi) {
-    const Grid& g = A.g;
+// the arrays it is made of), over weights W and mass term si of type R, in
+// R arithmetic; same operand order as pcg_row.  vi: the value at the cell.
+template <int DIM, typename R, typename V>
+__device__ __forceinline__ R face_row(const Grid& g, const R* const* W, V&& val, R si, int i, int j, int k, size_t id,
+                                      R& vi) {
     vi = val(id, i, j, k);
-    double acc = si * vi;
+    R acc = si * vi;
 #pragma unroll
     for (int dir = 0; dir < DIM; ++dir) {
         const size_t stride = dir == 0 ? 1 : dir == 1 ? (size_t)g.nxT : (size_t)g.nxT * g.nyT;
-        const double wl = A.W[dir][id], wh = A.W[dir][id + stride];
+        const R wl = W[dir][id], wh = W[dir][id + stride];
         const int c = dir == 0 ? i : dir == 1 ? j : k;
         const int cl = pcg_nbc(g, dir, 0, c), ch = pcg_nbc(g, dir, 1, c);
         const size_t nl = pcg_nb(g, dir, 0, i, j, k, id, stride), nh = pcg_nb(g, dir, 1, i, j, k, id, stride);
-        const double vl = dir == 0 ? val(nl, cl, j, k) : dir == 1 ? val(nl, i, cl, k) : val(nl, i, j, cl);
-        const double vh = dir == 0 ? val(nh, ch, j, k) : dir == 1 ? val(nh, i, ch, k) : val(nh, i, j, ch);
+        const R vl = dir == 0 ? val(nl, cl, j, k) : dir == 1 ? val(nl, i, cl, k) : val(nl, i, j, cl);
+        const R vh = dir == 0 ? val(nh, ch, j, k) : dir == 1 ? val(nh, i, ch, k) : val(nh, i, j, ch);
         acc += wl * (vi - vl);
         acc += wh * (vi - vh);
     }
     return acc;
+}
+
+template <int DIM, typename V>
+__device__ __forceinline__ double pcg_row_f(const PcgArgs& A, V&& val, double si, int i, int j, int k, size_t id,
+                                            double& vi) {
+    return face_row<DIM, double>(A.g, A.W, val, si, i, j, k, id, vi);
 }
 
 // Traversal of the interior by bricks of 64 columns x 4 rows x kBrickPlanes
@@ -380,7 +386,9 @@ __global__ void __launch_bounds__(256) pcg_update_kernel(PcgArgs A) {
         if (A.mgOmega > 0.0) {
             // the preconditioner's first fine-level pass (mg_pre_fine_kernel);
             // q is free again until the next product
-            A.q[id] = A.mgOmega * r / A.d[id];
+            const double qv = A.mgOmega * r / A.d[id];
+            if (A.q32) A.q32[id] = (float)qv;
+            else A.q[id] = qv;
         } else {
             const double z = r / A.d[id];
             A.z[id] = z;
@@ -456,34 +464,57 @@ __host__ __device__ constexpr double mg_omega(int dim) {
     return dim == 1 ? 2.0 / 3.0 : dim == 2 ? 0.8 : 6.0 / 7.0;
 }
 
-struct MgLevel {
+// Mixed-precision form (R = float): the V-cycle runs on FP32 copies of the
+// fine operator (face weights, mass term, diagonal: rounded once per solve),
+// an FP32 pre-smoothed vector written by the CG update and FP32 coarse levels,
+// in FP32 arithmetic, while the conjugate-gradient recurrences, the product,
+// the dot products and the stopping test stay FP64.  The preconditioner is
+// then the V-cycle of the rounded operator evaluated in FP32: symmetric and
+// positive definite up to the FP32 rounding of each application.  Its
+// fine-level passes move 28 (restrict) and 40 (post) B per cell instead of 48
+// and 64.
+template <typename R>
+struct MgLevelT {
     int n[3];
     int cells;
-    double *s, *w[3], *d, *b, *x, *t;
+    R *s, *w[3], *d, *b, *x, *t;
+};
+using MgLevel = MgLevelT<double>;
+
+// FP32 copies of the fine level (padded fields like PcgArgs'): W, s, d and the
+// pre-smoothed q (null in the FP64 form).
+struct MgFine32 {
+    float* W[3];
+    float *s, *d, *q;
 };
 
-struct MgArgs {
+template <typename R>
+struct MgArgsT {
     PcgArgs A;            // fine level (padded fields)
-    MgLevel F;            // finer compact level (unused when fine is the padded grid)
-    MgLevel C;            // this / the coarser compact level
+    MgLevelT<R> F;        // finer compact level (unused when fine is the padded grid)
+    MgLevelT<R> C;        // this / the coarser compact level
     const int* gate;      // skip when *gate != 0 (solve finished); null: always run
-    const double* src;    // smoothing source
-    double* dst;          // smoothing destination
+    const R* src;         // smoothing source
+    R* dst;               // smoothing destination
     int slot;             // partial-sum slot of the r.z sums (mg_post_fine_kernel)
     int init;             // mg_post_fine_kernel: also p = z
+    MgFine32 f32;         // R = float: the FP32 fine level
 };
+using MgArgs = MgArgsT<double>;
 
-__device__ __forceinline__ bool mg_skip(const MgArgs& M) { return M.gate && *M.gate; }
+template <typename R>
+__device__ __forceinline__ bool mg_skip(const MgArgsT<R>& M) { return M.gate && *M.gate; }
 
-__device__ __forceinline__ int mg_idx(const MgLevel& L, int i, int j, int k) {
+template <typename R>
+__device__ __forceinline__ int mg_idx(const MgLevelT<R>& L, int i, int j, int k) {
     return i + L.n[0] * (j + L.n[1] * k);
 }
 
-template <int DIM>
-__device__ __forceinline__ double mg_row(const MgLevel& L, const double* v, int i, int j, int k) {
+template <int DIM, typename R>
+__device__ __forceinline__ R mg_row(const MgLevelT<R>& L, const R* v, int i, int j, int k) {
     const int id = mg_idx(L, i, j, k);
-    const double vi = v[id];
-    double acc = L.s[id] * vi;
+    const R vi = v[id];
+    R acc = L.s[id] * vi;
 #pragma unroll
     for (int dir = 0; dir < DIM; ++dir) {
         const int n = L.n[dir];
@@ -497,17 +528,29 @@ __device__ __forceinline__ double mg_row(const MgLevel& L, const double* v, int 
     return acc;
 }
 
-__device__ __forceinline__ void mg_coords(const MgLevel& L, int c, int& i, int& j, int& k) {
+template <typename R>
+__device__ __forceinline__ void mg_coords(const MgLevelT<R>& L, int c, int& i, int& j, int& k) {
     i = c % L.n[0];
     const int r = c / L.n[0];
     j = r % L.n[1];
     k = r / L.n[1];
 }
 
-// Coarse operator of level C from the fine grid (F unused) or from level F.
-template <int DIM, bool FROM_FINE>
-__global__ void __launch_bounds__(256) mg_build_kernel(MgArgs M) {
-    const MgLevel& C = M.C;
+// The fine FP64 operator rounded to FP32 (every padded slot, so the high
+// boundary weights in the slot after each line come along).
+__global__ void __launch_bounds__(256) mg_fine32_kernel(PcgArgs A, MgFine32 F, int dim, size_t n) {
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        F.s[c] = (float)A.s[c];
+        F.d[c] = (float)A.d[c];
+        for (int d = 0; d < dim; ++d) F.W[d][c] = (float)A.W[d][c];
+    }
+}
+
+// Coarse operator of level C from the fine grid (F unused) or from level F,
+// summed in FP64 and stored in R.
+template <int DIM, bool FROM_FINE, typename R>
+__global__ void __launch_bounds__(256) mg_build_kernel(MgArgsT<R> M) {
+    const MgLevelT<R>& C = M.C;
     const Grid& g = M.A.g;
     const int nf0 = FROM_FINE ? g.nx : M.F.n[0], nf1 = FROM_FINE ? g.ny : M.F.n[1], nf2 = FROM_FINE ? g.nz : M.F.n[2];
     const double f0 = nf0 != C.n[0] ? 0.5 : 1.0, f1 = nf1 != C.n[1] ? 0.5 : 1.0, f2 = nf2 != C.n[2] ? 0.5 : 1.0;
@@ -532,16 +575,16 @@ __global__ void __launch_bounds__(256) mg_build_kernel(MgArgs M) {
                         if (DIM >= 3 && k == 2 * ck) w2 += M.F.w[2][f];
                     }
                 }
-        C.s[c] = ss;
-        C.w[0][c] = f0 * w0;
-        C.w[1][c] = DIM >= 2 ? f1 * w1 : 0.0;
-        C.w[2][c] = DIM >= 3 ? f2 * w2 : 0.0;
+        C.s[c] = (R)ss;
+        C.w[0][c] = (R)(f0 * w0);
+        C.w[1][c] = (R)(DIM >= 2 ? f1 * w1 : 0.0);
+        C.w[2][c] = (R)(DIM >= 3 ? f2 * w2 : 0.0);
     }
 }
 
-template <int DIM>
-__global__ void __launch_bounds__(256) mg_diag_kernel(MgArgs M) {
-    const MgLevel& C = M.C;
+template <int DIM, typename R>
+__global__ void __launch_bounds__(256) mg_diag_kernel(MgArgsT<R> M) {
+    const MgLevelT<R>& C = M.C;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C.cells; c += gridDim.x * blockDim.x) {
         int i, j, k;
         mg_coords(C, c, i, j, k);
@@ -555,13 +598,13 @@ __global__ void __launch_bounds__(256) mg_diag_kernel(MgArgs M) {
             dd += C.w[dir][c];
             dd += C.w[dir][up];
         }
-        C.d[c] = dd;
+        C.d[c] = (R)dd;
     }
 }
 
-// fine pre-smooth: q = om r / d
-template <int DIM>
-__global__ void __launch_bounds__(256) mg_pre_fine_kernel(MgArgs M) {
+// fine pre-smooth: q = om r / d (FP32 form: rounded into the FP32 q)
+template <int DIM, typename R>
+__global__ void __launch_bounds__(256) mg_pre_fine_kernel(MgArgsT<R> M) {
     if (mg_skip(M)) return;
     const PcgArgs& A = M.A;
     const Grid& g = A.g;
@@ -569,26 +612,35 @@ __global__ void __launch_bounds__(256) mg_pre_fine_kernel(MgArgs M) {
     int i, j, k;
     for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
-        A.q[id] = om * A.r[id] / A.d[id];
+        const double qv = om * A.r[id] / A.d[id];
+        if constexpr (sizeof(R) == 4) M.f32.q[id] = (float)qv;
+        else A.q[id] = qv;
     }
 }
 
 // coarse right-hand side from the fine residual r - S q
-template <int DIM>
-__global__ void __launch_bounds__(256) mg_restrict_fine_kernel(MgArgs M) {
+template <int DIM, typename R>
+__global__ void __launch_bounds__(256) mg_restrict_fine_kernel(MgArgsT<R> M) {
     if (mg_skip(M)) return;
     const PcgArgs& A = M.A;
     const Grid& g = A.g;
-    const MgLevel& C = M.C;
+    const MgLevelT<R>& C = M.C;
+    const float* __restrict__ q32 = M.f32.q;
+    auto q32v = [&](size_t id, int, int, int) __attribute__((always_inline)) { return q32[id]; };
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C.cells; c += gridDim.x * blockDim.x) {
         int ci, cj, ck;
         mg_coords(C, c, ci, cj, ck);
-        double acc = 0.0;
+        R acc = R(0);
         for (int k = 2 * ck; k < 2 * ck + 2 && k < g.nz; ++k)
             for (int j = 2 * cj; j < 2 * cj + 2 && j < g.ny; ++j)
                 for (int i = 2 * ci; i < 2 * ci + 2 && i < g.nx; ++i) {
                     const size_t id = cell_index(g, i, j, k);
-                    acc += A.r[id] - pcg_row<DIM>(A, A.q, A.s[id], i, j, k, id);
+                    if constexpr (sizeof(R) == 4) {
+                        float qi;
+                        acc += (float)A.r[id] - face_row<DIM, float>(g, M.f32.W, q32v, M.f32.s[id], i, j, k, id, qi);
+                    } else {
+                        acc += A.r[id] - pcg_row<DIM>(A, A.q, A.s[id], i, j, k, id);
+                    }
                 }
         C.b[c] = acc;
     }
@@ -599,26 +651,37 @@ __global__ void __launch_bounds__(256) mg_restrict_fine_kernel(MgArgs M) {
 // neighbours where the product needs it, so q' is never stored),
 // z = q' + om (r - S q') / d, with the partial sums of r.z into slot M.slot
 // (and p = z when M.init)
-template <int DIM>
-__global__ void __launch_bounds__(256) mg_post_fine_kernel(MgArgs M) {
+template <int DIM, typename R>
+__global__ void __launch_bounds__(256) mg_post_fine_kernel(MgArgsT<R> M) {
     if (mg_skip(M)) return;
     __shared__ double sh[256 / kWave];
     const PcgArgs& A = M.A;
     const Grid& g = A.g;
-    const MgLevel& C = M.C;
+    const MgLevelT<R>& C = M.C;
     constexpr double om = mg_omega(DIM);
-    const double* __restrict__ q = A.q;
-    const double* __restrict__ tc = C.t;
-    auto qp = [&](size_t id, int i, int j, int k) __attribute__((always_inline)) {
-        return q[id] + tc[mg_idx(C, i >> 1, j >> 1, k >> 1)];
-    };
+    const R* __restrict__ tc = C.t;
     double rz = 0.0;
     auto cell = [&](int i, int j, int k) {
         const size_t id = cell_index(g, i, j, k);
-        double qi;
-        const double a = pcg_row_f<DIM>(A, qp, A.s[id], i, j, k, id, qi);
         const double r = A.r[id];
-        const double z = qi + om * (r - a) / A.d[id];
+        double z;
+        if constexpr (sizeof(R) == 4) {
+            const float* __restrict__ q = M.f32.q;
+            auto qp = [&](size_t n, int a, int b, int c) __attribute__((always_inline)) {
+                return q[n] + tc[mg_idx(C, a >> 1, b >> 1, c >> 1)];
+            };
+            float qi;
+            const float a = face_row<DIM, float>(g, M.f32.W, qp, M.f32.s[id], i, j, k, id, qi);
+            z = (double)(qi + (float)om * ((float)r - a) / M.f32.d[id]);
+        } else {
+            const double* __restrict__ q = A.q;
+            auto qp = [&](size_t n, int a, int b, int c) __attribute__((always_inline)) {
+                return q[n] + tc[mg_idx(C, a >> 1, b >> 1, c >> 1)];
+            };
+            double qi;
+            const double a = pcg_row_f<DIM>(A, qp, A.s[id], i, j, k, id, qi);
+            z = qi + om * (r - a) / A.d[id];
+        }
         A.z[id] = z;
         rz += r * z;
         if (M.init) A.p[id] = z;
@@ -640,32 +703,63 @@ __global__ void __launch_bounds__(256) mg_post_fine_kernel(MgArgs M) {
     }
 }
 
-// coarse pre-smooth: x = om b / d
-template <int DIM>
-__global__ void __launch_bounds__(256) mg_pre_kernel(MgArgs M) {
-    if (mg_skip(M)) return;
-    const MgLevel& C = M.C;
-    constexpr double om = mg_omega(DIM);
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C.cells; c += gridDim.x * blockDim.x)
-        C.x[c] = om * C.b[c] / C.d[c];
+// Compact-level passes over the cells c0, c0 + cs, ...: the bodies of the
+// per-level kernels below and of mg_tail_kernel.
+// pre-smooth: x = om b / d
+template <int DIM, typename R>
+__device__ __forceinline__ void mg_pre_cells(const MgLevelT<R>& C, int c0, int cs) {
+    constexpr R om = (R)mg_omega(DIM);
+    for (int c = c0; c < C.cells; c += cs) C.x[c] = om * C.b[c] / C.d[c];
 }
 
 // coarser right-hand side from level F's residual b - A x
-template <int DIM>
-__global__ void __launch_bounds__(256) mg_restrict_kernel(MgArgs M) {
-    if (mg_skip(M)) return;
-    const MgLevel& F = M.F;
-    const MgLevel& C = M.C;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C.cells; c += gridDim.x * blockDim.x) {
+template <int DIM, typename R>
+__device__ __forceinline__ void mg_restrict_cells(const MgLevelT<R>& F, const MgLevelT<R>& C, int c0, int cs) {
+    for (int c = c0; c < C.cells; c += cs) {
         int ci, cj, ck;
         mg_coords(C, c, ci, cj, ck);
-        double acc = 0.0;
+        R acc = R(0);
         for (int k = 2 * ck; k < 2 * ck + 2 && k < F.n[2]; ++k)
             for (int j = 2 * cj; j < 2 * cj + 2 && j < F.n[1]; ++j)
                 for (int i = 2 * ci; i < 2 * ci + 2 && i < F.n[0]; ++i)
-                    acc += F.b[mg_idx(F, i, j, k)] - mg_row<DIM>(F, F.x, i, j, k);
+                    acc += F.b[mg_idx(F, i, j, k)] - mg_row<DIM, R>(F, F.x, i, j, k);
         C.b[c] = acc;
     }
+}
+
+// level F: x += t(parent) from level C
+template <typename R>
+__device__ __forceinline__ void mg_prolong_cells(const MgLevelT<R>& F, const MgLevelT<R>& C, int f0, int fs) {
+    for (int f = f0; f < F.cells; f += fs) {
+        int i, j, k;
+        mg_coords(F, f, i, j, k);
+        F.x[f] += C.t[mg_idx(C, i >> 1, j >> 1, k >> 1)];
+    }
+}
+
+// dst = src + om (b - A src) / d on level C
+template <int DIM, typename R>
+__device__ __forceinline__ void mg_smooth_cells(const MgLevelT<R>& C, const R* src, R* dst, int c0, int cs) {
+    constexpr R om = (R)mg_omega(DIM);
+    for (int c = c0; c < C.cells; c += cs) {
+        int i, j, k;
+        mg_coords(C, c, i, j, k);
+        dst[c] = src[c] + om * (C.b[c] - mg_row<DIM, R>(C, src, i, j, k)) / C.d[c];
+    }
+}
+
+// coarse pre-smooth: x = om b / d
+template <int DIM, typename R>
+__global__ void __launch_bounds__(256) mg_pre_kernel(MgArgsT<R> M) {
+    if (mg_skip(M)) return;
+    mg_pre_cells<DIM, R>(M.C, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+// coarser right-hand side from level F's residual b - A x
+template <int DIM, typename R>
+__global__ void __launch_bounds__(256) mg_restrict_kernel(MgArgsT<R> M) {
+    if (mg_skip(M)) return;
+    mg_restrict_cells<DIM, R>(M.F, M.C, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // Coarsest level (at most 4 cells per axis, so at most kMgCoarsestCells):
@@ -674,16 +768,15 @@ __global__ void __launch_bounds__(256) mg_restrict_kernel(MgArgs M) {
 // the same arithmetic as mg_pre_kernel followed by mg_smooth_kernel launches.
 constexpr int kMgCoarsestCells = 64;
 
-template <int DIM>
-__global__ void __launch_bounds__(kMgCoarsestCells) mg_coarsest_kernel(MgArgs M) {
-    if (mg_skip(M)) return;
-    __shared__ double xs[kMgCoarsestCells], ts[kMgCoarsestCells];
-    const MgLevel& C = M.C;
-    constexpr double om = mg_omega(DIM);
+// Called by every thread of the workgroup (blockDim.x >= C.cells).
+template <int DIM, typename R>
+__device__ __forceinline__ void mg_coarsest_body(const MgLevelT<R>& C) {
+    __shared__ R xs[kMgCoarsestCells], ts[kMgCoarsestCells];
+    constexpr R om = (R)mg_omega(DIM);
     const int c = threadIdx.x;
     const bool on = c < C.cells;
     int i = 0, j = 0, k = 0;
-    double b = 0.0, d = 1.0;
+    R b = R(0), d = R(1);
     if (on) {
         mg_coords(C, c, i, j, k);
         b = C.b[c];
@@ -692,9 +785,9 @@ __global__ void __launch_bounds__(kMgCoarsestCells) mg_coarsest_kernel(MgArgs M)
     }
     __syncthreads();
     for (int sw = 1; sw < kMgCoarseSweeps; ++sw) {
-        const double* src = (sw & 1) ? xs : ts;
-        double* dst = (sw & 1) ? ts : xs;
-        if (on) dst[c] = src[c] + om * (b - mg_row<DIM>(C, src, i, j, k)) / d;
+        const R* src = (sw & 1) ? xs : ts;
+        R* dst = (sw & 1) ? ts : xs;
+        if (on) dst[c] = src[c] + om * (b - mg_row<DIM, R>(C, src, i, j, k)) / d;
         __syncthreads();
     }
     if (on) {
@@ -704,29 +797,58 @@ __global__ void __launch_bounds__(kMgCoarsestCells) mg_coarsest_kernel(MgArgs M)
 }
 
 // level F: x += t(parent) from level C
-__global__ void __launch_bounds__(256) mg_prolong_kernel(MgArgs M) {
+template <typename R>
+__global__ void __launch_bounds__(256) mg_prolong_kernel(MgArgsT<R> M) {
     if (mg_skip(M)) return;
-    const MgLevel& F = M.F;
-    const MgLevel& C = M.C;
-    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < F.cells; f += gridDim.x * blockDim.x) {
-        int i, j, k;
-        mg_coords(F, f, i, j, k);
-        F.x[f] += C.t[mg_idx(C, i >> 1, j >> 1, k >> 1)];
-    }
+    mg_prolong_cells<R>(M.F, M.C, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // dst = src + om (b - A src) / d on level C
-template <int DIM>
-__global__ void __launch_bounds__(256) mg_smooth_kernel(MgArgs M) {
+template <int DIM, typename R>
+__global__ void __launch_bounds__(256) mg_smooth_kernel(MgArgsT<R> M) {
     if (mg_skip(M)) return;
-    const MgLevel& C = M.C;
-    constexpr double om = mg_omega(DIM);
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C.cells; c += gridDim.x * blockDim.x) {
-        int i, j, k;
-        mg_coords(C, c, i, j, k);
-        M.dst[c] = M.src[c] + om * (C.b[c] - mg_row<DIM>(C, M.src, i, j, k)) / C.d[c];
-    }
+    mg_smooth_cells<DIM, R>(M.C, M.src, M.dst, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
+// The small levels of a V-cycle in one workgroup: from lv[0] (its right-hand
+// side b already restricted) down to the coarsest lv[nlev - 1] and back up to
+// lv[0]'s answer in t, the same passes in the same order as the per-level
+// launches (pre, restrict, ..., coarsest, ..., prolong, smooth), with a
+// workgroup barrier between passes instead of a kernel boundary.  Levels of at
+// most SIFV_MG_TAIL_CELLS cells take this path; they are too small to fill
+// the GPU, so each launch of theirs costs its fixed start-up time.
+#ifndef SIFV_MG_TAIL_CELLS
+#define SIFV_MG_TAIL_CELLS 4096
+#endif
+constexpr int kMgTailThreads = 1024;
+constexpr int kMgTailMaxLevels = 8;
+
+template <typename R>
+struct MgTailArgsT {
+    const int* gate;  // skip when *gate != 0; null: always run
+    int nlev;
+    MgLevelT<R> lv[kMgTailMaxLevels];
+};
+
+template <int DIM, typename R>
+__global__ void __launch_bounds__(kMgTailThreads) mg_tail_kernel(MgTailArgsT<R> T) {
+    if (T.gate && *T.gate) return;
+    const int n = T.nlev;
+    const int c0 = threadIdx.x, cs = blockDim.x;
+    for (int l = 0; l + 1 < n; ++l) {
+        mg_pre_cells<DIM, R>(T.lv[l], c0, cs);
+        __syncthreads();
+        mg_restrict_cells<DIM, R>(T.lv[l], T.lv[l + 1], c0, cs);
+        __syncthreads();
+    }
+    mg_coarsest_body<DIM, R>(T.lv[n - 1]);
+    __syncthreads();
+    for (int l = n - 2; l >= 0; --l) {
+        mg_prolong_cells<R>(T.lv[l], T.lv[l + 1], c0, cs);
+        __syncthreads();
+        mg_smooth_cells<DIM, R>(T.lv[l], T.lv[l].x, T.lv[l].t, c0, cs);
+        __syncthreads();
+    }
+}
 
 }  // namespace sifv
