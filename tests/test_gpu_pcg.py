@@ -299,3 +299,32 @@ def test_mgpcg_mixed_semi_implicit_steps(gpu, oracle_c):
         l1, li = rel_errs(mesh.interior(sg.field(fid)), mesh.interior(so.field(fid)))
         print("  field", fid, l1, li)
         assert l1 <= 1e-8, (fid, l1, li)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_mgpcg_launched_and_workgroup_levels(gpu, oracle_c, precision):
+    """A 40 x 36 x 34 system (y periodic): its first coarse level (20 x 18 x 17,
+    6120 cells) runs as separate launches and the smaller ones (810, 125, 27
+    cells) in the one-workgroup tail, so both paths and the hand-over between
+    them are exercised; iterations and answer against the oracle's FP64
+    multigrid PCG (the FP32 V-cycle within the bounds of the tests above:
+    measured 66 / 66 iterations FP64, 104 / 66 with the FP32 V-cycle on this
+    stiff system)."""
+    cfg, mesh, rho, rhoc2, rhs = problems.pressure_problem_nd((40, 36, 34), (0, 1, 0), seed=5)
+    dt, tol = 50.0, 1e-7
+    eos = IdealGasEOS(1.4, 1.0, cfg)
+    g = SemiImplicitSolver(mesh, HLLCSolver(eos), MGPCGPressureSolver(), eos, None, cfg, precision=precision)
+    pg = np.zeros(mesh.totalCells())
+    ig = MGPCGPressureSolver(g).solve(mesh, rho, rhoc2, rhs, pg, dt, tol, 3000)
+    g.close()
+    cpu = oracle_c.CpuSolver(mesh, cfg, 2, eos, "c")
+    pc = np.zeros(mesh.totalCells())
+    ic = cpu.mgpcg(rho, rhoc2, rhs, pc, dt, tol, 3000)
+    l1, li = rel_errs(mesh.interior(pg), mesh.interior(pc))
+    print("levels", precision, ig, ic, l1, li)
+    if precision == "fp64":
+        assert 0 < ig < 3000 and abs(ig - ic) <= max(2, ic // 10), (ig, ic)
+        assert l1 <= 1e-9 and li <= 1e-8, (l1, li)
+    else:
+        assert 0 < ig < 3000 and ig <= 1.6 * ic + 3, (ig, ic)
+        assert l1 <= 1e-8 and li <= 1e-7, (l1, li)
