@@ -564,7 +564,8 @@ int launch_dt_cells(sifv_handle* h, bool advective, const double* rho = nullptr,
     int rc = ensure_partials(h, nb);
     if (rc) return rc;
     A.partial = h->partial;
-    hipLaunchKernelGGL(dt_cells_kernel, dim3(nb), dim3(kRedThreads), 0, h->stream, A);
+    void (*fn)(DtArgs) = h->dim == 1 ? dt_cells_kernel<1> : h->dim == 2 ? dt_cells_kernel<2> : dt_cells_kernel<3>;
+    hipLaunchKernelGGL(fn, dim3(nb), dim3(kRedThreads), 0, h->stream, A);
     HIPCHK(hipGetLastError());
     h->npartialValid = nb;
     return SIFV_OK;

@@ -54,19 +54,25 @@ struct DtArgs {
     double* partial;
 };
 
+// DIM fixed at compile time: a cell's velocity loads are issued together
+// (the cell limits and their minimum are the same operations either way).
+template <int DIM>
 __global__ void __launch_bounds__(kRedThreads) dt_cells_kernel(DtArgs A) {
     __shared__ double sh[kRedThreads / kWave];
     const Grid& g = A.g;
-    const size_t n = (size_t)g.nx * g.ny * g.nz;
     double m = 1.0e300;
     int i, j, k;
     for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         size_t id = cell_index(g, i, j, k);
         const double h[3] = {g.dxw[i + g.ngx], g.dyw[j + g.ngy], g.dzw[k + g.ngz]};
+        double v[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) v[d] = A.vel[d][id];
         double dc;
         if (A.advective) {
-            dc = h[0] / dmax_ref(fabs(A.vel[0][id]), 1e-14);
-            for (int d = 1; d < g.dim; ++d) dc = dmin_ref(dc, h[d] / dmax_ref(fabs(A.vel[d][id]), 1e-14));
+            dc = h[0] / dmax_ref(fabs(v[0]), 1e-14);
+#pragma unroll
+            for (int d = 1; d < DIM; ++d) dc = dmin_ref(dc, h[d] / dmax_ref(fabs(v[d]), 1e-14));
         } else {
             double c;
             if (A.mix.nph > 0) {
@@ -80,8 +86,9 @@ __global__ void __launch_bounds__(kRedThreads) dt_cells_kernel(DtArgs A) {
                 double pinf = A.eos.stiffened ? A.eos.pInf : 0.0;
                 c = sqrt(A.eos.gamma * dmax_ref(A.pres[id] + pinf, 1e-14) / dmax_ref(A.rho[id], 1e-14));
             }
-            dc = h[0] / (fabs(A.vel[0][id]) + c);
-            for (int d = 1; d < g.dim; ++d) dc = dmin_ref(dc, h[d] / (fabs(A.vel[d][id]) + c));
+            dc = h[0] / (fabs(v[0]) + c);
+#pragma unroll
+            for (int d = 1; d < DIM; ++d) dc = dmin_ref(dc, h[d] / (fabs(v[d]) + c));
         }
         if (A.viscMu > 0.0) dc = dmin_ref(dc, visc_cell_dt(h, g.dim, A.viscMu, A.rho[id]));
         if (A.capSigma > 0.0) dc = dmin_ref(dc, cap_cell_dt(h, g.dim, A.capSigma, A.rho[id]));
