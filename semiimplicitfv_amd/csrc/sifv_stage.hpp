@@ -485,6 +485,16 @@ __device__ __forceinline__ void add_sources(const StageParams<T>& P, size_t id, 
 // (the stored state is FP32, so the FP64 combination would only add one
 // final rounding back to FP32); the dt of the new state converts in FP32 for
 // an ideal gas and in FP64 for a stiffened gas.
+// Store of the new state in the 1D/2D stage update: non-temporal (the
+// values are next read by the following stage launch, not by this one;
+// measured 0.5 % faster per launch on the 4096^2 vortex); the 3D z pass
+// stores plainly.
+template <int DIM, typename T>
+__device__ __forceinline__ void store_state(T* a, T v) {
+    if constexpr (DIM <= 2) __builtin_nontemporal_store(v, a);
+    else *a = v;
+}
+
 template <int DIM, typename T, bool SRC>
 __device__ __forceinline__ double finish_cell_f32(const StageParams<T>& P, size_t id, T* acc,
                                                   const Cell<DIM, 0, float>& cell, const double* h, double dt,
@@ -513,7 +523,7 @@ __device__ __forceinline__ double finish_cell_f32(const StageParams<T>& P, size_
             if (!P.c4pow2) qt = fmaf(fmaf(-c4, qt, b), c4i, qt);
             un[q] = qt;
         }
-        P.dst[(size_t)q * P.total + id] = (T)un[q];
+        store_state<DIM>(&P.dst[(size_t)q * P.total + id], (T)un[q]);
     }
     if (!wantDt) return 1.0e300;
     if (P.eos.stiffened) {
@@ -617,7 +627,7 @@ __device__ __forceinline__ double finish_cell(const StageParams<T>& P, size_t id
     }
 #pragma unroll
     for (int q = 0; q < NCV; ++q)
-        if (!(SEMI && q == DIM + 1)) P.dst[(size_t)q * P.total + id] = (T)un[q];
+        if (!(SEMI && q == DIM + 1)) store_state<DIM>(&P.dst[(size_t)q * P.total + id], (T)un[q]);
     if constexpr (SEMI) {
         // advected pressure (src/SemiImplicitSolver.cpp:168,185-186); the
         // stage-0 pressure is the conversion of the backup state
