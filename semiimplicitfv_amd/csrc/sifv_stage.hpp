@@ -849,6 +849,15 @@ stage_xy_kernel(StageParams<T> P) {
     RawCell<DIM, NPH, T> nextRow;
     if (YDIR && PF) loadRaw(col, rStart + R, nextRow);
     T preU[DIM + 2 + 2 * NPH], preU0[DIM + 2 + 2 * NPH];
+    // U of the rows between entering the window and their update (R + 1 rows)
+    // kept in LDS, each lane its own column: the update reads the cell's U
+    // from there instead of from HBM (the row entered R iterations earlier).
+    // Single-phase full stages whose update does not preload U (LDS budget:
+    // 8 KB per wave at FP64).
+    constexpr bool RING = YDIR && PF && !PU && MODE == MODE_FULL && NPH == 0;
+    constexpr int RN = R + 1;
+    constexpr int NCVR = DIM + 2 + 2 * NPH;
+    __shared__ T sUr[RING ? RN : 1][RING ? NCVR : 1][kWave];
 
     for (int r = rStart; r < Y1; ++r) {
         T Ghi[NF];
@@ -865,6 +874,11 @@ stage_xy_kernel(StageParams<T> P) {
             }
             if constexpr (PF) {
                 raw_to_cell<DIM, NPH, T>(P, nextRow, win[WS - 1]);
+                if constexpr (RING) {
+                    const int slot = (r + R) % RN;
+#pragma unroll
+                    for (int q = 0; q < NCVR; ++q) sUr[slot][q][lane] = nextRow.U[q];
+                }
                 if (r + 1 < Y1) loadRaw(col, r + 1 + R, nextRow);
             } else {
                 loadCell(col, r + R, win[WS - 1]);
@@ -987,10 +1001,25 @@ stage_xy_kernel(StageParams<T> P) {
                     if (part_slot_used<DIM, NPH, SEMI>(q)) P.part[(size_t)q * P.total + id] = acc[q];
             } else {
                 const double h[3] = {dxc, dyr, dzk};
-                double dc = (YDIR && PU)
-                                ? finish_cell<DIM, NPH, SEMI, T, SRC, C>(P, id, acc, cur, h, dt, P.dtPartial != nullptr,
-                                                                         preU, preU0)
-                                : finish_cell<DIM, NPH, SEMI, T, SRC, C>(P, id, acc, cur, h, dt, P.dtPartial != nullptr);
+                double dc;
+                if constexpr (YDIR && PU) {
+                    dc = finish_cell<DIM, NPH, SEMI, T, SRC, C>(P, id, acc, cur, h, dt, P.dtPartial != nullptr, preU,
+                                                                preU0);
+                } else if constexpr (RING) {
+                    // the segment's first row entered the window before the
+                    // march (not through the ring): read from HBM
+                    T uring[NCVR];
+                    if (r > Y0) {
+#pragma unroll
+                        for (int q = 0; q < NCVR; ++q) uring[q] = sUr[r % RN][q][lane];
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < NCVR; ++q) uring[q] = P.src[(size_t)q * P.total + id];
+                    }
+                    dc = finish_cell<DIM, NPH, SEMI, T, SRC, C>(P, id, acc, cur, h, dt, P.dtPartial != nullptr, uring);
+                } else {
+                    dc = finish_cell<DIM, NPH, SEMI, T, SRC, C>(P, id, acc, cur, h, dt, P.dtPartial != nullptr);
+                }
                 dtMin = dmin_ref(dtMin, dc);
             }
         }
