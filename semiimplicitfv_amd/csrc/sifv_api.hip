@@ -435,7 +435,9 @@ int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const d
     // FP32 WENO of a stiffened gas: indicators scaled before squaring
     P.weno = (h->cfg.reconOrder <= SIFV_WENO5) ? (h->eoff ? 2 : 1) : 0;
     P.sigPrim = h->sigPrim ? 1 : 0;
-    P.pres = h->pres;
+    // semi-implicit stages write the stage-start pressure straight into the
+    // pressure solve's initial guess (the set-up then has no copy to make)
+    P.pres = semi ? h->jac[0] : h->pres;
     for (int d = 0; d < 3; ++d) P.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
     P.pAdv = h->jac[4];
     // semi-implicit stages also leave rho c^2 of the updated state for the
@@ -1415,7 +1417,7 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
             S.ar[ph] = h->nph ? h->consField<double>(dst[s], h->dim + 2 + pp) : nullptr;
             S.al[ph] = h->nph ? h->consField<double>(dst[s], h->dim + 2 + h->nph + pp) : nullptr;
         }
-        S.pres = h->pres;
+        S.pres = h->jac[0];   // the stage update's stage-start pressure
         for (int d = 0; d < 3; ++d) {
             S.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
             S.velOut[d] = h->vel[d] ? h->vel[d] : h->vel[0];
@@ -1460,6 +1462,7 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
         K.sigma = h->cfg.useIGR || h->sigmaUploaded ? h->sigma : nullptr;   // otherwise zero
         for (int d = 0; d < 3; ++d) K.vel[d] = h->vel[d] ? h->vel[d] : h->vel[0];
         K.pres = h->pres;
+        K.storePrim = s == ord - 1 ? 1 : 0;
         K.dt = h->scal;
         hipLaunchKernelGGL(semi_correct_kernel, dim3(nb), dim3(256), 0, h->stream, K);
         HIPCHK(hipGetLastError());

@@ -27,7 +27,7 @@ struct SemiArgs {
     const double* rho;        // star density (ghosts filled)
     const double* ar[kMaxPhases];
     const double* al[kMaxPhases];
-    const double* pres;       // stage-start pressure
+    const double* pres;       // stage-start pressure (p0 itself when the stage update wrote it there)
     const double* vel[3];     // star velocity (ghosts filled) / corrected velocity (out)
     double* velOut[3];
     const double* pAdv;
@@ -56,11 +56,11 @@ __global__ void __launch_bounds__(256) semi_setup_kernel(SemiArgs A) {
     int i, j, k;
     for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
         const size_t id = cell_index(g, i, j, k);
-        const double p = A.pres[id];
         double rc2;
         if (A.rc2Given) {
             rc2 = A.rhoc2[id];
         } else {
+            const double p = A.pres[id];
             const double rho = A.rho[id];
             double c;
             if (A.nph > 0) {
@@ -89,7 +89,8 @@ __global__ void __launch_bounds__(256) semi_setup_kernel(SemiArgs A) {
             div += (fr - fl) / h[d];
         }
         A.rhs[id] = A.pAdv[id] - rc2 * dt * div;
-        A.p0[id] = p;
+        // the initial guess is the stage-start pressure
+        if (A.p0 != A.pres) A.p0[id] = A.pres[id];
     }
 }
 
@@ -200,6 +201,10 @@ struct CorrectArgs {
     const double* sigma;
     double* vel[3];
     double* pres;
+    // 0: the velocity and pressure arrays are left as they are (a stage that
+    // is not the step's last: the next stage's update overwrites both before
+    // anything reads them)
+    int storePrim;
     const double* dt;
 };
 
@@ -243,7 +248,7 @@ __global__ void __launch_bounds__(256) semi_correct_kernel(CorrectArgs A) {
             const double m = mom[id] - dt * (pr - pl) / h[d];
             mom[id] = m;
             u[d] = m / rs;
-            A.vel[d][id] = u[d];
+            if (A.storePrim) A.vel[d][id] = u[d];
         }
         double ke = 0.5 * rho * u[0] * u[0];
         for (int d = 1; d < dim; ++d) ke += 0.5 * rho * u[d] * u[d];
@@ -261,7 +266,7 @@ __global__ void __launch_bounds__(256) semi_correct_kernel(CorrectArgs A) {
             E = (p + A.eos.gamma * pinf) / (A.eos.gamma - 1.0) + ke;
         }
         A.cons[(size_t)(dim + 1) * A.total + id] = E;
-        A.pres[id] = p;
+        if (A.storePrim) A.pres[id] = p;
     }
 }
 
