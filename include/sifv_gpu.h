@@ -259,6 +259,13 @@ int sifv_synchronize(sifv_handle* h);
 /* Launch-shape override for the fused stage kernel (rows marched per wave);
  * 0 restores the built-in choice.  Used by the autotuner. */
 int sifv_set_tile(sifv_handle* h, int rows_per_wave);
+/* 3D stages: the number of slabs of z segments a stage is split into, the x+y
+ * pass of slab k+1 running on the library's second stream beside the z pass of
+ * slab k (1..8; 1 = one launch of each pass).  0 restores the built-in choice
+ * (4 slabs when each slab's z launch keeps >= 8192 waves, else 1).  Results are
+ * the same bits for every value.  No counterpart in the reference (a
+ * launch-shape switch). */
+int sifv_set_stage_slabs(sifv_handle* h, int slabs);
 /* sifv_explicit_advance replays one captured RK step (dt finalize + stage
  * launches) as a HIP graph per step when the step's launch arguments repeat
  * (RK2/RK3 without IGR, timing off), and the 2D/3D Jacobi pressure solve

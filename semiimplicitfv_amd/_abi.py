@@ -133,6 +133,7 @@ def _declare(lib):
         "sifv_timing_read": (i, [vp, P(d), P(C.c_long), i]),
         "sifv_synchronize": (i, [vp]),
         "sifv_set_tile": (i, [vp, i]),
+        "sifv_set_stage_slabs": (i, [vp, i]),
         "sifv_set_graphs": (i, [vp, i]),
         "sifv_graph_replays": (i, [vp, P(C.c_long)]),
         "sifv_write_vtr": (i, [vp, C.c_char_p, i]),

@@ -124,6 +124,9 @@ int validate_config(const sifv_config* c) {
     return SIFV_OK;
 }
 
+// upper bound of the slab count of a pipelined 3D stage (events per handle)
+constexpr int kMaxSlabs = 8;
+
 struct Timing {
     bool on = false;
     std::vector<hipEvent_t> pool;
@@ -150,6 +153,11 @@ struct sifv_handle {
     int precision;
     int nph, nc;
     hipStream_t stream = nullptr;
+    // 3D: the z passes of a stage's slabs run on this stream while the x+y
+    // passes of the following slabs run on `stream` (fork / join by events)
+    hipStream_t side = nullptr;
+    hipEvent_t slabEv[kMaxSlabs] = {};
+    hipEvent_t joinEv = nullptr;
     double *d_dx = nullptr, *d_dy = nullptr, *d_dz = nullptr;
     void* cons[3] = {nullptr, nullptr, nullptr};
     int cur = 0;
@@ -167,6 +175,7 @@ struct sifv_handle {
     bool dtNextValid = false;
     bool primFresh = true;
     int rowsOverride = 0;
+    int slabsOverride = 0;   // 3D stage slabs (0: the built-in choice)
     std::vector<double> nodes[3];   // physical node coordinates (VTK output)
     double* visc = nullptr;         // viscous right-hand side of the stage (dim+1 fields)
     Timing timing;
@@ -410,6 +419,77 @@ int timed_launch(sifv_handle* h, const void* fn, dim3 grid, dim3 block, void** a
 
 // One RK stage of the fused kernel(s).  semi: advective fluxes + star update
 // (writes pres / vel / pAdv) instead of the explicit update.
+// Slabs of z segments per 3D stage: the x+y pass of slab k+1 (VALU-bound)
+// runs beside the z pass of slab k (memory-bound), which needs only slab k's
+// x+y part.  Built-in choice: SIFV_STAGE_SLABS slabs when each slab's z launch
+// still holds kSlabMinWaves waves (four rounds of the chip's 2048 slots at the
+// z pass's two waves per SIMD; the 512^3 water/air bubble: 4 slabs of 8192
+// waves, 86.9 -> 83.3 ms per step; 256^3 stays at one launch, where 4 slabs of
+// 2048 waves measured 0.8 % slower).  One slab (a
+// single launch of each pass on the handle's stream) under stream capture,
+// live kernel timing and the allocation-free entry point.
+#ifndef SIFV_STAGE_SLABS
+#define SIFV_STAGE_SLABS 4
+#endif
+static_assert(SIFV_STAGE_SLABS >= 1 && SIFV_STAGE_SLABS <= kMaxSlabs, "slab count out of range");
+constexpr long kSlabMinWaves = 8192;
+
+int stage_slabs(sifv_handle* h) {
+    if (h->dim != 3 || !h->side || h->noalloc || h->timing.on) return 1;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(h->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 1;
+    int zs, zn, zr;
+    zpass_shape(h, zs, zn, zr);
+    long s = h->slabsOverride;
+    if (s <= 0) s = std::min<long>(SIFV_STAGE_SLABS, (long)zs * zn * h->ny / kSlabMinWaves);
+    return (int)std::max<long>(1, std::min<long>(s, zn));
+}
+
+// Both passes of a 3D stage over `slabs` slabs of z segments.  The x+y
+// launches stay on h->stream in slab order; the z launch of slab k waits for
+// the event recorded after x+y launch k and runs on h->side; h->stream then
+// waits for the last z launch.  The passes of different slabs write disjoint
+// cells (the x+y part, the new state and the semi-implicit outputs of their
+// own planes) and both only read the stage's source state.
+template <typename T>
+int launch_stage_slabs(sifv_handle* h, const StageKernels<T>& k, const StageParams<T>& P, int slabs, bool wantDt,
+                       int nbs) {
+    StageParams<T> Z = P;
+    int zs, zn, zr;
+    zpass_shape(h, zs, zn, zr);
+    Z.rows = zr;
+    Z.nstrip = zs;
+    Z.dtPartial = nullptr;
+    if (wantDt) {
+        int rc = ensure_partials(h, zs * zn * h->ny + nbs);
+        if (rc) return rc;
+    }
+    int zoff = 0;
+    for (int s = 0; s < slabs; ++s) {
+        const int segLo = (int)((long)s * zn / slabs), segHi = (int)((long)(s + 1) * zn / slabs);
+        const int pLo = segLo * zr, pHi = std::min(segHi * zr, h->nz);
+        StageParams<T> X = P;
+        X.kOff = pLo;
+        void* xargs[] = {&X};
+        HIPCHK(hipLaunchKernel(reinterpret_cast<const void*>(k.xy), dim3(P.nstrip * P.nseg * (pHi - pLo)),
+                               dim3(kWave), xargs, 0, h->stream));
+        HIPCHK(hipEventRecord(h->slabEv[s], h->stream));
+        HIPCHK(hipStreamWaitEvent(h->side, h->slabEv[s], 0));
+        StageParams<T> Zs = Z;
+        Zs.segOff = segLo;
+        Zs.nseg = segHi - segLo;
+        const int nbz = zs * Zs.nseg * h->ny;
+        if (wantDt) Zs.dtPartial = h->partial + zoff;
+        zoff += nbz;
+        void* zargs[] = {&Zs};
+        HIPCHK(hipLaunchKernel(reinterpret_cast<const void*>(k.z), dim3(nbz), dim3(kWave), zargs, 0, h->side));
+    }
+    HIPCHK(hipEventRecord(h->joinEv, h->side));
+    HIPCHK(hipStreamWaitEvent(h->stream, h->joinEv, 0));
+    if (wantDt) h->npartialValid = zoff;
+    return SIFV_OK;
+}
+
 template <typename T>
 int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const double* coef, bool rk1, bool last,
                    const int* active, bool semi) {
@@ -469,10 +549,17 @@ int launch_stage_p(sifv_handle* h, const T* srcp, const T* u0p, T* dstp, const d
         P.dtPartial = h->partial;
     }
     void* args[] = {&P};
-    int rc = timed_launch(h, reinterpret_cast<const void*>(k.xy), dim3(nb), dim3(kWave), args);
-    if (rc) return rc;
+    int rc = SIFV_OK;
+    const int slabs = h->dim == 3 ? stage_slabs(h) : 1;
+    if (slabs > 1) {
+        rc = launch_stage_slabs<T>(h, k, P, slabs, wantDt, nbs);
+        if (rc) return rc;
+    } else {
+        rc = timed_launch(h, reinterpret_cast<const void*>(k.xy), dim3(nb), dim3(kWave), args);
+        if (rc) return rc;
+    }
     if (wantDt && h->dim < 3) h->npartialValid = nb;
-    if (h->dim == 3) {
+    if (h->dim == 3 && slabs == 1) {
         StageParams<T> Z = P;
         int zs, zn, zr;
         zpass_shape(h, zs, zn, zr);
@@ -1908,6 +1995,11 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
         if (e_ != hipSuccess) return bail(fail(SIFV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_))); \
     } while (0)
     CK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    if (h->dim == 3) {
+        CK(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+        for (int k = 0; k < kMaxSlabs; ++k) CK(hipEventCreateWithFlags(&h->slabEv[k], hipEventDisableTiming));
+        CK(hipEventCreateWithFlags(&h->joinEv, hipEventDisableTiming));
+    }
     CK(hipMalloc(&h->d_dx, sizeof(double) * h->nxT));
     CK(hipMalloc(&h->d_dy, sizeof(double) * h->nyT));
     CK(hipMalloc(&h->d_dz, sizeof(double) * h->nzT));
@@ -2011,6 +2103,10 @@ int sifv_destroy(sifv_handle* h) {
     if (h->hostResid) hipHostFree(h->hostResid);
     if (h->hostPinned) hipHostFree(h->hostPinned);
     if (h->stream) hipStreamDestroy(h->stream);
+    if (h->side) hipStreamDestroy(h->side);
+    for (int k = 0; k < kMaxSlabs; ++k)
+        if (h->slabEv[k]) hipEventDestroy(h->slabEv[k]);
+    if (h->joinEv) hipEventDestroy(h->joinEv);
     delete h;
     return SIFV_OK;
 }
@@ -2375,6 +2471,13 @@ int sifv_set_tile(sifv_handle* h, int rows_per_wave) {
     if (!h) return fail(SIFV_ERR_INVALID, "null handle");
     if (rows_per_wave < 0 || rows_per_wave > 4096) return fail(SIFV_ERR_INVALID, "rows_per_wave out of range");
     h->rowsOverride = rows_per_wave;
+    return SIFV_OK;
+}
+
+int sifv_set_stage_slabs(sifv_handle* h, int slabs) {
+    if (!h) return fail(SIFV_ERR_INVALID, "null handle");
+    if (slabs < 0 || slabs > kMaxSlabs) return fail(SIFV_ERR_INVALID, "slabs out of range");
+    h->slabsOverride = slabs;
     return SIFV_OK;
 }
 

@@ -121,6 +121,8 @@ struct StageParams {
     int rows;             // rows (y) or planes (z) marched per wave
     int nseg;             // segments per strip
     int nstrip;           // strips per row / plane
+    int kOff = 0;         // 3D x+y pass: first plane of this launch
+    int segOff = 0;       // 3D z pass: first z segment of this launch
     double* pres;         // semi-implicit: stage-start pressure (written)
     double* vel[3];       // semi-implicit: star velocity (written)
     double* pAdv;         // semi-implicit: advected pressure (written)
@@ -766,8 +768,9 @@ stage_xy_kernel(StageParams<T> P) {
     const int lane = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int perPlane = P.nstrip * P.nseg;
-    const int kz = (MODE == MODE_XY) ? blk / perPlane : 0;
-    const int rem = blk - kz * perPlane;
+    const int kzl = (MODE == MODE_XY) ? blk / perPlane : 0;
+    const int rem = blk - kzl * perPlane;
+    const int kz = (MODE == MODE_XY) ? P.kOff + kzl : 0;
     const int strip = rem / P.nseg;
     const int seg = rem - strip * P.nseg;
     const int X0 = strip * kOwn;
@@ -1058,7 +1061,7 @@ stage_z_kernel(StageParams<T> P) {
     const int j = blk / perRow;
     const int rem = blk - j * perRow;
     const int strip = rem / P.nseg;
-    const int seg = rem - strip * P.nseg;
+    const int seg = P.segOff + rem - strip * P.nseg;
     const int col = strip * kWave + lane;
     const int Z0 = seg * P.rows;
     const int Z1 = (Z0 + P.rows < g.nz) ? Z0 + P.rows : g.nz;

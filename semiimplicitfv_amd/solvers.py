@@ -103,6 +103,10 @@ class _GpuSolverBase:
     def set_tile(self, rows: int):
         _abi.check(self._lib.sifv_set_tile(self._h, rows))
 
+    def set_stage_slabs(self, slabs: int):
+        """3D: slabs of z segments a stage is pipelined over (0: built-in)."""
+        _abi.check(self._lib.sifv_set_stage_slabs(self._h, slabs))
+
     def set_graphs(self, on: bool = True):
         """advance() replays each RK step from a captured HIP graph (on=True)
         or enqueues every launch (on=False, the default)."""

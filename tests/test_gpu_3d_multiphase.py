@@ -29,11 +29,13 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 RS = {"hllc": HLLCSolver, "rusanov": RusanovSolver, "lf": LFSolver}
 
 
-def _pair(oracle, cfg, mesh, st, eos, rsolver, nsteps, rows=0):
+def _pair(oracle, cfg, mesh, st, eos, rsolver, nsteps, rows=0, slabs=0):
     sg, sc = st.copy(), st.copy()
     gpu = ExplicitSolver(mesh, rsolver, eos, None, cfg)
     if rows:
         gpu.set_tile(rows)
+    if slabs:
+        gpu.set_stage_slabs(slabs)
     cpu = oracle.CpuSolver(mesh, cfg, rsolver.kind(), eos, "c")
     gpu.upload(sg)
     dts = []
@@ -76,6 +78,22 @@ def test_3d_multi_strip_segments(gpu, oracle_c, rows):
     cfg, mesh, st, eos = make_case(3, (70, 19, 21), rk=2, seed=11, periods=(0, 1, 0), nonuniform=True)
     sg, sc, dts = _pair(oracle_c, cfg, mesh, st, eos, HLLCSolver(eos), 2, rows=rows)
     _check(mesh, sg, sc, dts)
+
+
+@pytest.mark.parametrize("slabs", [2, 3, 8])
+def test_3d_stage_slabs(gpu, oracle_c, slabs):
+    """A stage split into slabs of z segments (x+y pass of the next slab on one
+    stream beside the z pass of this one on another): 7 z segments of 3 planes
+    cut 3+4, 2+2+3 and into 7 one-segment slabs (8 capped at the segment
+    count), RK3 with the fused dt partials.  Same bits as one launch of each
+    pass, and the oracle's bar."""
+    cfg, mesh, st, eos = make_case(3, (70, 19, 21), rk=3, seed=13, periods=(0, 1, 0), nonuniform=True)
+    sg, sc, dts = _pair(oracle_c, cfg, mesh, st, eos, HLLCSolver(eos), 3, rows=3, slabs=slabs)
+    _check(mesh, sg, sc, dts)
+    s1, _, dts1 = _pair(oracle_c, cfg, mesh, st, eos, HLLCSolver(eos), 3, rows=3, slabs=1)
+    assert [d for d, _ in dts] == [d for d, _ in dts1]
+    for fid in sc.field_ids():
+        assert np.array_equal(mesh.interior(sg.field(fid)), mesh.interior(s1.field(fid))), f"field {fid}"
 
 
 def test_3d_periodic_vortex_tube_conservation(gpu):

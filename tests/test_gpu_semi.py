@@ -27,11 +27,13 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 RS = {"hllc": HLLCSolver, "rusanov": RusanovSolver, "lf": LFSolver}
 
 
-def _pair(oracle, cfg, mesh, st, eos, rsolver, nsteps, rows=0):
+def _pair(oracle, cfg, mesh, st, eos, rsolver, nsteps, rows=0, slabs=0):
     sg, sc = st.copy(), st.copy()
     gpu = SemiImplicitSolver(mesh, rsolver, JacobiPressureSolver(), eos, None, cfg)
     if rows:
         gpu.set_tile(rows)
+    if slabs:
+        gpu.set_stage_slabs(slabs)
     cpu = oracle.CpuSolver(mesh, cfg, rsolver.kind(), eos, "c")
     gpu.upload(sg)
     hist = []
@@ -84,6 +86,22 @@ def test_semi_3d_single_phase(gpu, oracle_c):
                                              BoundaryCondition.Symmetry, BoundaryCondition.NoSlipWall, 0, 0])
     sg, sc, hist = _pair(oracle_c, cfg, mesh, st, eos, HLLCSolver(eos), 3)
     _check(mesh, sg, sc, hist)
+
+
+@pytest.mark.parametrize("slabs", [2, 4])
+def test_semi_3d_stage_slabs(gpu, oracle_c, slabs):
+    """Semi-implicit 3D stages split into slabs of z segments (11 planes in
+    segments of 2: 6 segments cut 3+3 and 1+2+1+2): the oracle's bar, and the
+    same bits and Jacobi counts as one launch of each pass."""
+    cfg, mesh, st, eos = make_semi_case(3, (13, 10, 11), seed=2, periods=(0, 0, 1),
+                                        bcs=[BoundaryCondition.SlipWall, BoundaryCondition.Outflow,
+                                             BoundaryCondition.Symmetry, BoundaryCondition.NoSlipWall, 0, 0])
+    sg, sc, hist = _pair(oracle_c, cfg, mesh, st, eos, HLLCSolver(eos), 3, rows=2, slabs=slabs)
+    _check(mesh, sg, sc, hist)
+    s1, _, hist1 = _pair(oracle_c, cfg, mesh, st, eos, HLLCSolver(eos), 3, rows=2, slabs=1)
+    assert [(h[0], h[2]) for h in hist] == [(h[0], h[2]) for h in hist1]
+    for fid in sc.field_ids():
+        assert np.array_equal(mesh.interior(sg.field(fid)), mesh.interior(s1.field(fid))), f"field {fid}"
 
 
 def _check_floor(mesh, sg, sc, ref_fields):
