@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--bubble-precision", default="fp64", choices=["fp64", "mixed"],
                     help="mixed: FP32 Jacobi sweeps + FP64 refinement, or the FP32 V-cycle of the multigrid PCG")
     ap.add_argument("--bubble-stirred", action="store_true", help="the stirred bubble (stiff pressure solves)")
+    ap.add_argument("--stage-slabs", type=int, default=0,
+                    help="3D bubble: slabs of z segments per stage (0: the library's choice, 1: one launch per pass)")
     ap.add_argument("--bubble-graphs", action="store_true",
                     help="replay the Jacobi sweep chunks from captured HIP graphs")
     ap.add_argument("--igr-only", action="store_true",
@@ -339,7 +341,7 @@ def bench_adjoint(n: int = 512, nsteps: int = 400, every: int = 20, ndir: int = 
 
 
 def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred: bool = False,
-                 solver: str = "jacobi", graphs: bool = False):
+                 solver: str = "jacobi", graphs: bool = False, slabs: int = 0):
     """3D water/air bubble (unit cube, n^3 cells, air sphere r = 0.2, alpha
     tanh over 2 cells, alphaMin 1e-8, slip walls, p = 1e5 at rest), semi-
     implicit SSP-RK3 with the Jacobi pressure solve (tol 1e-6, <= 1000
@@ -356,7 +358,8 @@ def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred:
     Jacobi (stop on max |p_new - p| < 1e-6); solver "mgpcg": the same with the
     multigrid V-cycle preconditioner.  graphs: Jacobi chunks replayed from
     captured HIP graphs (captured in the warm-up step) instead of enqueued
-    sweep by sweep."""
+    sweep by sweep.  slabs: slabs of z segments per 3D stage (0: the library's
+    choice)."""
     import numpy as np
 
     from semiimplicitfv_amd import (BoundaryCondition, HLLCSolver, JacobiPressureSolver, MGPCGPressureSolver,
@@ -381,6 +384,8 @@ def bench_bubble(n: int = 512, steps: int = 3, precision: str = "fp64", stirred:
     ps = {"pcg": PCGPressureSolver, "mgpcg": MGPCGPressureSolver}.get(solver, JacobiPressureSolver)()
     g = SemiImplicitSolver(mesh, HLLCSolver(eos), ps, eos, None, cfg, precision=precision)
     g.set_graphs(graphs)
+    if slabs:
+        g.set_stage_slabs(slabs)
     xc, yc, zc = mesh.centroids()
     r = np.sqrt((xc[None, None, :] - 0.5) ** 2 + (yc[None, :, None] - 0.5) ** 2 + (zc[:, None, None] - 0.5) ** 2)
     amin = cfg.multiPhaseParams.alphaMin
@@ -547,7 +552,7 @@ def main():
         return
     if a.bubble_only:
         r = bench_bubble(a.bubble_n, a.steps, precision=a.bubble_precision, stirred=a.bubble_stirred,
-                         solver=a.bubble_solver, graphs=a.bubble_graphs)
+                         solver=a.bubble_solver, graphs=a.bubble_graphs, slabs=a.stage_slabs)
         r.pop("_pres", None)
         print(json.dumps(r))
         return
