@@ -649,6 +649,10 @@ def main():
             if stirred:
                 r32["note"] = ("both solves stop at the 1000-sweep cap unconverged, so pres_rel_L1_vs_fp64 "
                                "compares two different partial iterates, not FP32 rounding")
+            elif np.array_equal(p32, p64):
+                r32["note"] = ("every pressure solve ends within the FP64 opening chunk of 4 sweeps that the mixed "
+                               "form shares with the FP64 solver, so both run the same sweeps (same bits); the FP32 "
+                               "sweeps start on solves that outlast it")
             out[tag] = {"fp64": r64, "fp32_sweeps": r32, "pcg": rcg, "mgpcg": rmg, "mgpcg_fp32_vcycle": rmg32}
         return out
 

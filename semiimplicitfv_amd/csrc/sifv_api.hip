@@ -1030,15 +1030,28 @@ int run_jacobi(sifv_handle* h, const double* rho, const double* dt, double tol, 
 }
 
 // Pressure solve with FP32 Jacobi sweeps and FP64 residual refinement (see
-// jacobi_mix_pass_kernel): an FP64 pass measures the reference's update size
-// max |pn - p| and forms the residual, kInner FP32 sweeps solve for the
-// correction (2 sweeps after the first pass, doubling up to 16, so that a solve
-// the FP64 sweep finishes in a few iterations stays short), repeat.
-// *n counts FP64 passes and FP32 sweeps together, capped at
-// maxIter; *pfinal is the FP64 update of the last pass (the reference's answer
-// when the update size fell below tol).
+// jacobi_mix_pass_kernel).  It opens with the FP64 solver's first chunk of
+// kJacFirstChunk sweeps: a solve that ends there (a flow near rest: 2-3 sweeps)
+// is the FP64 solve, bit for bit, and never pays for the FP32 copy of the
+// system.  From the chunk's last iterate on, an FP64 pass measures the
+// reference's update size max |pn - p| and forms the residual, and FP32 sweeps
+// solve for the correction (2 sweeps after the first pass, doubling up to 16),
+// repeat.  *n counts FP64 sweeps, passes and FP32 sweeps together, capped at
+// maxIter; *pfinal is the last FP64 sweep's output or the FP64 update of the
+// last pass (the reference's answer when the update size fell below tol).
+constexpr int kJacFirstChunk = 4;
+
 int run_jacobi_mixed(sifv_handle* h, const double* rho, const double* dt, double tol, int maxIter, int* n,
                      const double** pfinal) {
+    const int first = std::min(kJacFirstChunk, maxIter);
+    int n64 = 0;
+    int rc0 = run_jacobi(h, rho, dt, tol, first, &n64);
+    if (rc0) return rc0;
+    if (h->hostResid[n64 - 1] < tol || first >= maxIter) {
+        *n = n64;
+        *pfinal = h->jac[n64 % 2];
+        return SIFV_OK;
+    }
     int inner = 2;
     const size_t T = h->total;
     if (!h->jacPn) {
@@ -1096,10 +1109,11 @@ int run_jacobi_mixed(sifv_handle* h, const double* rho, const double* dt, double
     St.iter = 0;
     St.tol = 0.0;
     St.dt = nullptr;
-    double* pin = h->jac[0];
-    double* pout = h->jac[1];
+    // continue from the iterate the FP64 chunk left (sweep k writes jac[(k+1) % 2])
+    double* pin = h->jac[first % 2];
+    double* pout = h->jac[(first + 1) % 2];
     const float* din = nullptr;
-    int total = 0, slot = 0;
+    int total = first, slot = first;
     *n = maxIter;
     while (true) {
         int rc;

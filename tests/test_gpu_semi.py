@@ -252,10 +252,37 @@ def test_mixed_precision_jacobi(gpu, case):
         g.close()
         out.append((s, its))
     (a, ia), (b, ib) = out
+    # every solve outlasts the FP64 opening chunk of 4 sweeps, so the FP32
+    # sweeps and FP64 passes run
+    assert all(x > 4 for x in ia), ia
     assert all(0 < x <= cfg.semiImplicitParams.maxPressureIters for x in ib)
     for fid in a.field_ids():
         l1, li = rel_errs(mesh.interior(b.field(fid)), mesh.interior(a.field(fid)))
         assert l1 <= 1e-8, (fid, l1, li, ia, ib)
+
+
+def test_mixed_precision_short_solves(gpu):
+    """A mixed-precision solve opens with the FP64 solver's first chunk of 4
+    sweeps; the 12^3 water/air bubble at rest converges within it (1-2 sweeps
+    per stage), so its steps are the FP64 steps: the same Jacobi counts and
+    the same bits."""
+    cfg, mesh, st, eos, rs = problems.bubble_3d(12)
+    out = []
+    for prec in ("fp64", "mixed"):
+        g = SemiImplicitSolver(mesh, HLLCSolver(eos), JacobiPressureSolver(), eos, None, cfg, precision=prec)
+        s = st.copy()
+        g.upload(s)
+        its = []
+        for k in range(3):
+            cfg.step = k
+            its.append((g.step_resident(cfg), g.lastPressureIterations()))
+        g.download(s)
+        g.close()
+        out.append((s, its))
+    (a, ia), (b, ib) = out
+    assert ia == ib and all(x <= 4 for _, x in ia), (ia, ib)
+    for fid in a.field_ids():
+        assert np.array_equal(mesh.interior(a.field(fid)), mesh.interior(b.field(fid))), fid
 
 
 @pytest.mark.parametrize("dim,n", [(2, (40, 36)), (3, (16, 14, 12))])
