@@ -96,6 +96,50 @@ def test_3d_stage_slabs(gpu, oracle_c, slabs):
         assert np.array_equal(mesh.interior(sg.field(fid)), mesh.interior(s1.field(fid))), f"field {fid}"
 
 
+def test_3d_stage_slabs_device_loop(gpu):
+    """The device-resident time loop (sifv_explicit_advance: dt, time and the
+    active flag on the device, every launch checking it) with 3D stages in 3
+    slabs: the same times, step counts and bits as one launch of each pass,
+    across calls that end mid-segment at an end time."""
+    cfg, mesh, st, eos = make_case(3, (40, 18, 24), rk=3, seed=17, periods=(1, 0, 1), nonuniform=True)
+    solvers = []
+    for slabs in (3, 1):
+        g = ExplicitSolver(mesh, HLLCSolver(eos), eos, None, cfg)
+        g.set_tile(4)   # 6 z segments of 4 planes: 3 slabs of 2
+        g.set_stage_slabs(slabs)
+        g.upload(st)
+        solvers.append(g)
+    probe = ExplicitSolver(mesh, HLLCSolver(eos), eos, None, cfg)
+    probe.upload(st)
+    t5, _ = probe.advance(cfg, 5, 1e30, 0.0)
+    probe.close()
+    t_end = 3.3 * t5   # after 5 + 9 steps (~2.8 t5) the third call reaches it mid-segment
+    hist = [[], []]
+    for nsteps in (5, 9, 6):
+        for q, g in enumerate(solvers):
+            t0 = hist[q][-1][0] if hist[q] else 0.0
+            hist[q].append(g.advance(cfg, nsteps, t_end, t0))
+    assert abs(hist[0][-1][0] - t_end) <= 1e-12 * t_end and 0 < hist[0][-1][1] < 6, hist
+    assert hist[0] == hist[1], hist
+    for fid in st.field_ids():
+        a, b = solvers[0].download_field(fid), solvers[1].download_field(fid)
+        assert np.array_equal(mesh.interior(a), mesh.interior(b)), fid
+    for g in solvers:
+        g.close()
+
+
+def test_stage_slabs_range(gpu):
+    """sifv_set_stage_slabs takes 0 (the library's choice) to 8 and refuses the rest."""
+    cfg, mesh, st, eos = make_case(3, (16, 12, 10), seed=2)
+    g = ExplicitSolver(mesh, HLLCSolver(eos), eos, None, cfg)
+    for ok in (0, 1, 8):
+        g.set_stage_slabs(ok)
+    for bad in (-1, 9):
+        with pytest.raises(_abi.SifvError):
+            g.set_stage_slabs(bad)
+    g.close()
+
+
 def test_3d_periodic_vortex_tube_conservation(gpu):
     """Fully periodic 3D: mass, momentum and energy conserved to round-off."""
     cfg, mesh, st, eos = make_case(3, (32, 32, 16), periods=(1, 1, 1), shock=False)
