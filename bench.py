@@ -560,7 +560,7 @@ def main():
         print(json.dumps(bench_igr(steps=min(a.steps, 20))))
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "bench.py measures the path on one GPU; runs on several GPUs are not supported",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     res = run_gpu(a.n, a.steps, a.warmup, a.precision, a.rows)
