@@ -465,16 +465,17 @@ int launch_stage_slabs(sifv_handle* h, const StageKernels<T>& k, const StagePara
         if (rc) return rc;
     }
     int zoff = 0;
-    for (int s = 0; s < slabs; ++s) {
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < slabs && e == hipSuccess; ++s) {
         const int segLo = (int)((long)s * zn / slabs), segHi = (int)((long)(s + 1) * zn / slabs);
         const int pLo = segLo * zr, pHi = std::min(segHi * zr, h->nz);
         StageParams<T> X = P;
         X.kOff = pLo;
         void* xargs[] = {&X};
-        HIPCHK(hipLaunchKernel(reinterpret_cast<const void*>(k.xy), dim3(P.nstrip * P.nseg * (pHi - pLo)),
-                               dim3(kWave), xargs, 0, h->stream));
-        HIPCHK(hipEventRecord(h->slabEv[s], h->stream));
-        HIPCHK(hipStreamWaitEvent(h->side, h->slabEv[s], 0));
+        e = hipLaunchKernel(reinterpret_cast<const void*>(k.xy), dim3(P.nstrip * P.nseg * (pHi - pLo)), dim3(kWave),
+                            xargs, 0, h->stream);
+        if (e == hipSuccess) e = hipEventRecord(h->slabEv[s], h->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(h->side, h->slabEv[s], 0);
         StageParams<T> Zs = Z;
         Zs.segOff = segLo;
         Zs.nseg = segHi - segLo;
@@ -482,10 +483,15 @@ int launch_stage_slabs(sifv_handle* h, const StageKernels<T>& k, const StagePara
         if (wantDt) Zs.dtPartial = h->partial + zoff;
         zoff += nbz;
         void* zargs[] = {&Zs};
-        HIPCHK(hipLaunchKernel(reinterpret_cast<const void*>(k.z), dim3(nbz), dim3(kWave), zargs, 0, h->side));
+        if (e == hipSuccess)
+            e = hipLaunchKernel(reinterpret_cast<const void*>(k.z), dim3(nbz), dim3(kWave), zargs, 0, h->side);
     }
-    HIPCHK(hipEventRecord(h->joinEv, h->side));
-    HIPCHK(hipStreamWaitEvent(h->stream, h->joinEv, 0));
+    // the handle's stream waits for every z launch already queued, also after a
+    // failed launch, so that nothing read through it races the second stream
+    const hipError_t ej = hipEventRecord(h->joinEv, h->side);
+    const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(h->stream, h->joinEv, 0) : ej;
+    HIPCHK(e);
+    HIPCHK(ew);
     if (wantDt) h->npartialValid = zoff;
     return SIFV_OK;
 }
