@@ -2074,6 +2074,7 @@ int sifv_create(const sifv_mesh* mesh, const sifv_config* cfg, int riemann, cons
 int sifv_destroy(sifv_handle* h) {
     if (!h) return SIFV_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->side) hipStreamSynchronize(h->side);
     drop_step_graph(h);
     drop_jac_graphs(h);
     for (auto& pr : h->timing.pending) {
@@ -2484,6 +2485,7 @@ int sifv_timing_read(sifv_handle* h, double* kernel_ms, long* launches, int rese
 int sifv_synchronize(sifv_handle* h) {
     if (!h) return fail(SIFV_ERR_INVALID, "null handle");
     HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->side) HIPCHK(hipStreamSynchronize(h->side));
     return SIFV_OK;
 }
 
