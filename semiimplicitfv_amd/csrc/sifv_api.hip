@@ -941,6 +941,13 @@ int launch_copy_interior(sifv_handle* h, const double* src, double* dst) {
 // predecessor converged returns at once, so a chunk may overshoot harmlessly.
 // *n = sweeps done (the reference's iteration count); iterate k lives in
 // jac[k % 2] and h->hostResid[0..n) holds the residual history.
+// Grid of the set-up and correction bricks (semi_brick): 64 columns x
+// kSemiBrickRows rows x kSemiBrickPlanes planes per block.
+dim3 semi_brick_grid(const sifv_handle* h) {
+    return dim3((h->nx + 63) / 64, (h->ny + kSemiBrickRows - 1) / kSemiBrickRows,
+                (h->nz + kSemiBrickPlanes - 1) / kSemiBrickPlanes);
+}
+
 // Grid of the tiled sweeps (jacobi_tile_kernel): 64-column x kTileRows-row tiles,
 // kTilePlanes planes per block in 3D.
 dim3 jacobi_tile_grid(const sifv_handle* h) {
@@ -1537,7 +1544,7 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
         S.resid = h->resid;
         S.maxIter = sp.maxPressureIters;
         S.dt = h->scal;
-        hipLaunchKernelGGL(semi_setup_kernel, dim3(nb), dim3(256), 0, h->stream, S);
+        hipLaunchKernelGGL(semi_setup_kernel, semi_brick_grid(h), dim3(64, kSemiBrickRows), 0, h->stream, S);
         HIPCHK(hipGetLastError());
         int n = 0;
         const double* pfinal = nullptr;
@@ -1571,7 +1578,7 @@ int enqueue_semi_step(sifv_handle* h, double targetDt) {
         K.pres = h->pres;
         K.storePrim = s == ord - 1 ? 1 : 0;
         K.dt = h->scal;
-        hipLaunchKernelGGL(semi_correct_kernel, dim3(nb), dim3(256), 0, h->stream, K);
+        hipLaunchKernelGGL(semi_correct_kernel, semi_brick_grid(h), dim3(64, kSemiBrickRows), 0, h->stream, K);
         HIPCHK(hipGetLastError());
     }
     h->cur = dst[ord - 1];

@@ -46,15 +46,47 @@ __device__ __forceinline__ void interior_ijk(const Grid& g, size_t t, int& i, in
     k = (int)(t / ((size_t)g.nx * g.ny));
 }
 
-__global__ void __launch_bounds__(256) semi_setup_kernel(SemiArgs A) {
+// Brick walk of the set-up and the correction: a block of 64 x kSemiBrickRows
+// threads owns 64 columns x kSemiBrickRows rows and marches kSemiBrickPlanes
+// planes of z, one cell per thread and plane, so that the y and z neighbours
+// a cell's stencil reads were just read by the same block (512^3 bubble:
+// set-up 1945 -> 1641 us, correction 3911 -> 3358 us against a grid-stride
+// walk over rows).  Blocks go to the 8 XCDs in contiguous runs, so
+// neighbouring bricks share an L2.  semi_brick gives the thread's column and
+// plane range; false when the column lies outside the mesh.
+constexpr int kSemiBrickRows = 4;
+constexpr int kSemiBrickPlanes = 8;
+
+__device__ __forceinline__ bool semi_brick(const Grid& g, int& i, int& j, int& k0, int& k1) {
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    const int gx = gridDim.x, gy = gridDim.y, nb = gx * gy * gridDim.z;
+    if (nb % 8 == 0) {
+        const int L = bx + gx * (by + gy * bz);
+        const int t = (L % 8) * (nb / 8) + L / 8;
+        bx = t % gx;
+        by = (t / gx) % gy;
+        bz = t / (gx * gy);
+    }
+    i = bx * 64 + (int)threadIdx.x;
+    j = by * kSemiBrickRows + (int)threadIdx.y;
+    k0 = bz * kSemiBrickPlanes;
+    k1 = min(g.nz, k0 + kSemiBrickPlanes);
+    return i < g.nx && j < g.ny;
+}
+
+__global__ void __launch_bounds__(64 * kSemiBrickRows) semi_setup_kernel(SemiArgs A) {
     const Grid& g = A.g;
-    const size_t n = (size_t)g.nx * g.ny * g.nz;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
     const double dt = *A.dt;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)A.maxIter; t += stride)
-        A.resid[t] = 0.0;
-    int i, j, k;
-    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
+    {
+        const size_t nthr = (size_t)blockDim.x * blockDim.y;
+        const size_t lin = ((size_t)blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z)) *
+                               nthr + threadIdx.y * blockDim.x + threadIdx.x;
+        const size_t stride = (size_t)gridDim.x * gridDim.y * gridDim.z * nthr;
+        for (size_t t = lin; t < (size_t)A.maxIter; t += stride) A.resid[t] = 0.0;
+    }
+    int i, j, k0, k1;
+    if (!semi_brick(g, i, j, k0, k1)) return;
+    for (int k = k0; k < k1; ++k) {
         const size_t id = cell_index(g, i, j, k);
         double rc2;
         if (A.rc2Given) {
@@ -208,16 +240,16 @@ struct CorrectArgs {
     const double* dt;
 };
 
-__global__ void __launch_bounds__(256) semi_correct_kernel(CorrectArgs A) {
+__global__ void __launch_bounds__(64 * kSemiBrickRows) semi_correct_kernel(CorrectArgs A) {
     const Grid& g = A.g;
-    const size_t n = (size_t)g.nx * g.ny * g.nz;
     const double dt = *A.dt;
     const int dim = A.dim;
-    int i, j, k;
     // sigma is zero without IGR (A.sigma null): p + sigma is p then
     const bool sig = A.sigma != nullptr;
     const size_t st[3] = {1, (size_t)g.nxT, (size_t)g.nxT * g.nyT};
-    for (InteriorIter it_(g.nx, g.ny, g.nz); it_.next(i, j, k);) {
+    int i, j, k0, k1;
+    if (!semi_brick(g, i, j, k0, k1)) return;
+    for (int k = k0; k < k1; ++k) {
         const size_t id = cell_index(g, i, j, k);
         const double pc = sig ? A.p[id] + A.sigma[id] : A.p[id];
         const double rho = A.cons[id];
