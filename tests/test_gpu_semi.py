@@ -140,6 +140,24 @@ def test_bubble_3d_golden(gpu, oracle_c):
     _check_floor(mesh, sg, sc, lambda fid: mesh.interior(d[f"f{fid}"]))
 
 
+def test_bubble_3d_golden_stage_slabs(gpu, oracle_c):
+    """The same water/air bubble with 3D stages in 3 slabs of z segments (12
+    planes in segments of 2): the reference's iteration counts and dt, the
+    oracle and reference fields at the rounding floor, and the same bits as
+    one launch of each pass."""
+    d = np.load(os.path.join(GOLDEN, "bubble3d_semi.npz"))
+    cfg, mesh, st, eos, rs = problems.bubble_3d(12, perturb=True)
+    sg, sc, hist = _pair(oracle_c, cfg, mesh, st, eos, rs, 3, rows=2, slabs=3)
+    for k, (dg, dc, ig, ic) in enumerate(hist):
+        assert ig == ic == d["iters"][k]
+        assert abs(dg - dc) <= 1e-15 * dc and abs(dg - d["dts"][k]) <= 1e-15 * d["dts"][k]
+    _check_floor(mesh, sg, sc, lambda fid: mesh.interior(d[f"f{fid}"]))
+    s1, _, hist1 = _pair(oracle_c, cfg, mesh, st, eos, rs, 3, rows=2, slabs=1)
+    assert [(h[0], h[2]) for h in hist] == [(h[0], h[2]) for h in hist1]
+    for fid in sc.field_ids():
+        assert np.array_equal(mesh.interior(sg.field(fid)), mesh.interior(s1.field(fid))), f"field {fid}"
+
+
 @pytest.mark.parametrize("tol,maxit", [(1e-6, 1000), (1e-300, 7), (1e-2, 1000)])
 def test_jacobi_standalone(gpu, oracle_c, tol, maxit):
     """sifv_jacobi_solve: iteration count, residual history and the whole
